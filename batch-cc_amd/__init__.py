@@ -1,0 +1,310 @@
+"""Python binding of the MI355X batch edge collision checker (include/bcc.h).
+
+The checker itself is native: gfx950 HIP kernels behind a C ABI in
+``libbcc.so`` next to this file.  This module is a thin ctypes layer for
+tests, the benchmark and Python callers.  It never falls back to a CPU
+path: if the library is missing, every call raises.
+
+Mirrors the reference's entry point ``batch_cc::batch_cc<Robot>(envs, edges,
+resolution, results)`` (src/batch_cc.hh:4-7): `check_dense(robot, scene,
+edges, resolution)` returns the edge-major verdict matrix results[edge, env]
+(True = the edge collides in that environment, src/batch_cc.cu:131).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbcc.so")
+
+ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2}
+DIMS = {"panda": 7, "fetch": 8, "baxter": 14}
+NUM_SPHERES = {"panda": 70, "fetch": 125, "baxter": 100}
+# packed host layout: floats per primitive of kind
+# spheres, capsules, z-capsules, cylinders, cuboids, z-cuboids
+KIND_STRIDE = (4, 8, 8, 8, 15, 15)
+KINDS = ("spheres", "capsules", "z_capsules", "cylinders", "cuboids", "z_cuboids")
+
+BCC_OK = 0
+
+_lib = None
+
+
+class BCCError(RuntimeError):
+    """Raised for any non-zero status of the C ABI (the reference throws
+    std::runtime_error for setup errors, src/batch_cc.cu:25-32)."""
+
+
+def _sig(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+    return f
+
+
+def lib():
+    """Load libbcc.so (built by batch-cc_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise BCCError(f"{LIB_PATH} is missing: run `python batch-cc_amd/build.py` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I = ctypes.c_int
+    _sig(L, "bcc_last_error", ctypes.c_char_p, [])
+    _sig(L, "bcc_robot_dimension", I, [I])
+    _sig(L, "bcc_robot_num_spheres", I, [I])
+    _sig(L, "bcc_device_count", I, [ctypes.POINTER(I)])
+    _sig(L, "bcc_scene_create", I, [P, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_scene_create_packed", I, [P, P, P, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_scene_destroy", I, [P])
+    _sig(L, "bcc_scene_num_envs", I, [P])
+    _sig(L, "bcc_check_dense", I, [I, P, P, I, I, P])
+    _sig(L, "bcc_check_dense_async", I, [I, P, P, I, I, P, P])
+    _sig(L, "bcc_time_dense", I, [I, P, P, I, I, P, I, I, P, ctypes.POINTER(ctypes.c_double)])
+    _sig(L, "bcc_work_dense", I, [I, P, P, I, I, P, P, P, P])
+    _sig(L, "bcc_malloc", I, [ctypes.POINTER(P), ctypes.c_size_t])
+    _sig(L, "bcc_free", I, [P])
+    _sig(L, "bcc_memcpy_htod", I, [P, P, ctypes.c_size_t])
+    _sig(L, "bcc_memcpy_dtoh", I, [P, P, ctypes.c_size_t])
+    _sig(L, "bcc_memset", I, [P, I, ctypes.c_size_t])
+    _sig(L, "bcc_synchronize", I, [])
+    _sig(L, "bcc_debug_fk", I, [I, P, I, P])
+    _sig(L, "bcc_debug_sincos", I, [P, I, P])
+    _sig(L, "bcc_debug_samples", I, [I, P, I, I, P, P, P])
+    _sig(L, "bcc_debug_prims", I, [P, I, P])
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != BCC_OK:
+        raise BCCError(f"bcc error {rc}: {lib().bcc_last_error().decode()}")
+
+
+def robot_id(robot) -> int:
+    if isinstance(robot, str):
+        return ROBOTS[robot]
+    return int(robot)
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ------------------------------------------------------------ host scenes
+
+@dataclass
+class PackedScene:
+    """Environments in the packed host layout of bcc_scene_create_packed:
+    env e's floats start at data[env_off[e]] and hold counts[e, k] records
+    of kind k (spheres, capsules, z-capsules, cylinders, cuboids, z-cuboids)
+    in that order, with 4, 8, 8, 8, 15, 15 floats each."""
+    data: np.ndarray      # float32 [total]
+    env_off: np.ndarray   # int64 [E]
+    counts: np.ndarray    # int32 [E, 6]
+
+    @property
+    def num_envs(self) -> int:
+        return int(self.counts.shape[0])
+
+    def env(self, e: int) -> dict:
+        out = {}
+        off = int(self.env_off[e])
+        for k, name in enumerate(KINDS):
+            n = int(self.counts[e, k])
+            out[name] = self.data[off:off + n * KIND_STRIDE[k]].reshape(n, KIND_STRIDE[k])
+            off += n * KIND_STRIDE[k]
+        return out
+
+    @staticmethod
+    def from_envs(envs: list[dict]) -> "PackedScene":
+        """envs: list of {kind: float32 array [n, stride]} (missing kinds = 0)."""
+        chunks = []
+        counts = np.zeros((len(envs), 6), np.int32)
+        offs = np.zeros(len(envs), np.int64)
+        pos = 0
+        for e, env in enumerate(envs):
+            offs[e] = pos
+            for k, name in enumerate(KINDS):
+                a = env.get(name)
+                if a is None or len(a) == 0:
+                    continue
+                a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1, KIND_STRIDE[k])
+                counts[e, k] = a.shape[0]
+                chunks.append(a.ravel())
+                pos += a.size
+        data = np.concatenate(chunks) if chunks else np.zeros(1, np.float32)
+        return PackedScene(np.ascontiguousarray(data, np.float32), offs, counts)
+
+
+class Scene:
+    """Device-resident environments (bcc_scene handle)."""
+
+    def __init__(self, packed: PackedScene):
+        self.packed = packed
+        self.handle = ctypes.c_void_p()
+        d = np.ascontiguousarray(packed.data, np.float32)
+        o = np.ascontiguousarray(packed.env_off, np.int64)
+        c = np.ascontiguousarray(packed.counts, np.int32)
+        _check(lib().bcc_scene_create_packed(_ptr(d), _ptr(o), _ptr(c), packed.num_envs, ctypes.byref(self.handle)))
+
+    @property
+    def num_envs(self) -> int:
+        return lib().bcc_scene_num_envs(self.handle)
+
+    def close(self):
+        if self.handle:
+            _check(lib().bcc_scene_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ device buffers
+
+class DeviceBuffer:
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        _check(lib().bcc_malloc(ctypes.byref(self.ptr), max(self.nbytes, 1)))
+
+    @staticmethod
+    def from_array(a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = DeviceBuffer(a.nbytes)
+        if a.nbytes:
+            _check(lib().bcc_memcpy_htod(b.ptr, _ptr(a), a.nbytes))
+        return b
+
+    def to_array(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes:
+            _check(lib().bcc_memcpy_dtoh(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def zero(self):
+        _check(lib().bcc_memset(self.ptr, 0, max(self.nbytes, 1)))
+
+    def free(self):
+        if self.ptr:
+            _check(lib().bcc_free(self.ptr))
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    rc = lib().bcc_device_count(ctypes.byref(n))
+    return n.value if rc == BCC_OK else 0
+
+
+def synchronize():
+    _check(lib().bcc_synchronize())
+
+
+# ------------------------------------------------------------ checks
+
+def _edges_array(robot, edges) -> np.ndarray:
+    dim = DIMS[robot] if isinstance(robot, str) else list(DIMS.values())[robot]
+    e = np.ascontiguousarray(edges, dtype=np.float32)
+    if e.ndim != 3 or e.shape[1:] != (2, dim):
+        raise BCCError(f"edges must be float32 [M, 2, {dim}], got {e.shape}")
+    return e
+
+
+def check_dense(robot, scene: Scene, edges, resolution: int = 32) -> np.ndarray:
+    """Verdicts [M, E] (uint8, 1 = collision) for every edge in every env."""
+    e = _edges_array(robot, edges)
+    M, E = e.shape[0], scene.num_envs
+    out = np.zeros((M, E), np.uint8)
+    _check(lib().bcc_check_dense(robot_id(robot), scene.handle, _ptr(e), M, int(resolution), _ptr(out)))
+    return out
+
+
+def check_dense_device(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, resolution: int,
+                       d_out: DeviceBuffer, stream=None):
+    _check(lib().bcc_check_dense_async(robot_id(robot), scene.handle, d_edges.ptr, int(num_edges), int(resolution),
+                                       d_out.ptr, stream))
+
+
+def time_dense(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, resolution: int, d_out: DeviceBuffer,
+               warmup: int, iters: int, stream=None) -> float:
+    """Average kernel milliseconds over `iters` launches (hipEvents)."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().bcc_time_dense(robot_id(robot), scene.handle, d_edges.ptr, int(num_edges), int(resolution),
+                                d_out.ptr, int(warmup), int(iters), stream, ctypes.byref(ms)))
+    return ms.value
+
+
+def work_dense(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, resolution: int = 32):
+    """Reference-order operation counts per pair: (work[M,E], trig[M,E], verdicts[M,E])."""
+    M, E = int(num_edges), scene.num_envs
+    dw = DeviceBuffer(M * E * 8)
+    dt = DeviceBuffer(M * E * 8)
+    do = DeviceBuffer(M * E)
+    _check(lib().bcc_work_dense(robot_id(robot), scene.handle, d_edges.ptr, M, int(resolution), dw.ptr, dt.ptr,
+                                do.ptr, None))
+    synchronize()
+    return (dw.to_array(np.uint64, (M, E)), dt.to_array(np.uint64, (M, E)), do.to_array(np.uint8, (M, E)))
+
+
+# ------------------------------------------------------------ test hooks
+
+def debug_fk(robot, q: np.ndarray) -> np.ndarray:
+    q = np.ascontiguousarray(q, np.float32)
+    n = q.shape[0]
+    ns = NUM_SPHERES[robot] if isinstance(robot, str) else list(NUM_SPHERES.values())[robot]
+    dq = DeviceBuffer.from_array(q)
+    do = DeviceBuffer(n * ns * 4 * 4)
+    _check(lib().bcc_debug_fk(robot_id(robot), dq.ptr, n, do.ptr))
+    return do.to_array(np.float32, (n, ns, 4))
+
+
+def debug_sincos(x: np.ndarray) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    dx = DeviceBuffer.from_array(x)
+    do = DeviceBuffer(x.size * 8)
+    _check(lib().bcc_debug_sincos(dx.ptr, x.size, do.ptr))
+    return do.to_array(np.float32, (x.size, 2))
+
+
+def debug_samples(robot, edges: np.ndarray, resolution: int = 32):
+    e = _edges_array(robot, edges)
+    m, dim = e.shape[0], e.shape[2]
+    de = DeviceBuffer.from_array(e)
+    dn = DeviceBuffer(m * 4)
+    dc = DeviceBuffer(m * 32 * dim * 4)
+    dl = DeviceBuffer(m * 32 * dim * 4)
+    _check(lib().bcc_debug_samples(robot_id(robot), de.ptr, m, int(resolution), dn.ptr, dc.ptr, dl.ptr))
+    return (dn.to_array(np.int32, (m,)), dc.to_array(np.float32, (m, 32, dim)),
+            dl.to_array(np.float32, (m, 32, dim)))
+
+
+def debug_prims(inp: np.ndarray) -> np.ndarray:
+    inp = np.ascontiguousarray(inp, np.float32)
+    assert inp.ndim == 2 and inp.shape[1] == 40
+    di = DeviceBuffer.from_array(inp)
+    do = DeviceBuffer(inp.shape[0] * 6 * 4)
+    _check(lib().bcc_debug_prims(di.ptr, inp.shape[0], do.ptr))
+    return do.to_array(np.float32, (inp.shape[0], 6))

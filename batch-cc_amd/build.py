@@ -1,0 +1,82 @@
+"""Build the gfx950 library libbcc.so in-tree, and the CPU oracle.
+
+    python batch-cc_amd/build.py            # regenerate robot code, compile, link
+    python batch-cc_amd/build.py --no-gen   # skip code generation
+
+Every HIP translation unit is compiled by hipcc for gfx950 only, with
+-ffp-contract=off (the arithmetic spec spells out every fused operation
+explicitly) and packed-FP32 instruction selection disabled.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+ROBOTS = ("panda", "fetch", "baxter")
+LIB = os.path.join(HERE, "libbcc.so")
+OBJDIR = os.path.join(HERE, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = [
+    "-O3", "--offload-arch=gfx950", "-fPIC", "-std=c++17",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+SOURCES = ["bcc_api.hip"] + [f"robot_{r}.hip" for r in ROBOTS]
+
+
+def run(cmd, **kw):
+    p = subprocess.run(cmd, capture_output=True, text=True, **kw)
+    out = "\n".join(line for line in (p.stdout + p.stderr).splitlines()
+                    if "packed-fp32-ops" not in line and "is not a recognized feature" not in line)
+    if p.returncode != 0:
+        raise RuntimeError(f"command failed ({p.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def generate():
+    gen = os.path.join(HERE, "robots", "gen_robot_code.py")
+    for r in ROBOTS:
+        run([sys.executable, gen, os.path.join(HERE, "robots", f"{r}.json"), os.path.join(CSRC, "robots")])
+
+
+def compile_one(src):
+    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+    run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], cwd=CSRC)
+    return obj
+
+
+def build_oracle():
+    run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
+
+
+def build(gen: bool = True, jobs: int = 4) -> str:
+    if gen:
+        generate()
+    os.makedirs(OBJDIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
+    tmp = LIB + ".tmp"
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    build_oracle()
+    return LIB
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--no-gen", action="store_true")
+    ap.add_argument("-j", type=int, default=4)
+    a = ap.parse_args()
+    print(build(gen=not a.no_gen, jobs=a.j))
+
+
+if __name__ == "__main__":
+    main()

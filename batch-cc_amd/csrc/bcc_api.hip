@@ -1,0 +1,669 @@
+// Host side of the C ABI (include/bcc.h): scene upload, argument checks,
+// launch dispatch per robot, timing, and the arithmetic test hooks.
+//
+// Replaces the host orchestration of src/batch_cc.cu:135-338.  Differences
+// by design: environments are packed once into one device buffer per scene
+// (one allocation instead of two per environment, src/batch_cc.cu:146-175),
+// errors are returned as status codes with a thread-local message instead of
+// exceptions/exit(), and nothing is printed.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "../../include/bcc.h"
+#include "bcc_dense.h"
+#include "robots/baxter_model.h"
+#include "robots/fetch_model.h"
+#include "robots/panda_model.h"
+
+namespace {
+
+thread_local std::string g_err = "no error";
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(BCC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BCC_HIP(call, what)                       \
+    do {                                          \
+        hipError_t _e = (call);                   \
+        if (_e != hipSuccess)                     \
+            return hip_fail(_e, what);            \
+    } while (0)
+
+constexpr int kStride[6] = {4, 8, 8, 8, 15, 15};  // packed host layout
+
+}  // namespace
+
+struct bcc_scene {
+    int num_envs = 0;
+    float* d_obs = nullptr;      // all environments, device layout (see bcc_device.h)
+    int* d_meta = nullptr;       // [num_envs][kEnvMetaInts]
+    float* d_ref_obs = nullptr;  // the reference's lists and order, for the work counter
+    int* d_ref_meta = nullptr;
+    size_t obs_floats = 0;
+};
+
+namespace {
+
+// Pack host environments into the device layout of bcc_device.h: per env
+// six lists (spheres[4], capsules[8], z-capsules[8], cuboids[16],
+// z-cuboids[16], aabbs[8]), each starting on a 16-float boundary.
+// Primitives go to the list of their kind, except that capsules with
+// xv == yv == 0 join the z-capsules and cuboids with exactly identity axes
+// (z-cuboids with axes (1,0),(0,1)) become aabbs: for those inputs the
+// special-case formulas are bit-identical to the general ones.  Cylinders
+// are dropped: the reference never tests them (src/utils.cuh:213-245).
+int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene** out)
+{
+    constexpr int kLists = 6;
+    const int stride[kLists] = {4, 8, 8, 16, 16, 8};
+    std::vector<std::vector<float>> lists((size_t)num_envs * kLists);
+    for (int e = 0; e < num_envs; ++e) {
+        const bcc_env_desc& d = envs[e];
+        std::vector<float>* L = &lists[(size_t)e * kLists];
+        for (uint32_t i = 0; i < d.num_spheres; ++i)
+            L[0].insert(L[0].end(), d.spheres + 4 * (size_t)i, d.spheres + 4 * (size_t)i + 4);
+        for (uint32_t i = 0; i < d.num_capsules; ++i) {
+            const float* c = d.capsules + 8 * (size_t)i;
+            std::vector<float>& dst = (c[3] == 0.0f && c[4] == 0.0f) ? L[2] : L[1];
+            dst.insert(dst.end(), c, c + 8);
+        }
+        for (uint32_t i = 0; i < d.num_z_capsules; ++i)
+            L[2].insert(L[2].end(), d.z_capsules + 8 * (size_t)i, d.z_capsules + 8 * (size_t)i + 8);
+        for (uint32_t i = 0; i < d.num_cuboids; ++i) {
+            const float* b = d.cuboids + 15 * (size_t)i;
+            const bool ident = b[3] == 1.0f && b[4] == 0.0f && b[5] == 0.0f && b[6] == 0.0f && b[7] == 1.0f &&
+                               b[8] == 0.0f && b[9] == 0.0f && b[10] == 0.0f && b[11] == 1.0f;
+            if (ident) {
+                const float a[8] = {b[0], b[1], b[2], b[12], b[13], b[14], 0.0f, 0.0f};
+                L[5].insert(L[5].end(), a, a + 8);
+            } else {
+                L[3].insert(L[3].end(), b, b + 15);
+                L[3].push_back(0.0f);
+            }
+        }
+        for (uint32_t i = 0; i < d.num_z_cuboids; ++i) {
+            const float* b = d.z_cuboids + 15 * (size_t)i;
+            const bool ident = b[3] == 1.0f && b[4] == 0.0f && b[6] == 0.0f && b[7] == 1.0f;
+            if (ident) {
+                const float a[8] = {b[0], b[1], b[2], b[12], b[13], b[14], 0.0f, 0.0f};
+                L[5].insert(L[5].end(), a, a + 8);
+            } else {
+                L[4].insert(L[4].end(), b, b + 15);
+                L[4].push_back(0.0f);
+            }
+        }
+    }
+    std::vector<int> meta((size_t)num_envs * bcc::kEnvMetaInts, 0);
+    size_t total = 0;
+    auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+    for (int e = 0; e < num_envs; ++e) {
+        for (int k = 0; k < kLists; ++k) {
+            total = align16(total);
+            if (total > 0x7fffffffULL)
+                return fail(BCC_ERR_INVALID, "scene too large (more than 2^31 floats)");
+            const std::vector<float>& l = lists[(size_t)e * kLists + k];
+            meta[(size_t)e * bcc::kEnvMetaInts + 2 * k] = (int)total;
+            meta[(size_t)e * bcc::kEnvMetaInts + 2 * k + 1] = (int)(l.size() / stride[k]);
+            total += l.size();
+        }
+    }
+    total = align16(total) + 16;
+    std::vector<float> host(total, 0.0f);
+    for (int e = 0; e < num_envs; ++e)
+        for (int k = 0; k < kLists; ++k) {
+            const std::vector<float>& l = lists[(size_t)e * kLists + k];
+            if (!l.empty())
+                std::memcpy(&host[meta[(size_t)e * bcc::kEnvMetaInts + 2 * k]], l.data(), sizeof(float) * l.size());
+        }
+    // reference layout: the five tested lists in the reference's order
+    std::vector<int> rmeta((size_t)num_envs * bcc::kEnvMetaInts, 0);
+    std::vector<float> rhost;
+    for (int e = 0; e < num_envs; ++e) {
+        const bcc_env_desc& d = envs[e];
+        const float* src[5] = {d.spheres, d.capsules, d.z_capsules, d.cuboids, d.z_cuboids};
+        const uint32_t cnt[5] = {d.num_spheres, d.num_capsules, d.num_z_capsules, d.num_cuboids, d.num_z_cuboids};
+        const int in_stride[5] = {4, 8, 8, 15, 15};
+        const int out_stride[5] = {4, 8, 8, 16, 16};
+        for (int k = 0; k < 5; ++k) {
+            rhost.resize(align16(rhost.size()), 0.0f);
+            if (rhost.size() > 0x7fffffffULL)
+                return fail(BCC_ERR_INVALID, "scene too large (more than 2^31 floats)");
+            rmeta[(size_t)e * bcc::kEnvMetaInts + 2 * k] = (int)rhost.size();
+            rmeta[(size_t)e * bcc::kEnvMetaInts + 2 * k + 1] = (int)cnt[k];
+            for (uint32_t i = 0; i < cnt[k]; ++i) {
+                rhost.insert(rhost.end(), src[k] + (size_t)in_stride[k] * i, src[k] + (size_t)in_stride[k] * (i + 1));
+                rhost.resize(rhost.size() + (out_stride[k] - in_stride[k]), 0.0f);
+            }
+        }
+    }
+    rhost.resize(align16(rhost.size()) + 16, 0.0f);
+
+    bcc_scene* s = new bcc_scene;
+    s->num_envs = num_envs;
+    s->obs_floats = total;
+    const size_t meta_bytes = sizeof(int) * std::max<size_t>(meta.size(), 1);
+    hipError_t err = hipMalloc((void**)&s->d_obs, sizeof(float) * total);
+    if (err == hipSuccess)
+        err = hipMalloc((void**)&s->d_meta, meta_bytes);
+    if (err == hipSuccess)
+        err = hipMalloc((void**)&s->d_ref_obs, sizeof(float) * rhost.size());
+    if (err == hipSuccess)
+        err = hipMalloc((void**)&s->d_ref_meta, meta_bytes);
+    if (err == hipSuccess)
+        err = hipMemcpy(s->d_obs, host.data(), sizeof(float) * total, hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(s->d_ref_obs, rhost.data(), sizeof(float) * rhost.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess && !meta.empty())
+        err = hipMemcpy(s->d_meta, meta.data(), sizeof(int) * meta.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess && !rmeta.empty())
+        err = hipMemcpy(s->d_ref_meta, rmeta.data(), sizeof(int) * rmeta.size(), hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        (void)hipFree(s->d_obs);
+        (void)hipFree(s->d_meta);
+        (void)hipFree(s->d_ref_obs);
+        (void)hipFree(s->d_ref_meta);
+        delete s;
+        return hip_fail(err, "scene upload");
+    }
+    *out = s;
+    return BCC_OK;
+}
+
+int check_env_desc(const bcc_env_desc& d, int e)
+{
+    const bool ok = (d.num_spheres == 0 || d.spheres) && (d.num_capsules == 0 || d.capsules) &&
+                    (d.num_z_capsules == 0 || d.z_capsules) && (d.num_cylinders == 0 || d.cylinders) &&
+                    (d.num_cuboids == 0 || d.cuboids) && (d.num_z_cuboids == 0 || d.z_cuboids);
+    if (!ok)
+        return fail(BCC_ERR_INVALID,
+                    "Host environment has positive counts with null pointers (environment " + std::to_string(e) + ")");
+    return BCC_OK;
+}
+
+int dispatch_dense(int robot, const bcc::DenseArgs& a, hipStream_t s)
+{
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_dense_panda(a, s); break;
+    case BCC_FETCH: e = bcc::launch_dense_fetch(a, s); break;
+    case BCC_BAXTER: e = bcc::launch_dense_baxter(a, s); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "dense kernel launch");
+    return BCC_OK;
+}
+
+int check_dense_args(int robot, const bcc_scene* scene, const void* edges, int num_edges, int resolution,
+                     const void* results)
+{
+    if (bcc_robot_dimension(robot) < 0)
+        return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
+    if (!scene)
+        return fail(BCC_ERR_INVALID, "scene is NULL");
+    if (num_edges < 0)
+        return fail(BCC_ERR_INVALID, "num_edges < 0");
+    if (resolution < 1 || resolution > (1 << 20))
+        return fail(BCC_ERR_INVALID, "resolution must be in [1, 2^20]");
+    if (num_edges > 0 && scene->num_envs > 0 && (!edges || !results))
+        return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
+    return BCC_OK;
+}
+
+// ------------------------------------------------------------ test-hook kernels
+
+__global__ void sincos_kernel(const float* __restrict__ x, int n, float* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float s, c;
+    bcc::bcc_sincosf(x[i], s, c);
+    out[2 * i] = s;
+    out[2 * i + 1] = c;
+}
+
+// in[40]: x y z rp | sphere (4) | capsule (8) | cuboid (15 + pad) | aabb (8);
+// out[6]: sphere, capsule, z-capsule, cuboid, z-cuboid and aabb test values.
+// Every record is read from the global input buffer.
+__global__ void prims_kernel(const float* __restrict__ in, int n, float* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float* p = in + (size_t)i * 40;
+    const float x = p[0], y = p[1], z = p[2], rp = p[3];
+    const bcc::cfptr cp = (bcc::cfptr)(p + 8);
+    const bcc::cfptr bp = (bcc::cfptr)(p + 16);
+    const bcc::cfptr ap = (bcc::cfptr)(p + 32);
+    float* o = out + (size_t)i * 6;
+    o[0] = bcc::bcc_sphere_val(p[4], p[5], p[6], p[7], x, y, z, rp);
+    o[1] = bcc::bcc_capsule_val(cp, x, y, z, rp);
+    o[2] = bcc::bcc_zcapsule_val(cp, x, y, z, rp);
+    o[3] = bcc::bcc_cuboid_val(bp, x, y, z, rp * rp);
+    o[4] = bcc::bcc_zcuboid_val(bp, x, y, z, rp * rp);
+    o[5] = bcc::bcc_aabb_val(ap, x, y, z, rp * rp);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* bcc_last_error(void)
+{
+    return g_err.c_str();
+}
+
+int bcc_robot_dimension(int robot)
+{
+    switch (robot) {
+    case BCC_PANDA: return 7;
+    case BCC_FETCH: return 8;
+    case BCC_BAXTER: return 14;
+    default: return -1;
+    }
+}
+
+int bcc_robot_num_spheres(int robot)
+{
+    switch (robot) {
+    case BCC_PANDA: return 70;
+    case BCC_FETCH: return 125;
+    case BCC_BAXTER: return 100;
+    default: return -1;
+    }
+}
+
+int bcc_device_count(int* count)
+{
+    if (!count)
+        return fail(BCC_ERR_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return hip_fail(e, "hipGetDeviceCount");
+    }
+    *count = n;
+    return BCC_OK;
+}
+
+int bcc_scene_create(const bcc_env_desc* envs, int num_envs, bcc_scene** out)
+{
+    if (!out)
+        return fail(BCC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (num_envs < 0 || (num_envs > 0 && !envs))
+        return fail(BCC_ERR_INVALID, "bad environment array");
+    std::vector<bcc_env_desc> v(envs, envs + num_envs);
+    for (int e = 0; e < num_envs; ++e)
+        if (int rc = check_env_desc(v[e], e))
+            return rc;
+    return build_scene(num_envs, v, out);
+}
+
+int bcc_scene_create_packed(const float* data, const int64_t* env_offsets, const int32_t* counts, int num_envs,
+                            bcc_scene** out)
+{
+    if (!out)
+        return fail(BCC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (num_envs < 0 || (num_envs > 0 && (!env_offsets || !counts)))
+        return fail(BCC_ERR_INVALID, "bad packed scene arrays");
+    std::vector<bcc_env_desc> v((size_t)num_envs);
+    for (int e = 0; e < num_envs; ++e) {
+        const float* base = data ? data + env_offsets[e] : nullptr;
+        const float* ptr[6];
+        for (int k = 0; k < 6; ++k) {
+            if (counts[6 * e + k] < 0)
+                return fail(BCC_ERR_INVALID, "negative primitive count in environment " + std::to_string(e));
+            ptr[k] = (counts[6 * e + k] > 0) ? base : nullptr;
+            if (counts[6 * e + k] > 0 && !base)
+                return fail(BCC_ERR_INVALID, "Host environment has positive counts with null pointers");
+            if (base)
+                base += (size_t)counts[6 * e + k] * kStride[k];
+        }
+        v[e] = bcc_env_desc{ptr[0], (uint32_t)counts[6 * e + 0], ptr[1], (uint32_t)counts[6 * e + 1],
+                            ptr[2], (uint32_t)counts[6 * e + 2], ptr[3], (uint32_t)counts[6 * e + 3],
+                            ptr[4], (uint32_t)counts[6 * e + 4], ptr[5], (uint32_t)counts[6 * e + 5]};
+    }
+    return build_scene(num_envs, v, out);
+}
+
+int bcc_scene_destroy(bcc_scene* scene)
+{
+    if (!scene)
+        return BCC_OK;
+    hipError_t a = hipFree(scene->d_obs);
+    hipError_t b = hipFree(scene->d_meta);
+    hipError_t c = hipFree(scene->d_ref_obs);
+    hipError_t d = hipFree(scene->d_ref_meta);
+    delete scene;
+    if (a == hipSuccess)
+        a = c;
+    if (b == hipSuccess)
+        b = d;
+    if (a != hipSuccess)
+        return hip_fail(a, "hipFree(scene)");
+    if (b != hipSuccess)
+        return hip_fail(b, "hipFree(scene meta)");
+    return BCC_OK;
+}
+
+int bcc_scene_num_envs(const bcc_scene* scene)
+{
+    return scene ? scene->num_envs : -1;
+}
+
+int bcc_check_dense_async(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
+                          uint8_t* d_results, void* stream)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
+    return dispatch_dense(robot, a, (hipStream_t)stream);
+}
+
+int bcc_check_dense(int robot, const bcc_scene* scene, const float* edges, int num_edges, int resolution,
+                    uint8_t* results)
+{
+    if (int rc = check_dense_args(robot, scene, edges, num_edges, resolution, results))
+        return rc;
+    const size_t pairs = (size_t)num_edges * (size_t)scene->num_envs;
+    if (pairs == 0)
+        return BCC_OK;
+    const int dim = bcc_robot_dimension(robot);
+    for (size_t i = 0; i < (size_t)num_edges * 2 * dim; ++i)
+        if (!std::isfinite(edges[i]))
+            return fail(BCC_ERR_INVALID, "edge " + std::to_string(i / (2 * dim)) + " has a non-finite coordinate");
+    float* d_edges = nullptr;
+    uint8_t* d_out = nullptr;
+    const size_t edge_bytes = sizeof(float) * 2 * dim * (size_t)num_edges;
+    hipError_t e = hipMalloc((void**)&d_edges, edge_bytes);
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&d_out, pairs);
+    if (e == hipSuccess)
+        e = hipMemcpy(d_edges, edges, edge_bytes, hipMemcpyHostToDevice);
+    int rc = BCC_OK;
+    if (e != hipSuccess) {
+        rc = hip_fail(e, "dense check setup");
+    } else {
+        bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_out};
+        rc = dispatch_dense(robot, a, nullptr);
+        if (rc == BCC_OK) {
+            e = hipMemcpy(results, d_out, pairs, hipMemcpyDeviceToHost);
+            if (e != hipSuccess)
+                rc = hip_fail(e, "dense check results");
+        }
+    }
+    (void)hipFree(d_edges);
+    (void)hipFree(d_out);
+    return rc;
+}
+
+int bcc_time_dense(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
+                   uint8_t* d_results, int warmup, int iters, void* stream, double* avg_ms)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    if (!avg_ms || iters < 1 || warmup < 0)
+        return fail(BCC_ERR_INVALID, "bad timing arguments");
+    hipStream_t s = (hipStream_t)stream;
+    bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
+    for (int i = 0; i < warmup; ++i)
+        if (int rc = dispatch_dense(robot, a, s))
+            return rc;
+    hipEvent_t t0, t1;
+    BCC_HIP(hipEventCreate(&t0), "hipEventCreate");
+    BCC_HIP(hipEventCreate(&t1), "hipEventCreate");
+    int rc = BCC_OK;
+    hipError_t e = hipEventRecord(t0, s);
+    for (int i = 0; i < iters && e == hipSuccess && rc == BCC_OK; ++i)
+        rc = dispatch_dense(robot, a, s);
+    if (e == hipSuccess)
+        e = hipEventRecord(t1, s);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(t1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc != BCC_OK)
+        return rc;
+    if (e != hipSuccess)
+        return hip_fail(e, "timing events");
+    *avg_ms = (double)ms / iters;
+    return BCC_OK;
+}
+
+namespace {
+
+const bcc_robot_model* host_model(int robot)
+{
+    switch (robot) {
+    case BCC_PANDA: return &bcc_panda_model;
+    case BCC_FETCH: return &bcc_fetch_model;
+    case BCC_BAXTER: return &bcc_baxter_model;
+    default: return nullptr;
+    }
+}
+
+// Program tables of a robot on the current device (uploaded on first use and
+// kept for the life of the process).
+int device_model(int robot, bcc::ModelDev* out)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, bcc::ModelDev> cache;
+    int dev = 0;
+    BCC_HIP(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({dev, robot});
+    if (it != cache.end()) {
+        *out = it->second;
+        return BCC_OK;
+    }
+    const bcc_robot_model* m = host_model(robot);
+    if (!m)
+        return fail(BCC_ERR_INVALID, "unknown robot id");
+    const int ne = m->num_env_groups, nsg = m->num_self_groups, np = m->num_program;
+    std::vector<int> ints;
+    std::vector<size_t> off;
+    auto put = [&](const int* a, int n) {
+        off.push_back(ints.size());
+        ints.insert(ints.end(), a, a + n);
+    };
+    put(m->env_gate, ne);
+    put(m->env_lone, ne);
+    put(m->env_fine_off, ne + 1);
+    put(m->env_fine, std::max(m->env_fine_off[ne], 1));
+    put(m->self_gate_a, nsg);
+    put(m->self_gate_b, nsg);
+    put(m->self_lone, nsg);
+    put(m->self_a_off, nsg + 1);
+    put(m->self_a, std::max(m->self_a_off[nsg], 1));
+    put(m->self_b_off, nsg + 1);
+    put(m->self_b, std::max(m->self_b_off[nsg], 1));
+    put(m->prog_kind, np);
+    put(m->prog_index, np);
+    put(m->prog_fk_ops, np);
+    put(m->prog_fk_trig, np);
+    int* d_ints = nullptr;
+    float* d_r = nullptr;
+    BCC_HIP(hipMalloc((void**)&d_ints, sizeof(int) * ints.size()), "model upload");
+    BCC_HIP(hipMalloc((void**)&d_r, sizeof(float) * m->num_spheres), "model upload");
+    BCC_HIP(hipMemcpy(d_ints, ints.data(), sizeof(int) * ints.size(), hipMemcpyHostToDevice), "model upload");
+    BCC_HIP(hipMemcpy(d_r, m->sphere_r, sizeof(float) * m->num_spheres, hipMemcpyHostToDevice), "model upload");
+    bcc::ModelDev d;
+    d.sphere_r = d_r;
+    const int* b = d_ints;
+    d.env_gate = b + off[0];
+    d.env_lone = b + off[1];
+    d.env_fine_off = b + off[2];
+    d.env_fine = b + off[3];
+    d.self_gate_a = b + off[4];
+    d.self_gate_b = b + off[5];
+    d.self_lone = b + off[6];
+    d.self_a_off = b + off[7];
+    d.self_a = b + off[8];
+    d.self_b_off = b + off[9];
+    d.self_b = b + off[10];
+    d.prog_kind = b + off[11];
+    d.prog_index = b + off[12];
+    d.prog_fk_ops = b + off[13];
+    d.prog_fk_trig = b + off[14];
+    d.num_program = np;
+    d.ref_fk_flops = m->ref_fk_flops;
+    d.ref_fk_trig = m->ref_fk_trig;
+    cache[{dev, robot}] = d;
+    *out = d;
+    return BCC_OK;
+}
+
+}  // namespace
+
+int bcc_work_dense(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
+                   uint64_t* d_work, uint64_t* d_trig, uint8_t* d_results, void* stream)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    const size_t pairs = (size_t)num_edges * (size_t)scene->num_envs;
+    if (pairs == 0)
+        return BCC_OK;
+    if (!d_work || !d_trig)
+        return fail(BCC_ERR_INVALID, "work/trig buffer is NULL");
+    hipStream_t s = (hipStream_t)stream;
+    BCC_HIP(hipMemsetAsync(d_work, 0, sizeof(uint64_t) * pairs, s), "work reset");
+    BCC_HIP(hipMemsetAsync(d_trig, 0, sizeof(uint64_t) * pairs, s), "work reset");
+    bcc::ModelDev md;
+    if (int rc = device_model(robot, &md))
+        return rc;
+    bcc::WorkArgs a{scene->d_ref_obs, scene->d_ref_meta, d_edges, num_edges, scene->num_envs, resolution, md,
+                    (unsigned long long*)d_work, (unsigned long long*)d_trig, d_results};
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_work_panda(a, s); break;
+    case BCC_FETCH: e = bcc::launch_work_fetch(a, s); break;
+    case BCC_BAXTER: e = bcc::launch_work_baxter(a, s); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id");
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "work kernel launch");
+    return BCC_OK;
+}
+
+int bcc_malloc(void** d_ptr, size_t bytes)
+{
+    if (!d_ptr)
+        return fail(BCC_ERR_INVALID, "d_ptr is NULL");
+    BCC_HIP(hipMalloc(d_ptr, bytes ? bytes : 1), "hipMalloc");
+    return BCC_OK;
+}
+
+int bcc_free(void* d_ptr)
+{
+    BCC_HIP(hipFree(d_ptr), "hipFree");
+    return BCC_OK;
+}
+
+int bcc_memcpy_htod(void* d_dst, const void* src, size_t bytes)
+{
+    BCC_HIP(hipMemcpy(d_dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return BCC_OK;
+}
+
+int bcc_memcpy_dtoh(void* dst, const void* d_src, size_t bytes)
+{
+    BCC_HIP(hipMemcpy(dst, d_src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return BCC_OK;
+}
+
+int bcc_memset(void* d_dst, int value, size_t bytes)
+{
+    BCC_HIP(hipMemset(d_dst, value, bytes), "hipMemset");
+    return BCC_OK;
+}
+
+int bcc_synchronize(void)
+{
+    BCC_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    return BCC_OK;
+}
+
+int bcc_debug_fk(int robot, const float* d_q, int n, float* d_out)
+{
+    if (n < 0 || (n > 0 && (!d_q || !d_out)))
+        return fail(BCC_ERR_INVALID, "bad fk arguments");
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_fk_panda(d_q, n, d_out, nullptr); break;
+    case BCC_FETCH: e = bcc::launch_fk_fetch(d_q, n, d_out, nullptr); break;
+    case BCC_BAXTER: e = bcc::launch_fk_baxter(d_q, n, d_out, nullptr); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id");
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "fk kernel");
+    BCC_HIP(hipDeviceSynchronize(), "fk kernel");
+    return BCC_OK;
+}
+
+int bcc_debug_sincos(const float* d_x, int n, float* d_out)
+{
+    if (n < 0 || (n > 0 && (!d_x || !d_out)))
+        return fail(BCC_ERR_INVALID, "bad sincos arguments");
+    if (n == 0)
+        return BCC_OK;
+    hipLaunchKernelGGL(sincos_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, d_x, n, d_out);
+    BCC_HIP(hipGetLastError(), "sincos kernel");
+    BCC_HIP(hipDeviceSynchronize(), "sincos kernel");
+    return BCC_OK;
+}
+
+int bcc_debug_samples(int robot, const float* d_edges, int m, int resolution, int32_t* d_n, float* d_cfg,
+                      float* d_last)
+{
+    if (m < 0 || resolution < 1 || (m > 0 && (!d_edges || !d_n || !d_cfg || !d_last)))
+        return fail(BCC_ERR_INVALID, "bad samples arguments");
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_samples_panda(d_edges, m, resolution, d_n, d_cfg, d_last, nullptr); break;
+    case BCC_FETCH: e = bcc::launch_samples_fetch(d_edges, m, resolution, d_n, d_cfg, d_last, nullptr); break;
+    case BCC_BAXTER: e = bcc::launch_samples_baxter(d_edges, m, resolution, d_n, d_cfg, d_last, nullptr); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id");
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "samples kernel");
+    BCC_HIP(hipDeviceSynchronize(), "samples kernel");
+    return BCC_OK;
+}
+
+int bcc_debug_prims(const float* d_in, int n, float* d_out)
+{
+    if (n < 0 || (n > 0 && (!d_in || !d_out)))
+        return fail(BCC_ERR_INVALID, "bad prims arguments");
+    if (n == 0)
+        return BCC_OK;
+    hipLaunchKernelGGL(prims_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, d_in, n, d_out);
+    BCC_HIP(hipGetLastError(), "prims kernel");
+    BCC_HIP(hipDeviceSynchronize(), "prims kernel");
+    return BCC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,388 @@
+// Dense plain-order kernel: every edge against every environment.
+//
+// Replaces batch_cc_kernel<Robot> (src/batch_cc.cu:74-133), which runs one
+// 32-lane block per (env, edge) pair.  Here a 256-thread workgroup owns one
+// environment and a chunk of edges; its 8 half-waves pull edges from the
+// chunk through an LDS counter, so a half-wave whose pair ends early (first
+// collision, or a short edge) immediately starts the next pair instead of
+// idling.  Results are written edge-major, results[edge * E + env].
+#pragma once
+
+#include <algorithm>
+
+#include "bcc_device.h"
+
+namespace bcc {
+
+constexpr int kBlock = 256;
+
+// Step count and per-joint increment of one edge, then the configuration of
+// lane t at step 0 (src/batch_cc.cu:103-116):
+//   dist  = sqrt(sum (s-e)^2)        (fmaf accumulation)
+//   n     = max(ceil(dist / 31 * resolution), 1), at most 2^22
+//   delta = (e - s) / (31 n)
+//   q     = fmaf(delta, t*n, s)
+template <int D>
+__device__ __forceinline__ int setup_pair(const float* __restrict__ se, int resolution, int t, float (&q)[D],
+                                          float (&d)[D])
+{
+    float s[D], e[D];
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        s[j] = se[j];
+        e[j] = se[D + j];
+        const float diff = s[j] - e[j];
+        acc = fmaf(diff, diff, acc);
+    }
+    const float dist = sqrtf(acc);
+    // n is capped at 2^22 steps: never reached by a finite edge, it keeps a
+    // non-finite input from spinning a half-wave for 2^31 steps
+    const float nf = fminf(fmaxf(ceilf((dist / 31.0f) * (float)resolution), 1.0f), 4194304.0f);
+    const int n = (int)nf;
+    const float div = (float)(31 * n);
+    const float tn = (float)(t * n);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        d[j] = (e[j] - s[j]) / div;
+        q[j] = fmaf(d[j], tn, s[j]);
+    }
+    return n;
+}
+
+template <class Robot>
+__global__ __launch_bounds__(kBlock) void dense_kernel(const float* __restrict__ obs, const int* __restrict__ meta,
+                                                       const float* __restrict__ edges, int num_edges, int num_envs,
+                                                       int resolution, int chunk, int chunks_per_env,
+                                                       uint8_t* __restrict__ out)
+{
+    constexpr int D = Robot::dim;
+    const int env = blockIdx.x / chunks_per_env;
+    const int ck = blockIdx.x - env * chunks_per_env;
+    const int e_begin = ck * chunk;
+    const int e_end = min(num_edges, e_begin + chunk);
+
+    __shared__ int next_edge;
+    if (threadIdx.x == 0)
+        next_edge = e_begin;
+    __syncthreads();
+
+    const EnvView E = load_env(obs, meta, env);
+    const int lane = threadIdx.x & 63;
+    const int t = lane & 31;
+    const int half0 = lane & 32;
+
+    float q[D], d[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        q[j] = 0.0f;
+        d[j] = 0.0f;
+    }
+    int cur = -1, step = 0, n = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        const bool want = (cur < 0) && !exhausted;
+        int g = 0;
+        if (want && t == 0)
+            g = atomicAdd(&next_edge, 1);
+        g = __shfl(g, half0);
+        if (want) {
+            if (g < e_end) {
+                cur = g;
+                step = 0;
+                n = setup_pair<D>(edges + (size_t)g * 2 * D, resolution, t, q, d);
+            } else {
+                exhausted = true;
+            }
+        }
+        const uint64_t live = __ballot(cur >= 0);
+        if (live == 0)
+            break;
+        const uint64_t hit = Robot::check(q, E, live);
+        if (cur >= 0) {
+            const bool h = (hit >> lane) & 1ull;
+            bool fin = h;
+            if (!h) {
+                ++step;
+                fin = (step == n);
+                if (!fin) {
+#pragma unroll
+                    for (int j = 0; j < D; ++j)
+                        q[j] = q[j] + d[j];
+                }
+            }
+            if (fin) {
+                if (t == 0)
+                    out[(size_t)cur * num_envs + env] = h ? 1 : 0;
+                cur = -1;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ test hooks
+
+template <class Robot>
+__global__ void fk_kernel(const float* __restrict__ qs, int n, float* __restrict__ out)
+{
+    constexpr int D = Robot::dim;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float q[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        q[j] = qs[(size_t)i * D + j];
+    Robot::spheres(q, out + (size_t)i * Robot::num_spheres * 4);
+}
+
+template <class Robot>
+__global__ void samples_kernel(const float* __restrict__ edges, int m, int resolution, int* __restrict__ n_out,
+                               float* __restrict__ cfg0, float* __restrict__ cfg_last)
+{
+    constexpr int D = Robot::dim;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int edge = gid >> 5;
+    const int t = gid & 31;
+    if (edge >= m)
+        return;
+    float q[D], d[D];
+    const int n = setup_pair<D>(edges + (size_t)edge * 2 * D, resolution, t, q, d);
+    if (t == 0)
+        n_out[edge] = n;
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        cfg0[((size_t)edge * 32 + t) * D + j] = q[j];
+    for (int i = 1; i < n; ++i) {
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            q[j] = q[j] + d[j];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        cfg_last[((size_t)edge * 32 + t) * D + j] = q[j];
+}
+
+// Device copy of a robot's check-program tables (robots/bcc_model.h),
+// uploaded once per device; used only by the work counter.
+struct ModelDev {
+    const float* sphere_r;
+    const int* env_gate;
+    const int* env_lone;
+    const int* env_fine_off;
+    const int* env_fine;
+    const int* self_gate_a;
+    const int* self_gate_b;
+    const int* self_lone;
+    const int* self_a_off;
+    const int* self_a;
+    const int* self_b_off;
+    const int* self_b;
+    const int* prog_kind;
+    const int* prog_index;
+    const int* prog_fk_ops;
+    const int* prog_fk_trig;
+    int num_program;
+    int ref_fk_flops;
+    int ref_fk_trig;
+};
+
+// One sample replayed in the reference's program order with its early
+// returns (fkcc<Robot>, src/utils.cuh:342-5442 and the other robots): the
+// generated FK fills every sphere centre, the program tables drive the tests.
+template <class Robot>
+__device__ __forceinline__ bool ref_sample_work(const float (&q)[Robot::dim], const RefEnvView& R, const ModelDev& m,
+                                                uint64_t& w, uint64_t& tr)
+{
+    float P[Robot::num_spheres * 4];
+    Robot::spheres(q, P);
+    for (int it = 0; it < m.num_program; ++it) {
+        const int g = m.prog_index[it];
+        bool col = false;
+        if (m.prog_kind[it] == 0) {
+            const float* c = P + 4 * m.env_gate[g];
+            if (bcc_ref_env_hit(R, c[0], c[1], c[2], (float)((double)c[3] + 0.005), w)) {
+                if (m.env_lone[g]) {
+                    col = true;
+                } else {
+                    for (int f = m.env_fine_off[g]; f < m.env_fine_off[g + 1] && !col; ++f) {
+                        const float* cf = P + 4 * m.env_fine[f];
+                        col = bcc_ref_env_hit(R, cf[0], cf[1], cf[2], (float)((double)cf[3] + 0.005), w);
+                    }
+                }
+            }
+        } else {
+            const float* a = P + 4 * m.self_gate_a[g];
+            const float* b = P + 4 * m.self_gate_b[g];
+            w += 11;
+            if (bcc_self_hit(a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3])) {
+                if (m.self_lone[g]) {
+                    col = true;
+                } else {
+                    for (int ia = m.self_a_off[g]; ia < m.self_a_off[g + 1] && !col; ++ia) {
+                        const float* fa = P + 4 * m.self_a[ia];
+                        for (int ib = m.self_b_off[g]; ib < m.self_b_off[g + 1] && !col; ++ib) {
+                            const float* fb = P + 4 * m.self_b[ib];
+                            w += 11;
+                            col = bcc_self_hit(fa[0], fa[1], fa[2], fa[3], fb[0], fb[1], fb[2], fb[3]);
+                        }
+                    }
+                }
+            }
+        }
+        if (col) {
+            w += (uint64_t)m.prog_fk_ops[it];
+            tr += (uint64_t)m.prog_fk_trig[it];
+            return true;
+        }
+    }
+    w += (uint64_t)m.ref_fk_flops;
+    tr += (uint64_t)m.ref_fk_trig;
+    return false;
+}
+
+// Reference-order work counter: one half-wave per pair (grid-stride), every
+// lane replays its sample, the step loop stops after the first step in which
+// any lane collided (src/batch_cc.cu:118-129).  Adds each pair's operation
+// and sin/cos counts into work[p] / trig[p] (zeroed by the caller) and writes
+// its verdict.
+template <class Robot>
+__global__ __launch_bounds__(kBlock) void work_kernel(const float* __restrict__ ref_obs,
+                                                      const int* __restrict__ ref_meta,
+                                                      const float* __restrict__ edges, int num_edges, int num_envs,
+                                                      int resolution, ModelDev model,
+                                                      unsigned long long* __restrict__ work,
+                                                      unsigned long long* __restrict__ trig,
+                                                      uint8_t* __restrict__ out)
+{
+    constexpr int D = Robot::dim;
+    const int lane = threadIdx.x & 63;
+    const int t = lane & 31;
+    const long long pairs = (long long)num_edges * num_envs;
+    const long long halves = ((long long)gridDim.x * kBlock) >> 5;
+    for (long long p = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 5; p < pairs; p += halves) {
+        const int edge = (int)(p / num_envs);
+        const int env = (int)(p - (long long)edge * num_envs);
+        const RefEnvView R = load_ref_env(ref_obs, ref_meta, env);
+        float q[D], d[D];
+        const int n = setup_pair<D>(edges + (size_t)edge * 2 * D, resolution, t, q, d);
+        uint64_t w = 0, tr = 0;
+        bool any = false;
+        for (int i = 0; i < n && !any; ++i) {
+            const bool c = ref_sample_work<Robot>(q, R, model, w, tr);
+            any = ((__ballot(c) >> (lane & 32)) & 0xffffffffull) != 0;
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                q[j] = q[j] + d[j];
+        }
+        atomicAdd(work + p, (unsigned long long)w);
+        atomicAdd(trig + p, (unsigned long long)tr);
+        if (t == 0)
+            out[p] = any ? 1 : 0;
+    }
+}
+
+struct WorkArgs {
+    const float* ref_obs;
+    const int* ref_meta;
+    const float* edges;
+    int num_edges;
+    int num_envs;
+    int resolution;
+    ModelDev model;
+    unsigned long long* work;
+    unsigned long long* trig;
+    uint8_t* out;
+};
+
+template <class Robot>
+hipError_t launch_work(const WorkArgs& a, hipStream_t stream)
+{
+    if (a.num_edges == 0 || a.num_envs == 0)
+        return hipSuccess;
+    const long long halves = (long long)a.num_edges * a.num_envs;
+    const long long blocks = std::min<long long>((halves + 7) / 8, 65536);
+    hipLaunchKernelGGL(work_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.ref_obs, a.ref_meta,
+                       a.edges, a.num_edges, a.num_envs, a.resolution, a.model, a.work, a.trig, a.out);
+    return hipGetLastError();
+}
+
+struct DenseArgs {
+    const float* obs;
+    const int* meta;
+    const float* edges;
+    int num_edges;
+    int num_envs;
+    int resolution;
+    uint8_t* out;
+};
+
+template <class Robot>
+hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
+{
+    if (a.num_edges == 0 || a.num_envs == 0)
+        return hipSuccess;
+    const int chunk = 64;
+    const int chunks_per_env = (a.num_edges + chunk - 1) / chunk;
+    const long long blocks = (long long)chunks_per_env * a.num_envs;
+    if (blocks > 0x7fffffffLL)
+        return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(dense_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
+                       a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.out);
+    return hipGetLastError();
+}
+
+template <class Robot>
+hipError_t launch_fk(const float* qs, int n, float* out, hipStream_t stream)
+{
+    if (n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(fk_kernel<Robot>, dim3((n + 63) / 64), dim3(64), 0, stream, qs, n, out);
+    return hipGetLastError();
+}
+
+template <class Robot>
+hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out, float* cfg0, float* cfg_last,
+                          hipStream_t stream)
+{
+    if (m == 0)
+        return hipSuccess;
+    const long long threads = (long long)m * 32;
+    hipLaunchKernelGGL(samples_kernel<Robot>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, stream, edges, m,
+                       resolution, n_out, cfg0, cfg_last);
+    return hipGetLastError();
+}
+
+// Per-robot entry points, one translation unit each (robot_<name>.hip).
+#define BCC_DECLARE_ROBOT(name)                                                                                \
+    hipError_t launch_dense_##name(const DenseArgs& a, hipStream_t stream);                                    \
+    hipError_t launch_fk_##name(const float* qs, int n, float* out, hipStream_t stream);                       \
+    hipError_t launch_samples_##name(const float* edges, int m, int resolution, int* n_out, float* cfg0,       \
+                                     float* cfg_last, hipStream_t stream);                                     \
+    hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream);
+
+#define BCC_DEFINE_ROBOT(name, Robot)                                                                          \
+    hipError_t launch_dense_##name(const DenseArgs& a, hipStream_t stream)                                     \
+    {                                                                                                          \
+        return launch_dense<Robot>(a, stream);                                                                 \
+    }                                                                                                          \
+    hipError_t launch_fk_##name(const float* qs, int n, float* out, hipStream_t stream)                        \
+    {                                                                                                          \
+        return launch_fk<Robot>(qs, n, out, stream);                                                           \
+    }                                                                                                          \
+    hipError_t launch_samples_##name(const float* edges, int m, int resolution, int* n_out, float* cfg0,       \
+                                     float* cfg_last, hipStream_t stream)                                      \
+    {                                                                                                          \
+        return launch_samples<Robot>(edges, m, resolution, n_out, cfg0, cfg_last, stream);                     \
+    }                                                                                                          \
+    hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream)                                       \
+    {                                                                                                          \
+        return launch_work<Robot>(a, stream);                                                                  \
+    }
+
+BCC_DECLARE_ROBOT(panda)
+BCC_DECLARE_ROBOT(fetch)
+BCC_DECLARE_ROBOT(baxter)
+
+}  // namespace bcc

@@ -1,0 +1,152 @@
+"""Seeded synthetic scenes and edges for the benchmark configurations.
+
+Obstacles are built as the reference's own field values (src/collision/shapes.hh):
+  - axis-aligned cuboid: centre, identity axes, half-extents
+    (factory::cuboid with zero Euler angles, src/collision/factory.hh:28-62);
+  - sphere: centre, radius;
+  - z-axis cylinder of radius r and height h centred at c: stored, as every
+    reference driver does, as a *capsule* through factory::cylinder::center
+    (src/collision/factory.hh:165-185, scripts/batch_cc_mbm.cpp:46-58):
+    p1 = c + (0,0,h/2), p2 = c - (0,0,h/2), v = p2 - p1 (float arithmetic),
+    rdv = float(1.0 / double(v.v)) (factory.hh:118-129);
+  - z-axis capsule: the same record (radius and length drawn like cylinders).
+Joint configurations use the reference's limits (src/Robots.hh:17-41,72-98,
+128-166): q = u * s_m + s_a.
+
+Configurations (sizes from the benchmark definition):
+  c0  panda  100 envs x 1000 edges; 16 cuboids, 8 spheres, 8 cylinders per env
+  c1  fetch  1000 envs x 10000 edges; U{4..64} obstacles of mixed kinds;
+             lognormal edge lengths (median 0.5 rad, sigma 0.8), clipped
+  c2  baxter 4096 envs, Zipf(1.2) edges per env on 1..4096, U{8..256}
+             obstacles with 20% capsules (ragged)
+  c3  panda  1000 envs x 100000 edges, c0 distributions
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import DIMS, PackedScene
+
+# Joint limits: lower = s_a, range = s_m (src/Robots.hh).
+LIMITS = {
+    "panda": (np.array([-2.9671, -1.8326, -2.9671, -3.1416, -2.9671, -0.0873, -2.9671], np.float32),
+              np.array([5.9342, 3.6652, 5.9342, 3.2289, 5.9342, 3.9095999999999997, 5.9342], np.float32)),
+    "fetch": (np.array([0.0, -1.6056, -1.221, -3.14159, -2.251, -3.14159, -2.16, -3.14159], np.float32),
+              np.array([0.38615, 3.2112, 2.739, 6.28318, 4.502, 6.28318, 4.32, 6.28318], np.float32)),
+    "baxter": (np.array([-1.70167993878, -2.147, -3.05417993878, -0.05, -3.059, -1.57079632679, -3.059] * 2,
+                        np.float32),
+               np.array([3.40335987756, 3.194, 6.10835987756, 2.6679999999999997, 6.118, 3.66479632679, 6.118] * 2,
+                        np.float32)),
+}
+
+CENTRE_LO = np.array([-1.0, -1.0, 0.0], np.float32)
+CENTRE_HI = np.array([1.0, 1.0, 1.5], np.float32)
+
+
+def f32(x):
+    return np.float32(x)
+
+
+def scale_cfg(robot: str, u: np.ndarray) -> np.ndarray:
+    """q = u * s_m + s_a in float32 (src/Robots.hh:51-63)."""
+    s_a, s_m = LIMITS[robot]
+    return (u.astype(np.float32) * s_m + s_a).astype(np.float32)
+
+
+def cylinder_as_capsule(c, r, h) -> np.ndarray:
+    """factory::cylinder::center with zero Euler angles -> capsule record
+    (x1 y1 z1 xv yv zv r rdv)."""
+    c = np.asarray(c, np.float32)
+    half = f32(f32(h) / f32(2.0))
+    p1 = np.array([c[0], c[1], f32(c[2] + half)], np.float32)
+    p2 = np.array([c[0], c[1], f32(c[2] - half)], np.float32)
+    v = (p2 - p1).astype(np.float32)
+    dot = f32(f32(f32(v[0] * v[0]) + f32(v[1] * v[1])) + f32(v[2] * v[2]))
+    rdv = f32(1.0 / float(dot)) if dot != 0 else f32(np.inf)
+    return np.array([p1[0], p1[1], p1[2], v[0], v[1], v[2], f32(r), rdv], np.float32)
+
+
+def aa_cuboid(c, half) -> np.ndarray:
+    c = np.asarray(c, np.float32)
+    h = np.asarray(half, np.float32)
+    return np.array([c[0], c[1], c[2], 1, 0, 0, 0, 1, 0, 0, 0, 1, h[0], h[1], h[2]], np.float32)
+
+
+def sphere(c, r) -> np.ndarray:
+    c = np.asarray(c, np.float32)
+    return np.array([c[0], c[1], c[2], f32(r)], np.float32)
+
+
+def _centre(rng):
+    return rng.uniform(CENTRE_LO, CENTRE_HI).astype(np.float32)
+
+
+def c0_env(rng, n_cub=16, n_sph=8, n_cyl=8) -> dict:
+    cub = [aa_cuboid(_centre(rng), rng.uniform(0.05, 0.3, 3)) for _ in range(n_cub)]
+    sph = [sphere(_centre(rng), rng.uniform(0.05, 0.2)) for _ in range(n_sph)]
+    cyl = [cylinder_as_capsule(_centre(rng), rng.uniform(0.03, 0.15), rng.uniform(0.1, 0.6)) for _ in range(n_cyl)]
+    return {"cuboids": np.array(cub, np.float32).reshape(-1, 15),
+            "spheres": np.array(sph, np.float32).reshape(-1, 4),
+            "capsules": np.array(cyl, np.float32).reshape(-1, 8)}
+
+
+def mixed_env(rng, n: int, capsule_frac: float | None = None) -> dict:
+    """n obstacles: kinds drawn uniformly from cuboid, sphere, cylinder,
+    capsule (or with the given capsule fraction, the rest split evenly)."""
+    if capsule_frac is None:
+        kinds = rng.integers(0, 4, n)
+    else:
+        is_cap = rng.random(n) < capsule_frac
+        kinds = np.where(is_cap, 3, rng.integers(0, 3, n))
+    cub, sph, cap = [], [], []
+    for k in kinds:
+        if k == 0:
+            cub.append(aa_cuboid(_centre(rng), rng.uniform(0.05, 0.3, 3)))
+        elif k == 1:
+            sph.append(sphere(_centre(rng), rng.uniform(0.05, 0.2)))
+        else:  # cylinder (stored as capsule) or capsule: same record
+            cap.append(cylinder_as_capsule(_centre(rng), rng.uniform(0.03, 0.15), rng.uniform(0.1, 0.6)))
+    return {"cuboids": np.array(cub, np.float32).reshape(-1, 15),
+            "spheres": np.array(sph, np.float32).reshape(-1, 4),
+            "capsules": np.array(cap, np.float32).reshape(-1, 8)}
+
+
+def uniform_edges(robot: str, m: int, rng) -> np.ndarray:
+    dim = DIMS[robot]
+    u = rng.random((m, 2, dim), dtype=np.float32)
+    return scale_cfg(robot, u.reshape(-1, dim)).reshape(m, 2, dim)
+
+
+def lognormal_edges(robot: str, m: int, rng, median=0.5, sigma=0.8) -> np.ndarray:
+    dim = DIMS[robot]
+    s_a, s_m = LIMITS[robot]
+    lo, hi = s_a, (s_a + s_m).astype(np.float32)
+    start = scale_cfg(robot, rng.random((m, dim), dtype=np.float32))
+    d = rng.standard_normal((m, dim))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    length = rng.lognormal(np.log(median), sigma, m)
+    end = np.clip(start + d * length[:, None], lo, hi).astype(np.float32)
+    return np.stack([start, end], 1).astype(np.float32)
+
+
+def make_config(name: str, seed: int = 0, scale: float = 1.0):
+    """Returns (robot, PackedScene, edges[M,2,dim]) for c0/c1/c3, with env and
+    edge counts multiplied by `scale` (tests use small scales)."""
+    rs = np.random.default_rng([seed, 1])   # scene stream
+    re = np.random.default_rng([seed, 2])   # edge stream
+    if name == "c0":
+        E, M = 100, 1000
+        E, M = max(1, int(E * scale)), max(1, int(M * scale))
+        envs = [c0_env(rs) for _ in range(E)]
+        return "panda", PackedScene.from_envs(envs), uniform_edges("panda", M, re)
+    if name == "c1":
+        E, M = 1000, 10000
+        E, M = max(1, int(E * scale)), max(1, int(M * scale))
+        envs = [mixed_env(rs, int(rs.integers(4, 65))) for _ in range(E)]
+        return "fetch", PackedScene.from_envs(envs), lognormal_edges("fetch", M, re)
+    if name == "c3":
+        E, M = 1000, 100000
+        E, M = max(1, int(E * scale)), max(1, int(M * scale))
+        envs = [c0_env(rs) for _ in range(E)]
+        return "panda", PackedScene.from_envs(envs), uniform_edges("panda", M, re)
+    raise ValueError(f"unknown dense config {name}")

@@ -1,0 +1,186 @@
+"""Benchmark: edge-env checks/sec of the plain batch edge collision checker on
+one MI355X, with its roofline and the CPU baseline beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1] [--seed 0]
+
+A step is one launch of the dense kernel over the whole synthetic workload
+(every edge against every environment), inputs resident in HBM.  Timing is
+hipEvents on the launch stream around exactly K launches after W untimed
+ones.  The roofline's algorithmic work is the reference's own source-level
+FP32 operation count for the same pairs in plain order, counted on the GPU by
+the reference-order replay kernel (bcc_work_dense).  The CPU baseline is the
+project's C restatement of the reference checker (the reference's CPU path,
+vamp, is not available), timed on the host cores on a bounded sample.
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet)
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    pkg_dir = os.path.join(REPO, "batch-cc_amd")
+    spec = importlib.util.spec_from_file_location("batch_cc_amd", os.path.join(pkg_dir, "__init__.py"),
+                                                  submodule_search_locations=[pkg_dir])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["batch_cc_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(robot, sc, edges, seconds=10.0, max_threads=16):
+    """Time the oracle (CPU restatement) on a seeded random sample of pairs."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib
+    orc = oracle_lib.Oracle()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(cores, max_threads))
+    E, M = sc.num_envs, edges.shape[0]
+    rid = {"panda": 0, "fetch": 1, "baxter": 2}[robot]
+    rng = np.random.default_rng(1234)
+    n = 2000
+    while True:
+        pe = rng.integers(0, E, n).astype(np.int32)
+        pg = rng.integers(0, M, n).astype(np.int32)
+        t0 = time.perf_counter()
+        orc.check_pairs(rid, sc, edges, pe, pg, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= seconds or n >= E * M:
+            break
+        n = int(min(E * M, max(n * 2, n * seconds / max(dt, 1e-3) * 1.1)))
+    return {
+        "value": n / dt,
+        "unit": "edge-env checks/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{n} random (env, edge) pairs of the same workload, {dt:.1f} s, "
+                  f"C restatement of the reference checker (vamp unavailable), {threads} threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3"])
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--scale", type=float, default=1.0, help="env/edge count multiplier (1.0 = full size)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-work", action="store_true", help="skip the reference-order work count")
+    a = ap.parse_args()
+
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(pairs are independent; multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    bcc = load_pkg()
+    from batch_cc_amd import workloads as W
+
+    t0 = time.perf_counter()
+    robot, sc, edges = W.make_config(a.config, seed=a.seed, scale=a.scale)
+    E, M = sc.num_envs, edges.shape[0]
+    pairs = E * M
+    print(f"[bench] workload {a.config}: {robot}, {E} envs x {M} edges = {pairs} pairs "
+          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+
+    scene = bcc.Scene(sc)
+    d_edges = bcc.DeviceBuffer.from_array(edges)
+    d_out = bcc.DeviceBuffer(pairs)
+    d_out.zero()
+    bcc.check_dense_device(robot, scene, d_edges, M, 32, d_out)
+    bcc.synchronize()
+    verdicts = d_out.to_array(np.uint8, (M, E))
+    coll = float(verdicts.mean())
+
+    wall0 = time.perf_counter()
+    ms = bcc.time_dense(robot, scene, d_edges, M, 32, d_out, a.warmup, a.steps)
+    bcc.synchronize()
+    wall = time.perf_counter() - wall0
+    print(f"[bench] {a.steps} timed launches: {ms:.3f} ms each (wall incl. warmup {wall:.2f} s)",
+          file=sys.stderr, flush=True)
+    again = d_out.to_array(np.uint8, (M, E))
+    if not np.array_equal(again, verdicts):
+        print("[bench] verdicts changed between launches", file=sys.stderr)
+        sys.exit(1)
+    value = pairs / (ms / 1e3)
+
+    roofline = None
+    if not a.no_work:
+        t1 = time.perf_counter()
+        work, trig, wv = bcc.work_dense(robot, scene, d_edges, M, 32)
+        if not np.array_equal(wv, verdicts):
+            print("[bench] work-counter verdicts differ from the dense kernel", file=sys.stderr)
+            sys.exit(1)
+        flops = float(work.sum(dtype=np.float64))
+        trigs = float(trig.sum(dtype=np.float64))
+        achieved = flops / (ms / 1e3) / 1e12
+        roofline = {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "dense_kernel",
+            "algorithmic_flops_per_launch": flops,
+            "sincos_per_launch": trigs,
+            "note": "reference source-level FP32 ops in plain order, counted on the GPU by the "
+                    "reference-order replay (bcc_work_dense); sin/cos counted separately",
+        }
+        print(f"[bench] work count: {flops:.4g} flops/launch ({time.perf_counter() - t1:.1f} s)",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(robot, sc, edges)
+        print(f"[bench] cpu baseline {cpu['value']:.4g} pairs/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
+
+    rec = {
+        "metric": "edge-env checks/sec at bit-exact verdicts, plain order, 1 GPU",
+        "value": value,
+        "unit": "edge-env checks/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{a.config}: {robot}, {E} envs x {M} edges, seed {a.seed}",
+            "robot": robot,
+            "envs": E,
+            "edges": M,
+            "pairs": pairs,
+            "resolution": 32,
+            "order": "plain",
+            "collision_fraction": coll,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+    scene.close()
+
+
+if __name__ == "__main__":
+    main()

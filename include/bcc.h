@@ -1,0 +1,165 @@
+/*
+ * bcc.h -- C ABI of the MI355X (gfx950) batch edge collision checker.
+ *
+ * This is the drop-in boundary for the reference's entry point
+ *   template <typename Robot>
+ *   void batch_cc::batch_cc(std::vector<Environment<float>>& h_envs,
+ *                           std::vector<std::array<Robot::Configuration, 2>>& edges,
+ *                           int resolution, std::vector<bool>& results);
+ * declared at src/batch_cc.hh:4-7 and defined at src/batch_cc.cu:244-338.
+ * The header-only C++ shim include/batch_cc.hh implements exactly that
+ * signature on top of these functions; other languages bind these directly.
+ *
+ * Conventions
+ *   - Every function returns an int status: BCC_OK (0) or a BCC_ERR_* code;
+ *     bcc_last_error() returns a thread-local message for the last failure.
+ *     Nothing here prints or exits (the reference prints timings and calls
+ *     exit() on CUDA errors, src/utils.cuh:252-258).
+ *   - Robots: BCC_PANDA (7 DoF), BCC_FETCH (8), BCC_BAXTER (14).
+ *   - Edges: float [num_edges][2][dim] = (start, end) joint configurations
+ *     (src/batch_cc.cu:292-295).
+ *   - Results: uint8 [num_edges][num_envs], 1 = the edge collides in that
+ *     environment, edge-major as the reference writes them
+ *     (src/batch_cc.cu:131, :319-321).
+ *   - Every call uses the current HIP device.  Async entry points take a
+ *     hipStream_t passed as void* (NULL = default stream).
+ */
+#ifndef BCC_H
+#define BCC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BCC_OK = 0,
+    BCC_ERR_INVALID = 1,      /* bad argument (reference: std::runtime_error, src/batch_cc.cu:45-56) */
+    BCC_ERR_HIP = 2,          /* HIP runtime error */
+    BCC_ERR_NOMEM = 3,
+    BCC_ERR_UNSUPPORTED = 4
+};
+
+enum { BCC_PANDA = 0, BCC_FETCH = 1, BCC_BAXTER = 2 };
+
+typedef struct bcc_scene bcc_scene;
+
+/*
+ * One environment as flat float arrays: the numeric fields of
+ * ppln::collision::Environment<float> (src/collision/environment.hh:11-35)
+ * in the field order of src/collision/shapes.hh:
+ *   spheres    [n][4]   x y z r                              (shapes.hh:225-245)
+ *   capsules   [n][8]   x1 y1 z1 xv yv zv r rdv              (shapes.hh:131-222)
+ *   z_capsules [n][8]   same layout, axis along z
+ *   cylinders  [n][8]   same layout; accepted and stored, never tested, as in
+ *                       the reference (src/utils.cuh:213-245)
+ *   cuboids    [n][15]  x y z a1x a1y a1z a2x a2y a2z a3x a3y a3z r1 r2 r3  (shapes.hh:30-126)
+ *   z_cuboids  [n][15]  same layout, axis 3 = z
+ * A positive count with a NULL pointer is rejected (src/batch_cc.cu:45-56).
+ */
+typedef struct bcc_env_desc {
+    const float *spheres;
+    uint32_t num_spheres;
+    const float *capsules;
+    uint32_t num_capsules;
+    const float *z_capsules;
+    uint32_t num_z_capsules;
+    const float *cylinders;
+    uint32_t num_cylinders;
+    const float *cuboids;
+    uint32_t num_cuboids;
+    const float *z_cuboids;
+    uint32_t num_z_cuboids;
+} bcc_env_desc;
+
+/* Thread-local text of the last error (never NULL). */
+const char *bcc_last_error(void);
+
+/* Degrees of freedom of a robot, or -1 (src/Robots.hh:13,69,125). */
+int bcc_robot_dimension(int robot);
+
+/* Number of visible HIP devices. */
+int bcc_device_count(int *count);
+
+/*
+ * Upload environments once; the handle keeps a device-resident copy
+ * (replaces the per-call setup_environment_on_device, src/batch_cc.cu:135-235).
+ */
+int bcc_scene_create(const bcc_env_desc *envs, int num_envs, bcc_scene **out);
+
+/*
+ * Same from one packed host buffer: env e holds counts[6e+k] primitives of
+ * kind k = spheres, capsules, z-capsules, cylinders, cuboids, z-cuboids,
+ * stored from data + env_offsets[e] in that order with 4, 8, 8, 8, 15, 15
+ * floats each.
+ */
+int bcc_scene_create_packed(const float *data, const int64_t *env_offsets, const int32_t *counts, int num_envs,
+                            bcc_scene **out);
+
+int bcc_scene_destroy(bcc_scene *scene);
+int bcc_scene_num_envs(const bcc_scene *scene);
+
+/*
+ * Plain dense check, synchronous, host buffers: every edge against every
+ * environment (src/batch_cc.cu:244-338 semantics; results[edge*E + env]).
+ */
+int bcc_check_dense(int robot, const bcc_scene *scene, const float *edges, int num_edges, int resolution,
+                    uint8_t *results);
+
+/* Plain dense check on device buffers, enqueued on `stream`. */
+int bcc_check_dense_async(int robot, const bcc_scene *scene, const float *d_edges, int num_edges, int resolution,
+                          uint8_t *d_results, void *stream);
+
+/*
+ * Average device time (ms) of `iters` launches of the dense kernel after
+ * `warmup` untimed ones, measured with hipEvents on the launch stream.
+ */
+int bcc_time_dense(int robot, const bcc_scene *scene, const float *d_edges, int num_edges, int resolution,
+                   uint8_t *d_results, int warmup, int iters, void *stream, double *avg_ms);
+
+/*
+ * Reference-order work accounting (no reference counterpart; feeds the
+ * roofline): replays every pair with the reference's per-lane early exits in
+ * plain step order (src/batch_cc.cu:118-129, src/utils.cuh:213-245) and
+ * writes, per pair, the reference's source-level FP32 operation count
+ * (d_work), its sin/cos count (d_trig) and the verdict (d_results).  All
+ * three are [num_edges][num_envs] device arrays; the verdicts equal those of
+ * bcc_check_dense.
+ */
+int bcc_work_dense(int robot, const bcc_scene *scene, const float *d_edges, int num_edges, int resolution,
+                   uint64_t *d_work, uint64_t *d_trig, uint8_t *d_results, void *stream);
+
+/* Device memory helpers for bindings that do not bring their own allocator. */
+int bcc_malloc(void **d_ptr, size_t bytes);
+int bcc_free(void *d_ptr);
+int bcc_memcpy_htod(void *d_dst, const void *src, size_t bytes);
+int bcc_memcpy_dtoh(void *dst, const void *d_src, size_t bytes);
+int bcc_memset(void *d_dst, int value, size_t bytes);
+int bcc_synchronize(void);
+
+/*
+ * Test hooks (no reference counterpart): expose the device arithmetic so
+ * tests can compare it bit for bit with a host restatement.
+ *   bcc_debug_fk       : d_q [n][dim] -> d_out [n][num_spheres][4] sphere centres + radius
+ *   bcc_debug_sincos   : d_x [n] -> d_out [n][2] (sin, cos)
+ *   bcc_debug_samples  : d_edges [m][2][dim] -> d_n [m], d_cfg [m][32][dim] step-0 lane configs,
+ *                        d_last [m][32][dim] configurations of the last step (i = n-1)
+ *   bcc_debug_prims    : d_in [n][40] = x y z rp, sphere (4), capsule (8), cuboid (15 + 1 pad),
+ *                        aabb (x y z h1 h2 h3 + 2 pad) -> d_out [n][6] values of the sphere,
+ *                        capsule, z-capsule (same capsule record), cuboid, z-cuboid (same
+ *                        cuboid record) and aabb tests
+ */
+int bcc_robot_num_spheres(int robot);
+int bcc_debug_fk(int robot, const float *d_q, int n, float *d_out);
+int bcc_debug_sincos(const float *d_x, int n, float *d_out);
+int bcc_debug_samples(int robot, const float *d_edges, int m, int resolution, int32_t *d_n, float *d_cfg,
+                      float *d_last);
+int bcc_debug_prims(const float *d_in, int n, float *d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BCC_H */

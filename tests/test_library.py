@@ -1,0 +1,98 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every declared entry point, argument errors are reported (not crashed on),
+the C++ drop-in shim compiles, and the synthetic workloads follow the
+reference's input conventions."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(REPO, "include", "bcc.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(bcc_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol(bcc):
+    L = bcc.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    nm = subprocess.run(["nm", "-D", "--defined-only", bcc.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (bcc_\w+)", nm))
+    assert set(names) <= exported
+
+
+def test_robot_tables(bcc):
+    L = bcc.lib()
+    assert [L.bcc_robot_dimension(i) for i in range(3)] == [7, 8, 14]
+    assert L.bcc_robot_dimension(3) == -1
+    assert [L.bcc_robot_num_spheres(i) for i in range(3)] == [70, 125, 100]
+
+
+def test_scene_rejects_null_pointer_with_positive_count(bcc):
+    import ctypes
+    L = bcc.lib()
+    counts = np.zeros((1, 6), np.int32)
+    counts[0, 0] = 2
+    offs = np.zeros(1, np.int64)
+    h = ctypes.c_void_p()
+    rc = L.bcc_scene_create_packed(None, offs.ctypes.data, counts.ctypes.data, 1, ctypes.byref(h))
+    assert rc == 1
+    assert b"null pointers" in L.bcc_last_error()
+    counts[0, 0] = -1
+    data = np.zeros(4, np.float32)
+    rc = L.bcc_scene_create_packed(data.ctypes.data, offs.ctypes.data, counts.ctypes.data, 1, ctypes.byref(h))
+    assert rc == 1 and b"negative" in L.bcc_last_error()
+
+
+def test_check_rejects_bad_arguments(bcc):
+    L = bcc.lib()
+    assert L.bcc_check_dense(9, None, None, 1, 32, None) == 1
+    assert b"unknown robot" in L.bcc_last_error()
+    assert L.bcc_check_dense(0, None, None, 1, 32, None) == 1
+    assert b"scene is NULL" in L.bcc_last_error()
+
+
+def test_python_binding_validates_edge_shape(bcc):
+    with pytest.raises(bcc.BCCError):
+        bcc._edges_array("panda", np.zeros((3, 2, 8), np.float32))
+
+
+def test_cpp_shim_compiles_and_throws_like_the_reference(tmp_path, bcc):
+    exe = tmp_path / "shim_driver"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I" + os.path.join(REPO, "include"),
+                    os.path.join(REPO, "tests", "cpp", "shim_driver.cpp"), "-o", str(exe),
+                    "-L" + os.path.dirname(bcc.LIB_PATH), "-lbcc", "-Wl,-rpath," + os.path.dirname(bcc.LIB_PATH)],
+                   check=True)
+    r = subprocess.run([str(exe), "nullcheck"], capture_output=True, text=True)
+    assert r.returncode == 0 and "runtime_error" in r.stdout
+
+
+def test_workloads_follow_reference_conventions(bcc):
+    from batch_cc_amd import workloads as W
+    robot, sc, edges = W.make_config("c0", seed=0, scale=0.05)
+    assert robot == "panda" and sc.num_envs == 5 and edges.shape == (50, 2, 7)
+    assert np.all(sc.counts[:, 0] == 8) and np.all(sc.counts[:, 1] == 8) and np.all(sc.counts[:, 4] == 16)
+    assert np.all(sc.counts[:, 3] == 0)          # cylinders stored as capsules
+    env = sc.env(0)
+    cap = env["capsules"]
+    assert np.all(cap[:, 3] == 0) and np.all(cap[:, 4] == 0) and np.all(cap[:, 5] < 0)
+    assert np.array_equal(cap[:, 7], np.float32(1.0 / (cap[:, 5].astype(np.float32) ** 2).astype(np.float64)))
+    cub = env["cuboids"]
+    assert np.array_equal(cub[:, 3:12], np.tile(np.float32([1, 0, 0, 0, 1, 0, 0, 0, 1]), (16, 1)))
+    lo, rng_ = W.LIMITS["panda"]
+    assert np.all(edges >= lo) and np.all(edges <= lo + rng_)
+    # deterministic
+    _, sc2, e2 = W.make_config("c0", seed=0, scale=0.05)
+    assert np.array_equal(sc.data, sc2.data) and np.array_equal(edges, e2)
+    robot, sc, edges = W.make_config("c1", seed=0, scale=0.01)
+    assert robot == "fetch" and edges.shape[2] == 8
+    n = sc.counts[:, [0, 1, 4]].sum(1)
+    assert np.all((n >= 4) & (n <= 64))
