@@ -78,6 +78,7 @@ def lib():
     _sig(L, "bcc_debug_sincos", I, [P, I, P])
     _sig(L, "bcc_debug_samples", I, [I, P, I, I, P, P, P])
     _sig(L, "bcc_debug_prims", I, [P, I, P])
+    _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
     _lib = L
     return L
 
@@ -308,3 +309,7 @@ def debug_prims(inp: np.ndarray) -> np.ndarray:
     do = DeviceBuffer(inp.shape[0] * 6 * 4)
     _check(lib().bcc_debug_prims(di.ptr, inp.shape[0], do.ptr))
     return do.to_array(np.float32, (inp.shape[0], 6))
+
+
+def debug_copy(d_src: DeviceBuffer, d_dst: DeviceBuffer, nbytes: int):
+    _check(lib().bcc_debug_copy(d_src.ptr, d_dst.ptr, int(nbytes)))

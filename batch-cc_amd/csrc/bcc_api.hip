@@ -262,6 +262,12 @@ __global__ void prims_kernel(const float* __restrict__ in, int n, float* __restr
     o[5] = bcc::bcc_aabb_val(ap, x, y, z, rp * rp);
 }
 
+__global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -651,6 +657,19 @@ int bcc_debug_samples(int robot, const float* d_edges, int m, int resolution, in
     if (e != hipSuccess)
         return hip_fail(e, "samples kernel");
     BCC_HIP(hipDeviceSynchronize(), "samples kernel");
+    return BCC_OK;
+}
+
+int bcc_debug_copy(const void* d_src, void* d_dst, size_t bytes)
+{
+    if ((bytes % 16) != 0 || (bytes && (!d_src || !d_dst)))
+        return fail(BCC_ERR_INVALID, "copy size must be a multiple of 16 bytes");
+    const size_t n = bytes / 16;
+    if (n == 0)
+        return BCC_OK;
+    hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, 0, (const uint4*)d_src, (uint4*)d_dst, n);
+    BCC_HIP(hipGetLastError(), "copy kernel");
+    BCC_HIP(hipDeviceSynchronize(), "copy kernel");
     return BCC_OK;
 }
 

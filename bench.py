@@ -82,7 +82,19 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="env/edge count multiplier (1.0 = full size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-work", action="store_true", help="skip the reference-order work count")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel on BYTES (counter calibration) and exit")
     a = ap.parse_args()
+
+    if a.copy_calibration:
+        bcc = load_pkg()
+        src = bcc.DeviceBuffer(a.copy_calibration)
+        dst = bcc.DeviceBuffer(a.copy_calibration)
+        src.zero()
+        for _ in range(3):
+            bcc.debug_copy(src, dst, a.copy_calibration)
+        print(json.dumps({"copy_calibration_bytes": a.copy_calibration, "launches": 3}))
+        return
 
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "

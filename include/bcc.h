@@ -157,6 +157,9 @@ int bcc_debug_sincos(const float *d_x, int n, float *d_out);
 int bcc_debug_samples(int robot, const float *d_edges, int m, int resolution, int32_t *d_n, float *d_cfg,
                       float *d_last);
 int bcc_debug_prims(const float *d_in, int n, float *d_out);
+/* Plain device copy of `bytes` (a multiple of 16): a known-traffic kernel used
+ * to calibrate hardware byte counters. */
+int bcc_debug_copy(const void *d_src, void *d_dst, size_t bytes);
 
 #ifdef __cplusplus
 }
