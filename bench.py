@@ -165,7 +165,7 @@ def main():
               flush=True)
 
     rec = {
-        "metric": "edge-env checks/sec at bit-exact verdicts, plain order, 1 GPU",
+        "metric": "edge-env checks/sec at bit-exact verdicts, learned-ordered vs plain, 1 GPU",
         "value": value,
         "unit": "edge-env checks/s",
         "n_gpus": 1,
