@@ -47,9 +47,9 @@ def generate():
         run([sys.executable, gen, os.path.join(HERE, "robots", f"{r}.json"), os.path.join(CSRC, "robots")])
 
 
-def compile_one(src):
-    obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
-    run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], cwd=CSRC)
+def compile_one(src, objdir=OBJDIR, defines=()):
+    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+    run([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj], cwd=CSRC)
     return obj
 
 
@@ -57,25 +57,31 @@ def build_oracle():
     run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
 
 
-def build(gen: bool = True, jobs: int = 4) -> str:
+def build(gen: bool = True, jobs: int = 4, defines=(), out: str = LIB) -> str:
+    """Compile and link; `defines` adds -D macros (tuning variants), `out`
+    writes the library elsewhere (the in-tree libbcc.so is the product)."""
     if gen:
         generate()
-    os.makedirs(OBJDIR, exist_ok=True)
+    objdir = OBJDIR if out == LIB else out + ".objs"
+    os.makedirs(objdir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda src: compile_one(src, objdir, defines), SOURCES))
+    tmp = out + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
-    build_oracle()
-    return LIB
+    os.replace(tmp, out)
+    if out == LIB:
+        build_oracle()
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--no-gen", action="store_true")
     ap.add_argument("-j", type=int, default=4)
+    ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor macro")
+    ap.add_argument("--out", default=LIB, help="library path (default: the in-tree libbcc.so)")
     a = ap.parse_args()
-    print(build(gen=not a.no_gen, jobs=a.j))
+    print(build(gen=not a.no_gen, jobs=a.j, defines=a.defines, out=os.path.abspath(a.out)))
 
 
 if __name__ == "__main__":

@@ -50,8 +50,14 @@ __device__ __forceinline__ int setup_pair(const float* __restrict__ se, int reso
     return n;
 }
 
+// Minimum waves per SIMD the register allocator must allow for the dense
+// kernel (occupancy target); set per build, see build.py.
+#ifndef BCC_DENSE_WAVES
+#define BCC_DENSE_WAVES 1
+#endif
+
 template <class Robot>
-__global__ __launch_bounds__(kBlock) void dense_kernel(const float* __restrict__ obs, const int* __restrict__ meta,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENSE_WAVES, 8))) void dense_kernel(const float* __restrict__ obs, const int* __restrict__ meta,
                                                        const float* __restrict__ edges, int num_edges, int num_envs,
                                                        int resolution, int chunk, int chunks_per_env,
                                                        uint8_t* __restrict__ out)

@@ -276,7 +276,7 @@ __device__ __forceinline__ bool bcc_ref_env_hit(const RefEnvView& E, float x, fl
 // the padded radii float(double(r) + 0.005) (src/utils.cuh:215).
 
 #define BCC_SWEEP(count, stride, base, TEST)                     \
-    for (int k = 0; k < (count); ++k) {                          \
+    _Pragma("unroll 2") for (int k = 0; k < (count); ++k) {      \
         const cfptr o = (base) + (stride) * k;                   \
         _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; }  \
         SETTLED_CHECK;                                           \

@@ -1,5 +1,10 @@
 // Kernels specialised for the baxter model: the generated collision program
 // (robots/baxter_check.inc) instantiated into the dense kernel template.
+// Occupancy target of the dense kernel for this robot (waves per SIMD),
+// chosen by measurement on MI355X; see build.py / tuning.
+#ifndef BCC_DENSE_WAVES
+#define BCC_DENSE_WAVES 4
+#endif
 #include "bcc_dense.h"
 
 namespace bcc {

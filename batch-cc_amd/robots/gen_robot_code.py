@@ -126,13 +126,27 @@ class Emitter:
         return self.new(f"-{a.code()}", hint)
 
 
-def gen_fk(em: Emitter, model: dict):
-    """Emit FK; returns {frame: (R[3][3] of E, t[3] of E)}."""
-    frames = {-1: ([[E(1.0 if i == k else 0.0) for k in range(3)] for i in range(3)],
-                   [E(0.0) for _ in range(3)])}
-    for j in model["joint_order"]:
-        jt = model["joints"][j]
-        Rp, tp = frames[jt["parent"]]
+class LazyFK:
+    """Emits each joint frame on first request (parents first), at the
+    emitter's current position; callers request frames only at function
+    scope so later code can use them."""
+
+    def __init__(self, em: Emitter, model: dict, prefix: str = ""):
+        self.em = em
+        self.model = model
+        self.p = prefix
+        self.frames = {-1: ([[E(1.0 if i == k else 0.0) for k in range(3)] for i in range(3)],
+                            [E(0.0) for _ in range(3)])}
+
+    def __getitem__(self, j):
+        if j not in self.frames:
+            self.emit(j)
+        return self.frames[j]
+
+    def emit(self, j):
+        em = self.em
+        jt = self.model["joints"][j]
+        Rp, tp = self[jt["parent"]]
         G = [[f32(v) for v in row] for row in jt["R"]]
         g = [f32(v) for v in jt["t"]]
         em.line(f"// joint {j} ({jt['type']}), parent frame {jt['parent']}")
@@ -144,9 +158,9 @@ def gen_fk(em: Emitter, model: dict):
                     if G[l][k] == 0.0:
                         continue
                     if acc is None:
-                        acc = em.mul(Rp[i][l], E(G[l][k]), f"a{j}_{i}{k}_{l}")
+                        acc = em.mul(Rp[i][l], E(G[l][k]), f"{self.p}a{j}_{i}{k}_{l}")
                     else:
-                        acc = em.fma(Rp[i][l], E(G[l][k]), acc, f"a{j}_{i}{k}_{l}")
+                        acc = em.fma(Rp[i][l], E(G[l][k]), acc, f"{self.p}a{j}_{i}{k}_{l}")
                 A[i][k] = acc if acc is not None else E(0.0)
         t = []
         for i in range(3):
@@ -154,27 +168,34 @@ def gen_fk(em: Emitter, model: dict):
             for l in range(3):
                 if g[l] == 0.0:
                     continue
-                acc = em.fma(Rp[i][l], E(g[l]), acc, f"t{j}_{i}_{l}")
+                acc = em.fma(Rp[i][l], E(g[l]), acc, f"{self.p}t{j}_{i}_{l}")
             t.append(acc)
         if jt["type"] == "revolute":
-            em.line(f"float s{j}, c{j};")
-            em.line(f"bcc_sincosf(q[{j}], s{j}, c{j});")
-            s = E(name=f"s{j}")
-            c = E(name=f"c{j}")
-            ns = em.neg(s, f"ns{j}")
+            em.line(f"float {self.p}s{j}, {self.p}c{j};")
+            em.line(f"bcc_sincosf(q[{j}], {self.p}s{j}, {self.p}c{j});")
+            s = E(name=f"{self.p}s{j}")
+            c = E(name=f"{self.p}c{j}")
+            ns = em.neg(s, f"{self.p}ns{j}")
             R = [[None] * 3 for _ in range(3)]
             for i in range(3):
-                ca0 = em.mul(c, A[i][0], f"ca{j}_{i}0")
-                R[i][0] = em.fma(s, A[i][1], ca0, f"R{j}_{i}0")
-                ca1 = em.mul(c, A[i][1], f"ca{j}_{i}1")
-                R[i][1] = em.fma(ns, A[i][0], ca1, f"R{j}_{i}1")
+                ca0 = em.mul(c, A[i][0], f"{self.p}ca{j}_{i}0")
+                R[i][0] = em.fma(s, A[i][1], ca0, f"{self.p}R{j}_{i}0")
+                ca1 = em.mul(c, A[i][1], f"{self.p}ca{j}_{i}1")
+                R[i][1] = em.fma(ns, A[i][0], ca1, f"{self.p}R{j}_{i}1")
                 R[i][2] = A[i][2]
-            frames[j] = (R, t)
+            self.frames[j] = (R, t)
         else:
             qj = E(name=f"q[{j}]")
-            t2 = [em.fma(qj, A[i][2], t[i], f"tq{j}_{i}") for i in range(3)]
-            frames[j] = (A, t2)
-    return frames
+            t2 = [em.fma(qj, A[i][2], t[i], f"{self.p}tq{j}_{i}") for i in range(3)]
+            self.frames[j] = (A, t2)
+
+
+def gen_fk(em: Emitter, model: dict):
+    """Emit the whole FK chain; returns the frame table."""
+    fk = LazyFK(em, model)
+    for j in model["joint_order"]:
+        fk[j]
+    return fk
 
 
 def centre(em: Emitter, frames, sph: dict, tag: str) -> list[E]:
@@ -196,7 +217,8 @@ def padded_radius(r: float) -> float:
     return f32(np.float64(np.float32(r)) + 0.005)
 
 
-GATE_CHUNK = 4
+GATE_CHUNK = 4      # gate/lone spheres per environment sweep
+FINE_CHUNK = 8      # fine spheres per gated sweep
 
 
 def gen_check(model: dict) -> str:
@@ -211,8 +233,12 @@ def gen_check(model: dict) -> str:
     sph = model["spheres"]
     em = Emitter()
     em.line(f"// generated by gen_robot_code.py from the {robot} model; do not edit")
-    frames = gen_fk(em, model)
+    frames = LazyFK(em, model)
     em.line("uint64_t hit = 0;")
+
+    def need_frames(spheres_used):
+        for s in spheres_used:
+            frames[sph[s]["frame"]]
 
     cache: dict[int, list[str]] = {}
 
@@ -230,6 +256,7 @@ def gen_check(model: dict) -> str:
         for c0 in range(0, len(order), gate_chunk):
             chunk = order[c0:c0 + gate_chunk]
             ng = len(chunk)
+            need_frames([envp[gi]["gate"] for gi in chunk] + [f for gi in chunk for f in envp[gi]["fine"]])
             em.line("{")
             em.indent += 1
             gx, gy, gz, grp = [], [], [], []
@@ -250,21 +277,28 @@ def gen_check(model: dict) -> str:
                 if g["lone"]:
                     em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot(gh[{k}]) & live); hit |= d; live &= ~d; }}")
                     continue
-                nf = len(g["fine"])
-                em.line(f"if (const uint64_t need = __ballot(gh[{k}]) & live) {{")
+                em.line(f"if (uint64_t need = __ballot(gh[{k}]) & live) {{")
                 em.indent += 1
-                fx, fy, fz, frp = [], [], [], []
-                for fi, f in enumerate(g["fine"]):
-                    c = c3(f, f"f{gi}_{fi}", reuse=False)
-                    fx.append(c[0]); fy.append(c[1]); fz.append(c[2])
-                    frp.append(flit(padded_radius(sph[f]["radius"])))
-                em.line(f"const float fx[{nf}] = {{{', '.join(fx)}}};")
-                em.line(f"const float fy[{nf}] = {{{', '.join(fy)}}};")
-                em.line(f"const float fz[{nf}] = {{{', '.join(fz)}}};")
-                em.line(f"const float frp[{nf}] = {{{', '.join(frp)}}};")
-                em.line(f"const bool h = bcc_env_any<{nf}>(env, fx, fy, fz, frp, need);")
-                em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
-                em.line("hit |= d; live &= ~d;")
+                fines = g["fine"]
+                for f0 in range(0, len(fines), FINE_CHUNK):
+                    sub = fines[f0:f0 + FINE_CHUNK]
+                    nf = len(sub)
+                    em.line("if (need) {")
+                    em.indent += 1
+                    fx, fy, fz, frp = [], [], [], []
+                    for fi, f in enumerate(sub):
+                        c = c3(f, f"f{gi}_{f0 + fi}", reuse=False)
+                        fx.append(c[0]); fy.append(c[1]); fz.append(c[2])
+                        frp.append(flit(padded_radius(sph[f]["radius"])))
+                    em.line(f"const float fx[{nf}] = {{{', '.join(fx)}}};")
+                    em.line(f"const float fy[{nf}] = {{{', '.join(fy)}}};")
+                    em.line(f"const float fz[{nf}] = {{{', '.join(fz)}}};")
+                    em.line(f"const float frp[{nf}] = {{{', '.join(frp)}}};")
+                    em.line(f"const bool h = bcc_env_any<{nf}>(env, fx, fy, fz, frp, need);")
+                    em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
+                    em.line("hit |= d; live &= ~d; need &= ~d;")
+                    em.indent -= 1
+                    em.line("}")
                 em.indent -= 1
                 em.line("}")
             em.indent -= 1
@@ -273,21 +307,28 @@ def gen_check(model: dict) -> str:
 
     def emit_self():
         selfp = model["self_program"]
+        gates = []
+        for g in selfp:
+            for s in g["gate"]:
+                if s not in gates:
+                    gates.append(s)
+        need_frames(gates)
+        gc = {s: c3(s, f"sg{s}", reuse=False) for s in gates}
         for gi, g in enumerate(selfp):
             a, b = g["gate"]
-            ac = c3(a, f"s{a}")
-            bc = c3(b, f"s{b}")
             ra = flit(sph[a]["radius"])
             rb = flit(sph[b]["radius"])
-            em.line(f"const bool sg{gi} = bcc_self_hit({ac[0]}, {ac[1]}, {ac[2]}, {ra}, {bc[0]}, {bc[1]}, {bc[2]}, {rb});")
-        for gi, g in enumerate(selfp):
             em.line(f"// self group {gi}" + (f" ({g['label']})" if g["label"] else ""))
+            gate = (f"bcc_self_hit({gc[a][0]}, {gc[a][1]}, {gc[a][2]}, {ra}, "
+                    f"{gc[b][0]}, {gc[b][1]}, {gc[b][2]}, {rb})")
             if g["lone"]:
-                em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot(sg{gi}) & live); hit |= d; live &= ~d; }}")
+                em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot({gate}) & live); hit |= d; live &= ~d; }}")
                 continue
-            em.line(f"if (const uint64_t need0 = __ballot(sg{gi}) & live) {{")
+            em.line(f"if (uint64_t need = __ballot({gate}) & live) {{")
             em.indent += 1
-            em.line("uint64_t need = need0;")
+            # frames for the fine spheres are recomputed in this (rarely taken)
+            # block instead of being kept live through the whole section
+            local = LazyFK(em, model, prefix=f"L{gi}_")
             A = []
             B = []
             for x, y in g["fine"]:
@@ -296,11 +337,14 @@ def gen_check(model: dict) -> str:
                 if y not in B:
                     B.append(y)
             assert set(map(tuple, g["fine"])) == {(x, y) for x in A for y in B}, "self group is not a cross product"
-            acs = [(c3(x, f"fa{gi}_{ai}", reuse=False), flit(sph[x]["radius"])) for ai, x in enumerate(A)]
+            for s in A + B:
+                local[sph[s]["frame"]]
+            acs = [([p.code() for p in centre(em, local, sph[x], f"fa{gi}_{ai}")], flit(sph[x]["radius"]))
+                   for ai, x in enumerate(A)]
             for bi, y in enumerate(B):
                 em.line("if (need) {")
                 em.indent += 1
-                yc = c3(y, f"fb{gi}_{bi}", reuse=False)
+                yc = [p.code() for p in centre(em, local, sph[y], f"fb{gi}_{bi}")]
                 ry = flit(sph[y]["radius"])
                 terms = [f"bcc_self_hit({xc[0]}, {xc[1]}, {xc[2]}, {rx}, {yc[0]}, {yc[1]}, {yc[2]}, {ry})"
                          for xc, rx in acs]
