@@ -68,6 +68,14 @@ def lib():
     _sig(L, "bcc_check_dense_async", I, [I, P, P, I, I, P, P])
     _sig(L, "bcc_time_dense", I, [I, P, P, I, I, P, I, I, P, ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_work_dense", I, [I, P, P, I, I, P, P, P, P])
+    _sig(L, "bcc_work_pairs", I, [I, P, P, I, P, I, P, P, P, P])
+    _sig(L, "bcc_check_ragged", I, [I, P, P, P, I, P])
+    _sig(L, "bcc_ragged_plan_create", I, [I, P, P, P, I, P, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_ragged_plan_launch", I, [P, P])
+    _sig(L, "bcc_ragged_plan_launch_eager", I, [P, P])
+    _sig(L, "bcc_ragged_plan_num_tiles", I, [P])
+    _sig(L, "bcc_ragged_plan_time", I, [P, I, I, P, ctypes.POINTER(ctypes.c_double)])
+    _sig(L, "bcc_ragged_plan_destroy", I, [P])
     _sig(L, "bcc_malloc", I, [ctypes.POINTER(P), ctypes.c_size_t])
     _sig(L, "bcc_free", I, [P])
     _sig(L, "bcc_memcpy_htod", I, [P, P, ctypes.c_size_t])
@@ -268,6 +276,71 @@ def work_dense(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, resol
                                 do.ptr, None))
     synchronize()
     return (dw.to_array(np.uint64, (M, E)), dt.to_array(np.uint64, (M, E)), do.to_array(np.uint8, (M, E)))
+
+
+def work_pairs(robot, scene: Scene, d_edges: DeviceBuffer, pair_env: np.ndarray, resolution: int = 32):
+    """Reference-order counts for an explicit pair list (pair p = edge p in env pair_env[p])."""
+    pe = np.ascontiguousarray(pair_env, np.int32)
+    n = pe.size
+    dpe = DeviceBuffer.from_array(pe)
+    dw = DeviceBuffer(n * 8)
+    dt = DeviceBuffer(n * 8)
+    do = DeviceBuffer(n)
+    _check(lib().bcc_work_pairs(robot_id(robot), scene.handle, d_edges.ptr, n, dpe.ptr, int(resolution), dw.ptr,
+                                dt.ptr, do.ptr, None))
+    synchronize()
+    return dw.to_array(np.uint64, (n,)), dt.to_array(np.uint64, (n,)), do.to_array(np.uint8, (n,))
+
+
+def check_ragged(robot, scene: Scene, env_off, edges, resolution: int = 32) -> np.ndarray:
+    """Verdicts [total] for a CSR batch: env e owns pairs env_off[e]:env_off[e+1]."""
+    off = np.ascontiguousarray(env_off, np.int64)
+    if off.size != scene.num_envs + 1:
+        raise BCCError("env_off must have num_envs + 1 entries")
+    e = _edges_array(robot, edges)
+    if e.shape[0] != int(off[-1]):
+        raise BCCError("edges must have env_off[-1] rows")
+    out = np.zeros(e.shape[0], np.uint8)
+    _check(lib().bcc_check_ragged(robot_id(robot), scene.handle, _ptr(off), _ptr(e), int(resolution), _ptr(out)))
+    return out
+
+
+class RaggedPlan:
+    """Tile list + captured hipGraph for repeated ragged launches on device buffers."""
+
+    def __init__(self, robot, scene: Scene, env_off, d_edges: DeviceBuffer, d_out: DeviceBuffer,
+                 resolution: int = 32, tile_pairs: int = 0):
+        self.off = np.ascontiguousarray(env_off, np.int64)
+        self.scene, self.d_edges, self.d_out = scene, d_edges, d_out
+        self.handle = ctypes.c_void_p()
+        _check(lib().bcc_ragged_plan_create(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
+                                            int(resolution), d_out.ptr, int(tile_pairs), ctypes.byref(self.handle)))
+
+    @property
+    def num_tiles(self) -> int:
+        return lib().bcc_ragged_plan_num_tiles(self.handle)
+
+    def launch(self, stream=None):
+        _check(lib().bcc_ragged_plan_launch(self.handle, stream))
+
+    def launch_eager(self, stream=None):
+        _check(lib().bcc_ragged_plan_launch_eager(self.handle, stream))
+
+    def time(self, warmup: int, iters: int, stream=None) -> float:
+        ms = ctypes.c_double(0.0)
+        _check(lib().bcc_ragged_plan_time(self.handle, int(warmup), int(iters), stream, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            _check(lib().bcc_ragged_plan_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ------------------------------------------------------------ test hooks

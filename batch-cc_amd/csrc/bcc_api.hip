@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -57,6 +58,7 @@ struct bcc_scene {
     float* d_ref_obs = nullptr;  // the reference's lists and order, for the work counter
     int* d_ref_meta = nullptr;
     size_t obs_floats = 0;
+    std::vector<int> obstacles;  // tested primitives per environment (host copy)
 };
 
 namespace {
@@ -158,6 +160,11 @@ int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene**
     bcc_scene* s = new bcc_scene;
     s->num_envs = num_envs;
     s->obs_floats = total;
+    s->obstacles.resize(num_envs);
+    for (int e = 0; e < num_envs; ++e) {
+        const bcc_env_desc& d = envs[e];
+        s->obstacles[e] = (int)(d.num_spheres + d.num_capsules + d.num_z_capsules + d.num_cuboids + d.num_z_cuboids);
+    }
     const size_t meta_bytes = sizeof(int) * std::max<size_t>(meta.size(), 1);
     hipError_t err = hipMalloc((void**)&s->d_obs, sizeof(float) * total);
     if (err == hipSuccess)
@@ -545,12 +552,29 @@ int device_model(int robot, bcc::ModelDev* out)
 
 }  // namespace
 
+static int work_common(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, const int* d_pair_env,
+                       int resolution, uint64_t* d_work, uint64_t* d_trig, uint8_t* d_results, void* stream);
+
 int bcc_work_dense(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
                    uint64_t* d_work, uint64_t* d_trig, uint8_t* d_results, void* stream)
 {
+    return work_common(robot, scene, d_edges, num_edges, nullptr, resolution, d_work, d_trig, d_results, stream);
+}
+
+int bcc_work_pairs(int robot, const bcc_scene* scene, const float* d_edges, int num_pairs, const int32_t* d_pair_env,
+                   int resolution, uint64_t* d_work, uint64_t* d_trig, uint8_t* d_results, void* stream)
+{
+    if (num_pairs > 0 && !d_pair_env)
+        return fail(BCC_ERR_INVALID, "pair_env is NULL");
+    return work_common(robot, scene, d_edges, num_pairs, d_pair_env, resolution, d_work, d_trig, d_results, stream);
+}
+
+static int work_common(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, const int* d_pair_env,
+                       int resolution, uint64_t* d_work, uint64_t* d_trig, uint8_t* d_results, void* stream)
+{
     if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
         return rc;
-    const size_t pairs = (size_t)num_edges * (size_t)scene->num_envs;
+    const size_t pairs = d_pair_env ? (size_t)num_edges : (size_t)num_edges * (size_t)scene->num_envs;
     if (pairs == 0)
         return BCC_OK;
     if (!d_work || !d_trig)
@@ -562,7 +586,7 @@ int bcc_work_dense(int robot, const bcc_scene* scene, const float* d_edges, int 
     if (int rc = device_model(robot, &md))
         return rc;
     bcc::WorkArgs a{scene->d_ref_obs, scene->d_ref_meta, d_edges, num_edges, scene->num_envs, resolution, md,
-                    (unsigned long long*)d_work, (unsigned long long*)d_trig, d_results};
+                    d_pair_env, (unsigned long long*)d_work, (unsigned long long*)d_trig, d_results};
     hipError_t e;
     switch (robot) {
     case BCC_PANDA: e = bcc::launch_work_panda(a, s); break;
@@ -573,6 +597,249 @@ int bcc_work_dense(int robot, const bcc_scene* scene, const float* d_edges, int 
     if (e != hipSuccess)
         return hip_fail(e, "work kernel launch");
     return BCC_OK;
+}
+
+}  // extern "C"
+
+struct bcc_ragged_plan {
+    int robot = 0;
+    const bcc_scene* scene = nullptr;
+    bcc::RaggedTile* d_tiles = nullptr;
+    int* d_counter = nullptr;
+    int num_tiles = 0;
+    int64_t num_pairs = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bcc::RaggedArgs args{};
+};
+
+namespace {
+
+int dispatch_ragged(int robot, const bcc::RaggedArgs& a, hipStream_t s)
+{
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_ragged_panda(a, s); break;
+    case BCC_FETCH: e = bcc::launch_ragged_fetch(a, s); break;
+    case BCC_BAXTER: e = bcc::launch_ragged_baxter(a, s); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "ragged kernel launch");
+    return BCC_OK;
+}
+
+// Default tile size and persistent grid factor per robot (measured; see
+// batch-cc_amd/tune.py).
+int default_tile_pairs(int robot)
+{
+    switch (robot) {
+    case BCC_PANDA: return 64;
+    case BCC_FETCH: return 64;
+    default: return 32;
+    }
+}
+
+int persistent_grid(int robot)
+{
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int waves_per_simd = robot == BCC_FETCH ? 6 : 4;
+    return cus * waves_per_simd;   // 4 waves per workgroup, 4 SIMDs per CU
+}
+
+}  // namespace
+
+extern "C" {
+
+void bcc_ragged_plan_destroy_impl(bcc_ragged_plan* p)
+{
+    if (!p)
+        return;
+    if (p->exec)
+        (void)hipGraphExecDestroy(p->exec);
+    if (p->graph)
+        (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->d_tiles);
+    (void)hipFree(p->d_counter);
+    delete p;
+}
+
+int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* d_edges,
+                           int resolution, uint8_t* d_results, int tile_pairs, bcc_ragged_plan** out)
+{
+    if (!out)
+        return fail(BCC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (int rc = check_dense_args(robot, scene, d_edges, 0, resolution, d_results))
+        return rc;
+    if (!env_offsets)
+        return fail(BCC_ERR_INVALID, "env_offsets is NULL");
+    const int E = scene->num_envs;
+    if (env_offsets[0] != 0)
+        return fail(BCC_ERR_INVALID, "env_offsets[0] must be 0");
+    for (int e = 0; e < E; ++e)
+        if (env_offsets[e + 1] < env_offsets[e])
+            return fail(BCC_ERR_INVALID, "env_offsets must be non-decreasing");
+    const int64_t total = env_offsets[E];
+    if (total > 0x7fffffffLL)
+        return fail(BCC_ERR_INVALID, "more than 2^31 pairs in one ragged batch");
+    if (total > 0 && (!d_edges || !d_results))
+        return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
+    if (tile_pairs <= 0)
+        tile_pairs = default_tile_pairs(robot);
+    std::vector<bcc::RaggedTile> tiles;
+    for (int e = 0; e < E; ++e)
+        for (int64_t b = env_offsets[e]; b < env_offsets[e + 1]; b += tile_pairs)
+            tiles.push_back(bcc::RaggedTile{e, (int)b, (int)std::min<int64_t>(b + tile_pairs, env_offsets[e + 1]), 0});
+    // heavy environments first: more obstacles, then larger tiles
+    std::stable_sort(tiles.begin(), tiles.end(), [&](const bcc::RaggedTile& x, const bcc::RaggedTile& y) {
+        const int64_t wx = (int64_t)scene->obstacles[x.env] * (x.end - x.begin);
+        const int64_t wy = (int64_t)scene->obstacles[y.env] * (y.end - y.begin);
+        return wx > wy;
+    });
+    bcc_ragged_plan* p = new bcc_ragged_plan;
+    p->robot = robot;
+    p->scene = scene;
+    p->num_tiles = (int)tiles.size();
+    p->num_pairs = total;
+    hipError_t err = hipMalloc((void**)&p->d_tiles, sizeof(bcc::RaggedTile) * std::max<size_t>(tiles.size(), 1));
+    if (err == hipSuccess)
+        err = hipMalloc((void**)&p->d_counter, sizeof(int));
+    if (err == hipSuccess && !tiles.empty())
+        err = hipMemcpy(p->d_tiles, tiles.data(), sizeof(bcc::RaggedTile) * tiles.size(), hipMemcpyHostToDevice);
+    if (err != hipSuccess) {
+        bcc_ragged_plan_destroy_impl(p);
+        return hip_fail(err, "ragged plan upload");
+    }
+    p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
+                              resolution, persistent_grid(robot), d_results};
+    // capture reset + persistent kernel once; launches replay the graph
+    hipStream_t cap;
+    err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    if (err == hipSuccess) {
+        err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        if (err == hipSuccess) {
+            const int rc = dispatch_ragged(robot, p->args, cap);
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(cap, &g);
+            p->graph = g;
+            if (rc != BCC_OK) {
+                (void)hipStreamDestroy(cap);
+                bcc_ragged_plan_destroy_impl(p);
+                return rc;
+            }
+            err = e2;
+        }
+        if (err == hipSuccess)
+            err = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        (void)hipStreamDestroy(cap);
+    }
+    if (err != hipSuccess) {
+        bcc_ragged_plan_destroy_impl(p);
+        return hip_fail(err, "ragged graph capture");
+    }
+    *out = p;
+    return BCC_OK;
+}
+
+int bcc_ragged_plan_launch(bcc_ragged_plan* plan, void* stream)
+{
+    if (!plan)
+        return fail(BCC_ERR_INVALID, "plan is NULL");
+    BCC_HIP(hipGraphLaunch(plan->exec, (hipStream_t)stream), "hipGraphLaunch");
+    return BCC_OK;
+}
+
+int bcc_ragged_plan_launch_eager(bcc_ragged_plan* plan, void* stream)
+{
+    if (!plan)
+        return fail(BCC_ERR_INVALID, "plan is NULL");
+    return dispatch_ragged(plan->robot, plan->args, (hipStream_t)stream);
+}
+
+int bcc_ragged_plan_num_tiles(const bcc_ragged_plan* plan)
+{
+    return plan ? plan->num_tiles : -1;
+}
+
+int bcc_ragged_plan_time(bcc_ragged_plan* plan, int warmup, int iters, void* stream, double* avg_ms)
+{
+    if (!plan || !avg_ms || iters < 1 || warmup < 0)
+        return fail(BCC_ERR_INVALID, "bad timing arguments");
+    hipStream_t s = (hipStream_t)stream;
+    for (int i = 0; i < warmup; ++i)
+        BCC_HIP(hipGraphLaunch(plan->exec, s), "hipGraphLaunch");
+    hipEvent_t t0, t1;
+    BCC_HIP(hipEventCreate(&t0), "hipEventCreate");
+    BCC_HIP(hipEventCreate(&t1), "hipEventCreate");
+    hipError_t e = hipEventRecord(t0, s);
+    for (int i = 0; i < iters && e == hipSuccess; ++i)
+        e = hipGraphLaunch(plan->exec, s);
+    if (e == hipSuccess)
+        e = hipEventRecord(t1, s);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(t1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (e != hipSuccess)
+        return hip_fail(e, "ragged timing");
+    *avg_ms = (double)ms / iters;
+    return BCC_OK;
+}
+
+int bcc_ragged_plan_destroy(bcc_ragged_plan* plan)
+{
+    bcc_ragged_plan_destroy_impl(plan);
+    return BCC_OK;
+}
+
+int bcc_check_ragged(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* edges,
+                     int resolution, uint8_t* results)
+{
+    if (int rc = check_dense_args(robot, scene, edges, 0, resolution, results))
+        return rc;
+    if (!env_offsets)
+        return fail(BCC_ERR_INVALID, "env_offsets is NULL");
+    const int64_t total = env_offsets[scene->num_envs];
+    if (total <= 0)
+        return BCC_OK;
+    if (!edges || !results)
+        return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
+    const int dim = bcc_robot_dimension(robot);
+    for (int64_t i = 0; i < total * 2 * dim; ++i)
+        if (!std::isfinite(edges[i]))
+            return fail(BCC_ERR_INVALID, "edge " + std::to_string(i / (2 * dim)) + " has a non-finite coordinate");
+    float* d_edges = nullptr;
+    uint8_t* d_out = nullptr;
+    const size_t edge_bytes = sizeof(float) * 2 * dim * (size_t)total;
+    hipError_t e = hipMalloc((void**)&d_edges, edge_bytes);
+    if (e == hipSuccess)
+        e = hipMalloc((void**)&d_out, (size_t)total);
+    if (e == hipSuccess)
+        e = hipMemcpy(d_edges, edges, edge_bytes, hipMemcpyHostToDevice);
+    int rc = BCC_OK;
+    bcc_ragged_plan* plan = nullptr;
+    if (e != hipSuccess)
+        rc = hip_fail(e, "ragged check setup");
+    else
+        rc = bcc_ragged_plan_create(robot, scene, env_offsets, d_edges, resolution, d_out, 0, &plan);
+    if (rc == BCC_OK) {
+        rc = bcc_ragged_plan_launch_eager(plan, nullptr);
+        if (rc == BCC_OK) {
+            e = hipMemcpy(results, d_out, (size_t)total, hipMemcpyDeviceToHost);
+            if (e != hipSuccess)
+                rc = hip_fail(e, "ragged check results");
+        }
+    }
+    bcc_ragged_plan_destroy_impl(plan);
+    (void)hipFree(d_edges);
+    (void)hipFree(d_out);
+    return rc;
 }
 
 int bcc_malloc(void** d_ptr, size_t bytes)

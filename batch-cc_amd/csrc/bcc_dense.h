@@ -127,6 +127,132 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
     }
 }
 
+// ------------------------------------------------------------ ragged batches
+// Ragged batch: env e owns the contiguous pairs [env_off[e], env_off[e+1])
+// (CSR); pair p checks edges[p] against environment e and writes out[p].
+// The pairs are cut into tiles (env, first pair, end pair) on the host.  The
+// kernel is persistent: every wave loops, taking the next tile from a global
+// counter; its two half-waves pull pairs from the tile through a per-wave
+// LDS counter, one step per iteration as in dense_kernel.  A wave grabs a new
+// tile only when both halves are idle, so the two halves always share one
+// environment (obstacle reads stay wave-uniform scalar loads).
+struct RaggedTile {
+    int env;
+    int begin;   // first pair
+    int end;     // one past the last pair
+    int pad;
+};
+
+template <class Robot>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENSE_WAVES, 8))) void ragged_kernel(
+    const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges,
+    const RaggedTile* __restrict__ tiles, int num_tiles, int* __restrict__ tile_counter, int resolution,
+    uint8_t* __restrict__ out)
+{
+    constexpr int D = Robot::dim;
+    constexpr int kWaves = kBlock / 64;
+    __shared__ int next_pair[kWaves];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int t = lane & 31;
+    const int half0 = lane & 32;
+
+    float q[D], d[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        q[j] = 0.0f;
+        d[j] = 0.0f;
+    }
+    for (;;) {
+        // -- take a tile (wave-uniform)
+        int tile = 0;
+        if (lane == 0)
+            tile = atomicAdd(tile_counter, 1);
+        tile = __builtin_amdgcn_readfirstlane(__shfl(tile, 0));
+        if (tile >= num_tiles)
+            return;
+        const RaggedTile T = tiles[tile];
+        const int env = __builtin_amdgcn_readfirstlane(T.env);
+        const int p_end = __builtin_amdgcn_readfirstlane(T.end);
+        if (lane == 0)
+            next_pair[wave] = T.begin;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const EnvView E = load_env(obs, meta, env);
+        int cur = -1, step = 0, n = 0;
+        bool exhausted = false;
+        for (;;) {
+            const bool want = (cur < 0) && !exhausted;
+            int g = 0;
+            if (want && t == 0)
+                g = atomicAdd(&next_pair[wave], 1);
+            g = __shfl(g, half0);
+            if (want) {
+                if (g < p_end) {
+                    cur = g;
+                    step = 0;
+                    n = setup_pair<D>(edges + (size_t)g * 2 * D, resolution, t, q, d);
+                } else {
+                    exhausted = true;
+                }
+            }
+            const uint64_t live = __ballot(cur >= 0);
+            if (live == 0)
+                break;
+            const uint64_t hit = Robot::check(q, E, live);
+            if (cur >= 0) {
+                const bool h = (hit >> lane) & 1ull;
+                bool fin = h;
+                if (!h) {
+                    ++step;
+                    fin = (step == n);
+                    if (!fin) {
+#pragma unroll
+                        for (int j = 0; j < D; ++j)
+                            q[j] = q[j] + d[j];
+                    }
+                }
+                if (fin) {
+                    if (t == 0)
+                        out[cur] = h ? 1 : 0;
+                    cur = -1;
+                }
+            }
+        }
+    }
+}
+
+// Resets the tile counter inside a captured graph (a kernel node rather than
+// a memset node).
+static __global__ void reset_counter_kernel(int* counter)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        *counter = 0;
+}
+
+struct RaggedArgs {
+    const float* obs;
+    const int* meta;
+    const float* edges;
+    const RaggedTile* tiles;
+    int num_tiles;
+    int* tile_counter;
+    int resolution;
+    int grid;        // persistent workgroups
+    uint8_t* out;
+};
+
+template <class Robot>
+hipError_t launch_ragged(const RaggedArgs& a, hipStream_t stream)
+{
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.tile_counter);
+    if (a.num_tiles == 0)
+        return hipGetLastError();
+    hipLaunchKernelGGL(ragged_kernel<Robot>, dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges, a.tiles,
+                       a.num_tiles, a.tile_counter, a.resolution, a.out);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ test hooks
 
 template <class Robot>
@@ -258,18 +384,21 @@ __global__ __launch_bounds__(kBlock) void work_kernel(const float* __restrict__ 
                                                       const int* __restrict__ ref_meta,
                                                       const float* __restrict__ edges, int num_edges, int num_envs,
                                                       int resolution, ModelDev model,
+                                                      const int* __restrict__ pair_env,
                                                       unsigned long long* __restrict__ work,
                                                       unsigned long long* __restrict__ trig,
                                                       uint8_t* __restrict__ out)
 {
+    // dense: pair p = edge * num_envs + env; with pair_env (ragged): env =
+    // pair_env[p], edge = p, and num_edges is the number of pairs
     constexpr int D = Robot::dim;
     const int lane = threadIdx.x & 63;
     const int t = lane & 31;
-    const long long pairs = (long long)num_edges * num_envs;
+    const long long pairs = pair_env ? (long long)num_edges : (long long)num_edges * num_envs;
     const long long halves = ((long long)gridDim.x * kBlock) >> 5;
     for (long long p = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 5; p < pairs; p += halves) {
-        const int edge = (int)(p / num_envs);
-        const int env = (int)(p - (long long)edge * num_envs);
+        const int edge = pair_env ? (int)p : (int)(p / num_envs);
+        const int env = pair_env ? pair_env[p] : (int)(p - (long long)edge * num_envs);
         const RefEnvView R = load_ref_env(ref_obs, ref_meta, env);
         float q[D], d[D];
         const int n = setup_pair<D>(edges + (size_t)edge * 2 * D, resolution, t, q, d);
@@ -297,6 +426,7 @@ struct WorkArgs {
     int num_envs;
     int resolution;
     ModelDev model;
+    const int* pair_env;
     unsigned long long* work;
     unsigned long long* trig;
     uint8_t* out;
@@ -307,10 +437,10 @@ hipError_t launch_work(const WorkArgs& a, hipStream_t stream)
 {
     if (a.num_edges == 0 || a.num_envs == 0)
         return hipSuccess;
-    const long long halves = (long long)a.num_edges * a.num_envs;
+    const long long halves = a.pair_env ? (long long)a.num_edges : (long long)a.num_edges * a.num_envs;
     const long long blocks = std::min<long long>((halves + 7) / 8, 65536);
     hipLaunchKernelGGL(work_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.ref_obs, a.ref_meta,
-                       a.edges, a.num_edges, a.num_envs, a.resolution, a.model, a.work, a.trig, a.out);
+                       a.edges, a.num_edges, a.num_envs, a.resolution, a.model, a.pair_env, a.work, a.trig, a.out);
     return hipGetLastError();
 }
 
@@ -366,7 +496,8 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
     hipError_t launch_fk_##name(const float* qs, int n, float* out, hipStream_t stream);                       \
     hipError_t launch_samples_##name(const float* edges, int m, int resolution, int* n_out, float* cfg0,       \
                                      float* cfg_last, hipStream_t stream);                                     \
-    hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream);
+    hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream);                                     \
+    hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream);
 
 #define BCC_DEFINE_ROBOT(name, Robot)                                                                          \
     hipError_t launch_dense_##name(const DenseArgs& a, hipStream_t stream)                                     \
@@ -385,6 +516,10 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
     hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream)                                       \
     {                                                                                                          \
         return launch_work<Robot>(a, stream);                                                                  \
+    }                                                                                                          \
+    hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream)                                   \
+    {                                                                                                          \
+        return launch_ragged<Robot>(a, stream);                                                                \
     }
 
 BCC_DECLARE_ROBOT(panda)

@@ -20,6 +20,7 @@ Configurations (sizes from the benchmark definition):
   c2  baxter 4096 envs, Zipf(1.2) edges per env on 1..4096, U{8..256}
              obstacles with 20% capsules (ragged)
   c3  panda  1000 envs x 100000 edges, c0 distributions
+The ragged batch c2 comes from make_ragged().
 """
 from __future__ import annotations
 
@@ -127,6 +128,31 @@ def lognormal_edges(robot: str, m: int, rng, median=0.5, sigma=0.8) -> np.ndarra
     length = rng.lognormal(np.log(median), sigma, m)
     end = np.clip(start + d * length[:, None], lo, hi).astype(np.float32)
     return np.stack([start, end], 1).astype(np.float32)
+
+
+def zipf_counts(rng, n: int, a: float = 1.2, kmax: int = 4096) -> np.ndarray:
+    """n draws of a Zipf(a) law truncated to 1..kmax."""
+    k = np.arange(1, kmax + 1, dtype=np.float64)
+    p = k ** -a
+    p /= p.sum()
+    return rng.choice(np.arange(1, kmax + 1), size=n, p=p).astype(np.int64)
+
+
+def make_ragged(name: str, seed: int = 0, scale: float = 1.0):
+    """Ragged batches: returns (robot, PackedScene, env_off[E+1], edges[total,2,dim]).
+    c2: baxter, 4096 envs, Zipf(1.2) edges per env on 1..4096, U{8..256}
+    obstacles per env with 20% capsules, uniform endpoints."""
+    rs = np.random.default_rng([seed, 1])
+    re = np.random.default_rng([seed, 2])
+    if name != "c2":
+        raise ValueError(f"unknown ragged config {name}")
+    E = max(1, int(4096 * scale))
+    counts = zipf_counts(re, E)
+    envs = [mixed_env(rs, int(rs.integers(8, 257)), capsule_frac=0.2) for _ in range(E)]
+    off = np.zeros(E + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    edges = uniform_edges("baxter", int(off[-1]), re)
+    return "baxter", PackedScene.from_envs(envs), off, edges
 
 
 def make_config(name: str, seed: int = 0, scale: float = 1.0):

@@ -72,12 +72,39 @@ def cpu_baseline(robot, sc, edges, seconds=10.0, max_threads=16):
     }
 
 
+def cpu_baseline_pairs(robot, sc, edges, pair_env, seconds=10.0, max_threads=16):
+    """CPU baseline on a ragged batch: a seeded random sample of its pairs."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib
+    orc = oracle_lib.Oracle()
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(cores, max_threads))
+    total = edges.shape[0]
+    rid = {"panda": 0, "fetch": 1, "baxter": 2}[robot]
+    rng = np.random.default_rng(1234)
+    n = 500
+    while True:
+        idx = rng.integers(0, total, n).astype(np.int32)
+        t0 = time.perf_counter()
+        orc.check_pairs(rid, sc, edges, pair_env[idx], idx, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= seconds or n >= total:
+            break
+        n = int(min(total, max(n * 2, n * seconds / max(dt, 1e-3) * 1.1)))
+    return {"value": n / dt, "unit": "edge-env checks/s", "cores": threads, "kind": "port",
+            "sample": f"{n} random pairs of the ragged batch, {dt:.1f} s, C restatement of the reference "
+                      f"checker (vamp unavailable), {threads} threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c3"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3"])
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--scale", type=float, default=1.0, help="env/edge count multiplier (1.0 = full size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -105,28 +132,46 @@ def main():
     from batch_cc_amd import workloads as W
 
     t0 = time.perf_counter()
-    robot, sc, edges = W.make_config(a.config, seed=a.seed, scale=a.scale)
-    E, M = sc.num_envs, edges.shape[0]
-    pairs = E * M
-    print(f"[bench] workload {a.config}: {robot}, {E} envs x {M} edges = {pairs} pairs "
+    ragged = a.config == "c2"
+    if ragged:
+        robot, sc, env_off, edges = W.make_ragged(a.config, seed=a.seed, scale=a.scale)
+        E, M = sc.num_envs, edges.shape[0]
+        pairs = M
+        shape = (M,)
+        desc = f"{robot}, {E} envs, {M} ragged pairs (Zipf(1.2) edges per env)"
+    else:
+        robot, sc, edges = W.make_config(a.config, seed=a.seed, scale=a.scale)
+        E, M = sc.num_envs, edges.shape[0]
+        pairs = E * M
+        shape = (M, E)
+        desc = f"{robot}, {E} envs x {M} edges"
+    print(f"[bench] workload {a.config}: {desc} = {pairs} pairs "
           f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
 
     scene = bcc.Scene(sc)
     d_edges = bcc.DeviceBuffer.from_array(edges)
     d_out = bcc.DeviceBuffer(pairs)
     d_out.zero()
-    bcc.check_dense_device(robot, scene, d_edges, M, 32, d_out)
+    plan = None
+    if ragged:
+        plan = bcc.RaggedPlan(robot, scene, env_off, d_edges, d_out)
+        plan.launch()
+    else:
+        bcc.check_dense_device(robot, scene, d_edges, M, 32, d_out)
     bcc.synchronize()
-    verdicts = d_out.to_array(np.uint8, (M, E))
+    verdicts = d_out.to_array(np.uint8, shape)
     coll = float(verdicts.mean())
 
     wall0 = time.perf_counter()
-    ms = bcc.time_dense(robot, scene, d_edges, M, 32, d_out, a.warmup, a.steps)
+    if ragged:
+        ms = plan.time(a.warmup, a.steps)
+    else:
+        ms = bcc.time_dense(robot, scene, d_edges, M, 32, d_out, a.warmup, a.steps)
     bcc.synchronize()
     wall = time.perf_counter() - wall0
     print(f"[bench] {a.steps} timed launches: {ms:.3f} ms each (wall incl. warmup {wall:.2f} s)",
           file=sys.stderr, flush=True)
-    again = d_out.to_array(np.uint8, (M, E))
+    again = d_out.to_array(np.uint8, shape)
     if not np.array_equal(again, verdicts):
         print("[bench] verdicts changed between launches", file=sys.stderr)
         sys.exit(1)
@@ -135,7 +180,11 @@ def main():
     roofline = None
     if not a.no_work:
         t1 = time.perf_counter()
-        work, trig, wv = bcc.work_dense(robot, scene, d_edges, M, 32)
+        if ragged:
+            pair_env = np.repeat(np.arange(E, dtype=np.int32), np.diff(env_off))
+            work, trig, wv = bcc.work_pairs(robot, scene, d_edges, pair_env, 32)
+        else:
+            work, trig, wv = bcc.work_dense(robot, scene, d_edges, M, 32)
         if not np.array_equal(wv, verdicts):
             print("[bench] work-counter verdicts differ from the dense kernel", file=sys.stderr)
             sys.exit(1)
@@ -149,7 +198,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "dense_kernel",
+            "kernel": "ragged_kernel (hipGraph replay)" if ragged else "dense_kernel",
             "algorithmic_flops_per_launch": flops,
             "sincos_per_launch": trigs,
             "note": "reference source-level FP32 ops in plain order, counted on the GPU by the "
@@ -159,8 +208,11 @@ def main():
               file=sys.stderr, flush=True)
 
     cpu = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and not ragged:
         cpu = cpu_baseline(robot, sc, edges)
+    elif not a.no_cpu_baseline:
+        cpu = cpu_baseline_pairs(robot, sc, edges, np.repeat(np.arange(E, dtype=np.int32), np.diff(env_off)))
+    if cpu is not None:
         print(f"[bench] cpu baseline {cpu['value']:.4g} pairs/s on {cpu['cores']} threads", file=sys.stderr,
               flush=True)
 
@@ -178,10 +230,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{a.config}: {robot}, {E} envs x {M} edges, seed {a.seed}",
+            "workload": f"{a.config}: {desc}, seed {a.seed}",
             "robot": robot,
             "envs": E,
             "edges": M,
+            "layout": "ragged CSR, one persistent launch replayed as a hipGraph" if ragged else "dense E x M",
             "pairs": pairs,
             "resolution": 32,
             "order": "plain",

@@ -131,6 +131,38 @@ int bcc_time_dense(int robot, const bcc_scene *scene, const float *d_edges, int 
 int bcc_work_dense(int robot, const bcc_scene *scene, const float *d_edges, int num_edges, int resolution,
                    uint64_t *d_work, uint64_t *d_trig, uint8_t *d_results, void *stream);
 
+/* Same accounting for an explicit pair list: pair p checks edge p against
+ * environment d_pair_env[p]; arrays are [num_pairs]. */
+int bcc_work_pairs(int robot, const bcc_scene *scene, const float *d_edges, int num_pairs, const int32_t *d_pair_env,
+                   int resolution, uint64_t *d_work, uint64_t *d_trig, uint8_t *d_results, void *stream);
+
+/*
+ * Ragged batches (no reference counterpart: the reference only takes the
+ * dense cross product, src/batch_cc.cu:244-261).  Environment e owns pairs
+ * [env_offsets[e], env_offsets[e+1]) (CSR, host array of num_envs + 1);
+ * pair p checks edges[p] ([2][dim] floats) against environment e and writes
+ * results[p] (1 = collision).  Same verdict semantics as bcc_check_dense.
+ */
+int bcc_check_ragged(int robot, const bcc_scene *scene, const int64_t *env_offsets, const float *edges,
+                     int resolution, uint8_t *results);
+
+/*
+ * A ragged plan owns the tile list (env, pair range) of one CSR layout and a
+ * captured hipGraph (tile-counter reset + one persistent kernel launch) over
+ * caller-owned device buffers d_edges / d_results.  tile_pairs = 0 picks the
+ * tuned default.  The plan keeps pointers to the scene and buffers: they
+ * must outlive it.  Launches of one plan must not overlap (they share the
+ * tile counter).
+ */
+typedef struct bcc_ragged_plan bcc_ragged_plan;
+int bcc_ragged_plan_create(int robot, const bcc_scene *scene, const int64_t *env_offsets, const float *d_edges,
+                           int resolution, uint8_t *d_results, int tile_pairs, bcc_ragged_plan **out);
+int bcc_ragged_plan_launch(bcc_ragged_plan *plan, void *stream);        /* graph replay */
+int bcc_ragged_plan_launch_eager(bcc_ragged_plan *plan, void *stream);  /* same kernels, no graph */
+int bcc_ragged_plan_num_tiles(const bcc_ragged_plan *plan);
+int bcc_ragged_plan_time(bcc_ragged_plan *plan, int warmup, int iters, void *stream, double *avg_ms);
+int bcc_ragged_plan_destroy(bcc_ragged_plan *plan);
+
 /* Device memory helpers for bindings that do not bring their own allocator. */
 int bcc_malloc(void **d_ptr, size_t bytes);
 int bcc_free(void *d_ptr);
