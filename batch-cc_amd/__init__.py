@@ -76,6 +76,23 @@ def lib():
     _sig(L, "bcc_ragged_plan_num_tiles", I, [P])
     _sig(L, "bcc_ragged_plan_time", I, [P, I, I, P, ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_ragged_plan_destroy", I, [P])
+    _sig(L, "bcc_check_dense_ex", I, [I, P, P, I, I, P, P, P, P])
+    _sig(L, "bcc_time_dense_ex", I, [I, P, P, I, I, P, P, I, I, P, ctypes.POINTER(ctypes.c_double)])
+    _sig(L, "bcc_model_num_params", I, [])
+    _sig(L, "bcc_model_create", I, [I, ctypes.c_uint64, ctypes.POINTER(P)])
+    _sig(L, "bcc_model_destroy", I, [P])
+    _sig(L, "bcc_model_get_params", I, [P, P])
+    _sig(L, "bcc_model_set_params", I, [P, P])
+    _sig(L, "bcc_model_save", I, [P, ctypes.c_char_p])
+    _sig(L, "bcc_model_load", I, [ctypes.c_char_p, ctypes.POINTER(P)])
+    _sig(L, "bcc_encode_envs", I, [P, P])
+    _sig(L, "bcc_model_features", I, [P, P, P, ctypes.c_int64, P, P, P, P])
+    _sig(L, "bcc_label_pairs", I, [I, P, P, I, P, P, I, P, P])
+    _sig(L, "bcc_model_grads", I, [P, P, P, I, P, ctypes.POINTER(ctypes.c_float)])
+    _sig(L, "bcc_model_train", I, [P, P, P, I, I, I, ctypes.c_float, P])
+    _sig(L, "bcc_model_logits", I, [P, P, ctypes.c_int64, P])
+    _sig(L, "bcc_model_order_dense", I, [P, P, P, I, P, P])
+    _sig(L, "bcc_debug_gemm", I, [I, I, P, I, P, I, P, I, I, I, I, ctypes.c_float, ctypes.c_float])
     _sig(L, "bcc_malloc", I, [ctypes.POINTER(P), ctypes.c_size_t])
     _sig(L, "bcc_free", I, [P])
     _sig(L, "bcc_memcpy_htod", I, [P, P, ctypes.c_size_t])
@@ -386,3 +403,133 @@ def debug_prims(inp: np.ndarray) -> np.ndarray:
 
 def debug_copy(d_src: DeviceBuffer, d_dst: DeviceBuffer, nbytes: int):
     _check(lib().bcc_debug_copy(d_src.ptr, d_dst.ptr, int(nbytes)))
+
+
+# ------------------------------------------------------------ learned ordering
+
+def check_dense_ex(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, resolution: int,
+                   d_out: DeviceBuffer, d_order: DeviceBuffer | None = None, d_rounds: DeviceBuffer | None = None,
+                   stream=None):
+    _check(lib().bcc_check_dense_ex(robot_id(robot), scene.handle, d_edges.ptr, int(num_edges), int(resolution),
+                                    d_order.ptr if d_order else None, d_rounds.ptr if d_rounds else None,
+                                    d_out.ptr, stream))
+
+
+def time_dense_ex(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, resolution: int, d_out: DeviceBuffer,
+                  d_order: DeviceBuffer | None, warmup: int, iters: int, stream=None) -> float:
+    ms = ctypes.c_double(0.0)
+    _check(lib().bcc_time_dense_ex(robot_id(robot), scene.handle, d_edges.ptr, int(num_edges), int(resolution),
+                                   d_order.ptr if d_order else None, d_out.ptr, int(warmup), int(iters), stream,
+                                   ctypes.byref(ms)))
+    return ms.value
+
+
+def label_pairs(robot, scene: Scene, d_edges: DeviceBuffer, pair_env, pair_edge, resolution: int = 32):
+    """Bin masks (uint32 per pair) from the exact checker."""
+    pe = np.ascontiguousarray(pair_env, np.int32)
+    pg = np.ascontiguousarray(pair_edge, np.int32)
+    n = pe.size
+    dpe, dpg = DeviceBuffer.from_array(pe), DeviceBuffer.from_array(pg)
+    dl = DeviceBuffer(n * 4)
+    _check(lib().bcc_label_pairs(robot_id(robot), scene.handle, d_edges.ptr, n, dpe.ptr, dpg.ptr, int(resolution),
+                                 dl.ptr, None))
+    synchronize()
+    return dl
+
+
+def encode_envs(scene: Scene) -> np.ndarray:
+    d = DeviceBuffer(scene.num_envs * 64 * 4)
+    _check(lib().bcc_encode_envs(scene.handle, d.ptr))
+    return d.to_array(np.float32, (scene.num_envs, 64))
+
+
+class Model:
+    """The learned step-ordering MLP (bcc_model)."""
+    IN, HID, OUT = 96, 256, 32
+
+    def __init__(self, robot=None, seed: int = 0, handle=None):
+        self.handle = ctypes.c_void_p()
+        if handle is not None:
+            self.handle = handle
+        else:
+            _check(lib().bcc_model_create(robot_id(robot), int(seed), ctypes.byref(self.handle)))
+
+    @staticmethod
+    def num_params() -> int:
+        return lib().bcc_model_num_params()
+
+    @staticmethod
+    def load(path: str) -> "Model":
+        h = ctypes.c_void_p()
+        _check(lib().bcc_model_load(path.encode(), ctypes.byref(h)))
+        return Model(handle=h)
+
+    def save(self, path: str):
+        _check(lib().bcc_model_save(self.handle, path.encode()))
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), np.float32)
+        _check(lib().bcc_model_get_params(self.handle, _ptr(out)))
+        return out
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().bcc_model_set_params(self.handle, _ptr(p)))
+
+    def features(self, scene: Scene, d_edges: DeviceBuffer, pair_env, pair_edge) -> DeviceBuffer:
+        pe = np.ascontiguousarray(pair_env, np.int32)
+        pg = np.ascontiguousarray(pair_edge, np.int32)
+        dpe, dpg = DeviceBuffer.from_array(pe), DeviceBuffer.from_array(pg)
+        dx = DeviceBuffer(pe.size * self.IN * 2)
+        _check(lib().bcc_model_features(self.handle, scene.handle, d_edges.ptr, pe.size, dpe.ptr, dpg.ptr, dx.ptr,
+                                        None))
+        return dx
+
+    def grads(self, d_x: DeviceBuffer, d_labels: DeviceBuffer, rows: int):
+        dg = DeviceBuffer(self.num_params() * 4)
+        loss = ctypes.c_float(0.0)
+        _check(lib().bcc_model_grads(self.handle, d_x.ptr, d_labels.ptr, int(rows), dg.ptr, ctypes.byref(loss)))
+        return dg.to_array(np.float32, (self.num_params(),)), loss.value
+
+    def train(self, d_x: DeviceBuffer, d_labels: DeviceBuffer, rows: int, epochs: int, batch: int = 4096,
+              lr: float = 1e-3) -> np.ndarray:
+        losses = np.zeros(epochs, np.float32)
+        _check(lib().bcc_model_train(self.handle, d_x.ptr, d_labels.ptr, int(rows), int(epochs), int(batch),
+                                     float(lr), _ptr(losses)))
+        return losses
+
+    def logits(self, d_x: DeviceBuffer, rows: int) -> np.ndarray:
+        dl = DeviceBuffer(rows * self.OUT * 4)
+        _check(lib().bcc_model_logits(self.handle, d_x.ptr, int(rows), dl.ptr))
+        return dl.to_array(np.float32, (rows, self.OUT))
+
+    def order_dense(self, scene: Scene, d_edges: DeviceBuffer, num_edges: int) -> DeviceBuffer:
+        d = DeviceBuffer(int(num_edges) * scene.num_envs * 4)
+        _check(lib().bcc_model_order_dense(self.handle, scene.handle, d_edges.ptr, int(num_edges), d.ptr, None))
+        synchronize()
+        return d
+
+    def close(self):
+        if self.handle:
+            _check(lib().bcc_model_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def debug_gemm(ta: bool, tb: bool, a: np.ndarray, b: np.ndarray, m: int, n: int, k: int, alpha=1.0, beta=0.0,
+               c: np.ndarray | None = None) -> np.ndarray:
+    """a, b: uint16 BF16 bit patterns (row-major, shapes as stored)."""
+    da, db = DeviceBuffer.from_array(np.ascontiguousarray(a, np.uint16)), DeviceBuffer.from_array(
+        np.ascontiguousarray(b, np.uint16))
+    c0 = np.zeros((m, n), np.float32) if c is None else np.ascontiguousarray(c, np.float32)
+    dc = DeviceBuffer.from_array(c0)
+    lda = a.shape[1]
+    ldb = b.shape[1]
+    _check(lib().bcc_debug_gemm(int(ta), int(tb), da.ptr, lda, db.ptr, ldb, dc.ptr, n, m, n, k, float(alpha),
+                                float(beta)))
+    return dc.to_array(np.float32, (m, n))

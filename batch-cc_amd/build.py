@@ -29,7 +29,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["bcc_api.hip"] + [f"robot_{r}.hip" for r in ROBOTS]
+SOURCES = ["bcc_api.hip", "bcc_mlp.hip"] + [f"robot_{r}.hip" for r in ROBOTS]
 
 
 def run(cmd, **kw):

@@ -21,11 +21,12 @@
 
 #include "../../include/bcc.h"
 #include "bcc_dense.h"
+#include "bcc_internal.h"
 #include "robots/baxter_model.h"
 #include "robots/fetch_model.h"
 #include "robots/panda_model.h"
 
-namespace {
+namespace bcc_detail {
 
 thread_local std::string g_err = "no error";
 
@@ -40,26 +41,18 @@ int hip_fail(hipError_t e, const char* what)
     return fail(BCC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define BCC_HIP(call, what)                       \
-    do {                                          \
-        hipError_t _e = (call);                   \
-        if (_e != hipSuccess)                     \
-            return hip_fail(_e, what);            \
-    } while (0)
+}  // namespace bcc_detail
+
+using bcc_detail::fail;
+using bcc_detail::g_err;
+using bcc_detail::hip_fail;
+
+namespace {
 
 constexpr int kStride[6] = {4, 8, 8, 8, 15, 15};  // packed host layout
 
 }  // namespace
 
-struct bcc_scene {
-    int num_envs = 0;
-    float* d_obs = nullptr;      // all environments, device layout (see bcc_device.h)
-    int* d_meta = nullptr;       // [num_envs][kEnvMetaInts]
-    float* d_ref_obs = nullptr;  // the reference's lists and order, for the work counter
-    int* d_ref_meta = nullptr;
-    size_t obs_floats = 0;
-    std::vector<int> obstacles;  // tested primitives per environment (host copy)
-};
 
 namespace {
 
@@ -431,8 +424,31 @@ int bcc_check_dense(int robot, const bcc_scene* scene, const float* edges, int n
     return rc;
 }
 
+int bcc_check_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
+                       const uint32_t* d_order, uint8_t* d_rounds, uint8_t* d_results, void* stream)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
+    a.order = d_order;
+    a.rounds = d_rounds;
+    return dispatch_dense(robot, a, (hipStream_t)stream);
+}
+
+int bcc_time_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
+                      const uint32_t* d_order, uint8_t* d_results, int warmup, int iters, void* stream,
+                      double* avg_ms);
+
 int bcc_time_dense(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
                    uint8_t* d_results, int warmup, int iters, void* stream, double* avg_ms)
+{
+    return bcc_time_dense_ex(robot, scene, d_edges, num_edges, resolution, nullptr, d_results, warmup, iters, stream,
+                             avg_ms);
+}
+
+int bcc_time_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
+                      const uint32_t* d_order, uint8_t* d_results, int warmup, int iters, void* stream,
+                      double* avg_ms)
 {
     if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
         return rc;
@@ -440,6 +456,7 @@ int bcc_time_dense(int robot, const bcc_scene* scene, const float* d_edges, int 
         return fail(BCC_ERR_INVALID, "bad timing arguments");
     hipStream_t s = (hipStream_t)stream;
     bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
+    a.order = d_order;
     for (int i = 0; i < warmup; ++i)
         if (int rc = dispatch_dense(robot, a, s))
             return rc;
@@ -467,7 +484,9 @@ int bcc_time_dense(int robot, const bcc_scene* scene, const float* d_edges, int 
     return BCC_OK;
 }
 
-namespace {
+}  // extern "C"
+
+namespace bcc_detail {
 
 const bcc_robot_model* host_model(int robot)
 {
@@ -550,7 +569,11 @@ int device_model(int robot, bcc::ModelDev* out)
     return BCC_OK;
 }
 
-}  // namespace
+}  // namespace bcc_detail
+
+using bcc_detail::device_model;
+
+extern "C" {
 
 static int work_common(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, const int* d_pair_env,
                        int resolution, uint64_t* d_work, uint64_t* d_trig, uint8_t* d_results, void* stream);

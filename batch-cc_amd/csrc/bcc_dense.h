@@ -56,10 +56,90 @@ __device__ __forceinline__ int setup_pair(const float* __restrict__ se, int reso
 #define BCC_DENSE_WAVES 1
 #endif
 
+// Sample of round r, lane t under a learned order.  Round 0 is plain step 0
+// (the first sample of every bin, i.e. one sample every 1/31 of the edge);
+// then come the remaining samples of the four predicted bins (top, one byte
+// per rank), bin by bin, then the remaining samples of the other bins in
+// plain step order.  Every one of the 32 n samples appears exactly once over
+// the n rounds, each computed exactly as in plain order: start + delta *
+// (bin * n) (fmaf), then i additions of delta.  An order that does not name
+// four distinct bins below 32 (top = 0xffffffff after validation) means
+// plain order.
+template <int D>
+__device__ __forceinline__ void ordered_sample(const float (&s)[D], const float (&d)[D], int n, uint32_t top,
+                                               int r, int t, float (&q)[D])
+{
+    int bin, i;
+    if (top == 0xffffffffu || r == 0) {
+        bin = t;
+        i = r;
+    } else {
+        const int j = 32 * (r - 1) + t;
+        const int a = 4 * (n - 1);
+        if (j < a) {
+            const int k = j / (n - 1);
+            bin = (top >> (8 * k)) & 31;
+            i = 1 + j - k * (n - 1);
+        } else {
+            const int j2 = j - a;
+            i = 1 + j2 / 28;
+            int m = j2 - (i - 1) * 28;   // m-th bin outside the top four, ascending
+            const uint32_t taken = (1u << (top & 31)) | (1u << ((top >> 8) & 31)) | (1u << ((top >> 16) & 31)) |
+                                   (1u << ((top >> 24) & 31));
+            bin = 0;
+            for (int b = 0; b < 32; ++b) {
+                if ((taken >> b) & 1u)
+                    continue;
+                if (m == 0) {
+                    bin = b;
+                    break;
+                }
+                --m;
+            }
+        }
+    }
+    const float bn = (float)(bin * n);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        float c = fmaf(d[j], bn, s[j]);
+        for (int k = 0; k < i; ++k)
+            c = c + d[j];
+        q[j] = c;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ int setup_pair_sd(const float* __restrict__ se, int resolution, float (&s)[D],
+                                             float (&d)[D])
+{
+    float e[D];
+    float acc = 0.0f;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        s[j] = se[j];
+        e[j] = se[D + j];
+        const float diff = s[j] - e[j];
+        acc = fmaf(diff, diff, acc);
+    }
+    const float dist = sqrtf(acc);
+    const float nf = fminf(fmaxf(ceilf((dist / 31.0f) * (float)resolution), 1.0f), 4194304.0f);
+    const int n = (int)nf;
+    const float div = (float)(31 * n);
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        d[j] = (e[j] - s[j]) / div;
+    return n;
+}
+
+// order (optional): learned top-4 bins per pair, [edge][env]; when given the
+// half visits samples in ordered_sample order, otherwise in plain order.
+// rounds (optional): number of rounds (steps) each pair ran, [edge][env].
 template <class Robot>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENSE_WAVES, 8))) void dense_kernel(const float* __restrict__ obs, const int* __restrict__ meta,
                                                        const float* __restrict__ edges, int num_edges, int num_envs,
                                                        int resolution, int chunk, int chunks_per_env,
+                                                       const uint32_t* __restrict__ order,
+                                                       uint8_t* __restrict__ rounds,
                                                        uint8_t* __restrict__ out)
 {
     constexpr int D = Robot::dim;
@@ -78,13 +158,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
     const int t = lane & 31;
     const int half0 = lane & 32;
 
-    float q[D], d[D];
+    float q[D], d[D], s0[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
         q[j] = 0.0f;
         d[j] = 0.0f;
+        s0[j] = 0.0f;
     }
     int cur = -1, step = 0, n = 0;
+    uint32_t top = 0;
     bool exhausted = false;
 
     for (;;) {
@@ -97,7 +179,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
             if (g < e_end) {
                 cur = g;
                 step = 0;
-                n = setup_pair<D>(edges + (size_t)g * 2 * D, resolution, t, q, d);
+                if (order) {
+                    n = setup_pair_sd<D>(edges + (size_t)g * 2 * D, resolution, s0, d);
+                    top = order[(size_t)g * num_envs + env];
+                    const uint32_t taken = (1u << (top & 31)) | (1u << ((top >> 8) & 31)) |
+                                           (1u << ((top >> 16) & 31)) | (1u << ((top >> 24) & 31));
+                    if (__popc(taken) != 4 || (top & 0xe0e0e0e0u))
+                        top = 0xffffffffu;
+                    ordered_sample<D>(s0, d, n, top, 0, t, q);
+                } else {
+                    n = setup_pair<D>(edges + (size_t)g * 2 * D, resolution, t, q, d);
+                }
             } else {
                 exhausted = true;
             }
@@ -113,14 +205,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
                 ++step;
                 fin = (step == n);
                 if (!fin) {
+                    if (order) {
+                        ordered_sample<D>(s0, d, n, top, step, t, q);
+                    } else {
 #pragma unroll
-                    for (int j = 0; j < D; ++j)
-                        q[j] = q[j] + d[j];
+                        for (int j = 0; j < D; ++j)
+                            q[j] = q[j] + d[j];
+                    }
                 }
             }
             if (fin) {
-                if (t == 0)
+                if (t == 0) {
                     out[(size_t)cur * num_envs + env] = h ? 1 : 0;
+                    if (rounds)
+                        rounds[(size_t)cur * num_envs + env] = (uint8_t)min(h ? step + 1 : step, 255);
+                }
                 cur = -1;
             }
         }
@@ -418,6 +517,70 @@ __global__ __launch_bounds__(kBlock) void work_kernel(const float* __restrict__ 
     }
 }
 
+// Training labels: for each pair, bit t of labels[p] is set when bin t (lane
+// t's n samples) holds a colliding sample.  Every sample is evaluated (no
+// early exit) with the reference-order replay.  Pair p: env = pair_env[p],
+// edge = pair_edge[p].
+template <class Robot>
+__global__ __launch_bounds__(kBlock) void label_kernel(const float* __restrict__ ref_obs,
+                                                       const int* __restrict__ ref_meta,
+                                                       const float* __restrict__ edges, int num_pairs,
+                                                       int resolution, ModelDev model,
+                                                       const int* __restrict__ pair_env,
+                                                       const int* __restrict__ pair_edge,
+                                                       uint32_t* __restrict__ labels)
+{
+    constexpr int D = Robot::dim;
+    const int lane = threadIdx.x & 63;
+    const int t = lane & 31;
+    const long long halves = ((long long)gridDim.x * kBlock) >> 5;
+    for (long long p = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 5; p < num_pairs; p += halves) {
+        const int env = pair_env[p];
+        const int edge = pair_edge[p];
+        const RefEnvView R = load_ref_env(ref_obs, ref_meta, env);
+        float s[D], d[D], q[D];
+        const int n = setup_pair_sd<D>(edges + (size_t)edge * 2 * D, resolution, s, d);
+        const float tn = (float)(t * n);
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            q[j] = fmaf(d[j], tn, s[j]);
+        bool any = false;
+        uint64_t w = 0, tr = 0;
+        for (int i = 0; i < n; ++i) {
+            any = ref_sample_work<Robot>(q, R, model, w, tr) || any;
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                q[j] = q[j] + d[j];
+        }
+        const uint32_t bits = (uint32_t)(__ballot(any) >> (lane & 32));
+        if (t == 0)
+            labels[p] = bits;
+    }
+}
+
+struct LabelArgs {
+    const float* ref_obs;
+    const int* ref_meta;
+    const float* edges;
+    int num_pairs;
+    int resolution;
+    ModelDev model;
+    const int* pair_env;
+    const int* pair_edge;
+    uint32_t* labels;
+};
+
+template <class Robot>
+hipError_t launch_label(const LabelArgs& a, hipStream_t stream)
+{
+    if (a.num_pairs == 0)
+        return hipSuccess;
+    const long long blocks = std::min<long long>(((long long)a.num_pairs + 7) / 8, 65536);
+    hipLaunchKernelGGL(label_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.ref_obs, a.ref_meta,
+                       a.edges, a.num_pairs, a.resolution, a.model, a.pair_env, a.pair_edge, a.labels);
+    return hipGetLastError();
+}
+
 struct WorkArgs {
     const float* ref_obs;
     const int* ref_meta;
@@ -452,6 +615,8 @@ struct DenseArgs {
     int num_envs;
     int resolution;
     uint8_t* out;
+    const uint32_t* order = nullptr;   // learned top-4 bins per pair, or null (plain order)
+    uint8_t* rounds = nullptr;         // rounds per pair, or null
 };
 
 template <class Robot>
@@ -465,7 +630,7 @@ hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
     hipLaunchKernelGGL(dense_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
-                       a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.out);
+                       a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order, a.rounds, a.out);
     return hipGetLastError();
 }
 
@@ -497,7 +662,8 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
     hipError_t launch_samples_##name(const float* edges, int m, int resolution, int* n_out, float* cfg0,       \
                                      float* cfg_last, hipStream_t stream);                                     \
     hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream);                                     \
-    hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream);
+    hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream);                                 \
+    hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream);
 
 #define BCC_DEFINE_ROBOT(name, Robot)                                                                          \
     hipError_t launch_dense_##name(const DenseArgs& a, hipStream_t stream)                                     \
@@ -520,6 +686,10 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
     hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream)                                   \
     {                                                                                                          \
         return launch_ragged<Robot>(a, stream);                                                                \
+    }                                                                                                          \
+    hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream)                                     \
+    {                                                                                                          \
+        return launch_label<Robot>(a, stream);                                                                 \
     }
 
 BCC_DECLARE_ROBOT(panda)

@@ -1,0 +1,38 @@
+// Internals shared by the host translation units of libbcc (not part of the
+// C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bcc.h"
+#include "bcc_dense.h"
+#include "robots/bcc_model.h"
+
+struct bcc_scene {
+    int num_envs = 0;
+    float* d_obs = nullptr;      // all environments, device layout (see bcc_device.h)
+    int* d_meta = nullptr;       // [num_envs][kEnvMetaInts]
+    float* d_ref_obs = nullptr;  // the reference's lists and order, for the work counter
+    int* d_ref_meta = nullptr;
+    size_t obs_floats = 0;
+    std::vector<int> obstacles;  // tested primitives per environment (host copy)
+};
+
+namespace bcc_detail {
+
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+const bcc_robot_model* host_model(int robot);
+int device_model(int robot, bcc::ModelDev* out);
+
+}  // namespace bcc_detail
+
+#define BCC_HIP(call, what)                                  \
+    do {                                                     \
+        hipError_t _e = (call);                              \
+        if (_e != hipSuccess)                                \
+            return bcc_detail::hip_fail(_e, what);           \
+    } while (0)

@@ -163,6 +163,61 @@ int bcc_ragged_plan_num_tiles(const bcc_ragged_plan *plan);
 int bcc_ragged_plan_time(bcc_ragged_plan *plan, int warmup, int iters, void *stream, double *avg_ms);
 int bcc_ragged_plan_destroy(bcc_ragged_plan *plan);
 
+/*
+ * Dense check with options (device buffers):
+ *   d_order  (optional) learned visiting order per pair, [num_edges][num_envs]
+ *            uint32: four distinct sample bins (0..31), one byte per rank.
+ *            Step 0 is checked as in plain order, then the remaining samples
+ *            of those bins, then the rest in plain order; any other value
+ *            means plain order.  Verdicts do not depend on the order.
+ *   d_rounds (optional) rounds (32-sample steps) each pair ran, uint8.
+ */
+int bcc_check_dense_ex(int robot, const bcc_scene *scene, const float *d_edges, int num_edges, int resolution,
+                       const uint32_t *d_order, uint8_t *d_rounds, uint8_t *d_results, void *stream);
+int bcc_time_dense_ex(int robot, const bcc_scene *scene, const float *d_edges, int num_edges, int resolution,
+                      const uint32_t *d_order, uint8_t *d_results, int warmup, int iters, void *stream,
+                      double *avg_ms);
+
+/*
+ * Learned step ordering (no reference counterpart).  A bcc_model is a
+ * 3-layer MLP (96 -> 256 -> 256 -> 32) over a 64-cell max-pooled obstacle
+ * occupancy code of the environment and the normalised edge endpoints,
+ * predicting which of the 32 sample bins of a pair hold a collision.  BF16
+ * weights and activations, FP32 accumulation on MFMA, FP32 master weights
+ * and Adam state; trained on the GPU from labels the exact checker writes.
+ * Parameter order: W1 [96][256], b1, W2 [256][256], b2, W3 [256][32], b3.
+ */
+typedef struct bcc_model bcc_model;
+int bcc_model_num_params(void);
+int bcc_model_create(int robot, uint64_t seed, bcc_model **out);
+int bcc_model_destroy(bcc_model *model);
+int bcc_model_get_params(const bcc_model *model, float *host_params);
+int bcc_model_set_params(bcc_model *model, const float *host_params);
+int bcc_model_save(const bcc_model *model, const char *path);      /* params + Adam state */
+int bcc_model_load(const char *path, bcc_model **out);
+/* 64-cell occupancy code per environment: d_enc [num_envs][64] */
+int bcc_encode_envs(const bcc_scene *scene, float *d_enc);
+/* BF16 input rows [num_pairs][96] for pairs (d_pair_env[p], d_pair_edge[p]) */
+int bcc_model_features(const bcc_model *model, const bcc_scene *scene, const float *d_edges, int64_t num_pairs,
+                       const int32_t *d_pair_env, const int32_t *d_pair_edge, uint16_t *d_x, void *stream);
+/* training labels: bit t of d_labels[p] = bin t of the pair holds a colliding sample */
+int bcc_label_pairs(int robot, const bcc_scene *scene, const float *d_edges, int num_pairs,
+                    const int32_t *d_pair_env, const int32_t *d_pair_edge, int resolution, uint32_t *d_labels,
+                    void *stream);
+/* one forward + backward pass (mean binary cross-entropy), FP32 gradients into d_grad, no update */
+int bcc_model_grads(bcc_model *model, const uint16_t *d_x, const uint32_t *d_labels, int rows, float *d_grad,
+                    float *loss);
+/* Adam training over rows of d_x / d_labels; epoch_loss (host, optional) receives the mean loss per epoch */
+int bcc_model_train(bcc_model *model, const uint16_t *d_x, const uint32_t *d_labels, int num_rows, int epochs,
+                    int batch, float lr, float *epoch_loss);
+int bcc_model_logits(bcc_model *model, const uint16_t *d_x, int64_t rows, float *d_logits);
+/* top-4 bins of every (edge, env) pair of a dense batch into d_order [num_edges][num_envs] */
+int bcc_model_order_dense(bcc_model *model, const bcc_scene *scene, const float *d_edges, int num_edges,
+                          uint32_t *d_order, void *stream);
+/* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
+int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,
+                   int m, int n, int k, float alpha, float beta);
+
 /* Device memory helpers for bindings that do not bring their own allocator. */
 int bcc_malloc(void **d_ptr, size_t bytes);
 int bcc_free(void *d_ptr);
