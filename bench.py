@@ -99,6 +99,75 @@ def cpu_baseline_pairs(robot, sc, edges, pair_env, seconds=10.0, max_threads=16)
                       f"checker (vamp unavailable), {threads} threads"}
 
 
+def learned_leg(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, plain_ms):
+    """Train the step-ordering model on labels the exact checker writes for
+    seeded (env, edge) pairs drawn with a separate edge stream, then time one
+    learned step = model inference for every pair + the ordered check."""
+    rng = np.random.default_rng([a.seed, 3])
+    n_train_edges = max(1, min(M, 100_000))
+    train_edges = W.uniform_edges(robot, n_train_edges, rng) if robot != "fetch" else \
+        W.lognormal_edges(robot, n_train_edges, rng)
+    d_train = bcc.DeviceBuffer.from_array(train_edges)
+    n = int(a.train_pairs)
+    pe = rng.integers(0, E, n).astype(np.int32)
+    pg = rng.integers(0, n_train_edges, n).astype(np.int32)
+    t0 = time.perf_counter()
+    d_labels = bcc.label_pairs(robot, scene, d_train, pe, pg)
+    t_label = time.perf_counter() - t0
+    model = bcc.Model(robot, seed=a.seed)
+    t0 = time.perf_counter()
+    d_x = model.features(scene, d_train, pe, pg)
+    losses = model.train(d_x, d_labels, n, epochs=a.train_epochs, batch=8192, lr=2e-3)
+    bcc.synchronize()
+    t_train = time.perf_counter() - t0
+    print(f"[bench] learned: {n} labels in {t_label:.1f} s, {a.train_epochs} epochs in {t_train:.1f} s, "
+          f"loss {losses[0]:.3f} -> {losses[-1]:.3f}", file=sys.stderr, flush=True)
+    del d_x
+    d_out = bcc.DeviceBuffer(E * M)
+    # one learned step: order every pair, then the ordered check
+    for _ in range(max(1, a.warmup)):
+        d_order = model.order_dense(scene, d_edges, M)
+        bcc.check_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order)
+    bcc.synchronize()
+    t_order = []
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        d_order = model.order_dense(scene, d_edges, M)
+        bcc.synchronize()
+        t_order.append((time.perf_counter() - t0) * 1e3)
+    order_ms = float(np.median(t_order))
+    check_ms = bcc.time_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order, a.warmup, a.steps)
+    got = d_out.to_array(np.uint8, (M, E))
+    mismatches = int((got != plain_verdicts).sum())
+    # rounds per pair, plain vs learned
+    d_rounds = bcc.DeviceBuffer(E * M)
+    bcc.check_dense_ex(robot, scene, d_edges, M, 32, d_out, None, d_rounds)
+    r_plain = d_rounds.to_array(np.uint8, (M, E)).astype(np.float64)
+    bcc.check_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order, d_rounds)
+    r_learn = d_rounds.to_array(np.uint8, (M, E)).astype(np.float64)
+    coll = plain_verdicts == 1
+    print(f"[bench] learned step: order {order_ms:.2f} ms + ordered check {check_ms:.3f} ms "
+          f"(plain check {plain_ms:.3f} ms); verdict mismatches {mismatches}", file=sys.stderr, flush=True)
+    if mismatches:
+        print("[bench] learned-ordered verdicts differ from plain order", file=sys.stderr)
+        sys.exit(1)
+    return {
+        "ms_per_step": order_ms + check_ms,
+        "order_ms": order_ms,
+        "ordered_check_ms": check_ms,
+        "speedup_vs_plain": plain_ms / (order_ms + check_ms),
+        "check_only_speedup_vs_plain": plain_ms / check_ms,
+        "verdict_mismatches_vs_plain": mismatches,
+        "mean_rounds_colliding_plain": float(r_plain[coll].mean()) if coll.any() else None,
+        "mean_rounds_colliding_learned": float(r_learn[coll].mean()) if coll.any() else None,
+        "mean_rounds_all_plain": float(r_plain.mean()),
+        "mean_rounds_all_learned": float(r_learn.mean()),
+        "train": {"pairs": n, "epochs": a.train_epochs, "label_s": t_label, "train_s": t_train,
+                  "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+                  "edges": "separate seeded edge stream, same environments"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +178,10 @@ def main():
     ap.add_argument("--scale", type=float, default=1.0, help="env/edge count multiplier (1.0 = full size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-work", action="store_true", help="skip the reference-order work count")
+    ap.add_argument("--no-learned", dest="learned", action="store_false",
+                    help="skip training the step-ordering model and timing the learned-ordered check")
+    ap.add_argument("--train-pairs", type=int, default=2_000_000)
+    ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration) and exit")
     a = ap.parse_args()
@@ -177,6 +250,10 @@ def main():
         sys.exit(1)
     value = pairs / (ms / 1e3)
 
+    learned = None
+    if a.learned and not ragged:
+        learned = learned_leg(bcc, W, a, robot, scene, sc, d_edges, E, M, verdicts, ms)
+
     roofline = None
     if not a.no_work:
         t1 = time.perf_counter()
@@ -243,6 +320,11 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if learned is not None:
+        # `value` is the plain-order path; the learned-ordered step (model
+        # inference for every pair + ordered check) is reported beside it
+        learned["value"] = pairs / (learned["ms_per_step"] / 1e3)
+        rec["learned"] = learned
     print(json.dumps(rec))
     scene.close()
 
