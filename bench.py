@@ -173,7 +173,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "mbm"],
+                    help="mbm = the README run: 100 bookshelf_small scenes x 1000 Halton edges (panda)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--scale", type=float, default=1.0, help="env/edge count multiplier (1.0 = full size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -212,6 +213,15 @@ def main():
         pairs = M
         shape = (M,)
         desc = f"{robot}, {E} envs, {M} ragged pairs (Zipf(1.2) edges per env)"
+    elif a.config == "mbm":
+        from batch_cc_amd import mbm
+        robot = "panda"
+        problems = os.path.join(REPO, "tests", "golden", "mbm_panda.json")
+        sc, edges = mbm.readme_run(problems, robot, max(1, int(100 * a.scale)), max(1, int(1000 * a.scale)))
+        E, M = sc.num_envs, edges.shape[0]
+        pairs = E * M
+        shape = (M, E)
+        desc = f"{robot}, {E} MotionBenchMaker bookshelf_small envs x {M} Halton edges"
     else:
         robot, sc, edges = W.make_config(a.config, seed=a.seed, scale=a.scale)
         E, M = sc.num_envs, edges.shape[0]
@@ -305,7 +315,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic",
+        "data": "MotionBenchMaker problem file subset + Halton edges" if a.config == "mbm" else "synthetic",
         "config": {
             "workload": f"{a.config}: {desc}, seed {a.seed}",
             "robot": robot,
