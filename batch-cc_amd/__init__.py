@@ -76,6 +76,12 @@ def lib():
     _sig(L, "bcc_ragged_plan_num_tiles", I, [P])
     _sig(L, "bcc_ragged_plan_time", I, [P, I, I, P, ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_ragged_plan_destroy", I, [P])
+    _sig(L, "bcc_ragged_plan_create_tuned", I, [I, P, P, P, I, P, I, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_ragged_plan_config", I, [P, ctypes.POINTER(I), ctypes.POINTER(I)])
+    _sig(L, "bcc_ragged_group_create", I, [P, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_ragged_group_launch", I, [P, P])
+    _sig(L, "bcc_ragged_group_time", I, [P, I, I, P, ctypes.POINTER(ctypes.c_double)])
+    _sig(L, "bcc_ragged_group_destroy", I, [P])
     _sig(L, "bcc_check_dense_ex", I, [I, P, P, I, I, P, P, P, P])
     _sig(L, "bcc_time_dense_ex", I, [I, P, P, I, I, P, P, I, I, P, ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_model_num_params", I, [])
@@ -326,16 +332,25 @@ class RaggedPlan:
     """Tile list + captured hipGraph for repeated ragged launches on device buffers."""
 
     def __init__(self, robot, scene: Scene, env_off, d_edges: DeviceBuffer, d_out: DeviceBuffer,
-                 resolution: int = 32, tile_pairs: int = 0):
+                 resolution: int = 32, tile_pairs: int = 0, groups_per_cu: int = 0):
         self.off = np.ascontiguousarray(env_off, np.int64)
+        self.robot = robot
         self.scene, self.d_edges, self.d_out = scene, d_edges, d_out
         self.handle = ctypes.c_void_p()
-        _check(lib().bcc_ragged_plan_create(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
-                                            int(resolution), d_out.ptr, int(tile_pairs), ctypes.byref(self.handle)))
+        _check(lib().bcc_ragged_plan_create_tuned(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
+                                                  int(resolution), d_out.ptr, int(tile_pairs), int(groups_per_cu),
+                                                  ctypes.byref(self.handle)))
 
     @property
     def num_tiles(self) -> int:
         return lib().bcc_ragged_plan_num_tiles(self.handle)
+
+    @property
+    def config(self) -> tuple:
+        """(tile_pairs, groups_per_cu) the plan was built with."""
+        t, g = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().bcc_ragged_plan_config(self.handle, ctypes.byref(t), ctypes.byref(g)))
+        return t.value, g.value
 
     def launch(self, stream=None):
         _check(lib().bcc_ragged_plan_launch(self.handle, stream))
@@ -351,6 +366,36 @@ class RaggedPlan:
     def close(self):
         if self.handle:
             _check(lib().bcc_ragged_plan_destroy(self.handle))
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class RaggedGroup:
+    """Several ragged plans (e.g. one per robot) replayed as one hipGraph with
+    the plans as independent branches.  Keeps the plans alive."""
+
+    def __init__(self, plans: list):
+        self.plans = list(plans)
+        arr = (ctypes.c_void_p * len(self.plans))(*[p.handle.value for p in self.plans])
+        self.handle = ctypes.c_void_p()
+        _check(lib().bcc_ragged_group_create(arr, len(self.plans), ctypes.byref(self.handle)))
+
+    def launch(self, stream=None):
+        _check(lib().bcc_ragged_group_launch(self.handle, stream))
+
+    def time(self, warmup: int, iters: int, stream=None) -> float:
+        ms = ctypes.c_double(0.0)
+        _check(lib().bcc_ragged_group_time(self.handle, int(warmup), int(iters), stream, ctypes.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self.handle:
+            _check(lib().bcc_ragged_group_destroy(self.handle))
             self.handle = ctypes.c_void_p()
 
     def __del__(self):

@@ -22,6 +22,7 @@
 #include "../../include/bcc.h"
 #include "bcc_dense.h"
 #include "bcc_internal.h"
+#include "bcc_tuning.h"
 #include "robots/baxter_model.h"
 #include "robots/fetch_model.h"
 #include "robots/panda_model.h"
@@ -631,9 +632,16 @@ struct bcc_ragged_plan {
     int* d_counter = nullptr;
     int num_tiles = 0;
     int64_t num_pairs = 0;
+    int tile_pairs = 0;
+    int groups_per_cu = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bcc::RaggedArgs args{};
+};
+
+struct bcc_ragged_group {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -652,24 +660,28 @@ int dispatch_ragged(int robot, const bcc::RaggedArgs& a, hipStream_t s)
     return BCC_OK;
 }
 
-// Default tile size and persistent grid factor per robot (measured; see
-// batch-cc_amd/tune.py).
-int default_tile_pairs(int robot)
+// Tile size and persistent grid from the tuning table (bcc_tuning.h, written
+// by batch-cc_amd/tune.py from timings on the GPU): the first entry of the
+// robot whose bound is at least the batch's mean pairs per environment.
+const bcc_tune_entry& tuned(int robot, double mean_pairs_per_env)
 {
-    switch (robot) {
-    case BCC_PANDA: return 64;
-    case BCC_FETCH: return 64;
-    default: return 32;
+    const bcc_tune_entry* last = nullptr;
+    for (const bcc_tune_entry& t : kTuneTable) {
+        if (t.robot != robot)
+            continue;
+        last = &t;
+        if (mean_pairs_per_env <= (double)t.max_mean_pairs)
+            return t;
     }
+    return *last;   // every robot has entries (checked by the library tests)
 }
 
-int persistent_grid(int robot)
+int device_cus()
 {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int waves_per_simd = robot == BCC_FETCH ? 6 : 4;
-    return cus * waves_per_simd;   // 4 waves per workgroup, 4 SIMDs per CU
+    return cus;
 }
 
 }  // namespace
@@ -692,6 +704,14 @@ void bcc_ragged_plan_destroy_impl(bcc_ragged_plan* p)
 int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* d_edges,
                            int resolution, uint8_t* d_results, int tile_pairs, bcc_ragged_plan** out)
 {
+    return bcc_ragged_plan_create_tuned(robot, scene, env_offsets, d_edges, resolution, d_results, tile_pairs, 0,
+                                        out);
+}
+
+int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_t* env_offsets,
+                                 const float* d_edges, int resolution, uint8_t* d_results, int tile_pairs,
+                                 int groups_per_cu, bcc_ragged_plan** out)
+{
     if (!out)
         return fail(BCC_ERR_INVALID, "out is NULL");
     *out = nullptr;
@@ -710,8 +730,13 @@ int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env
         return fail(BCC_ERR_INVALID, "more than 2^31 pairs in one ragged batch");
     if (total > 0 && (!d_edges || !d_results))
         return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
-    if (tile_pairs <= 0)
-        tile_pairs = default_tile_pairs(robot);
+    if (tile_pairs < 0 || groups_per_cu < 0 || groups_per_cu > 32)
+        return fail(BCC_ERR_INVALID, "tile_pairs must be >= 0 and groups_per_cu in 0..32");
+    const bcc_tune_entry& tune = tuned(robot, E > 0 ? (double)total / E : 0.0);
+    if (tile_pairs == 0)
+        tile_pairs = tune.tile_pairs;
+    if (groups_per_cu == 0)
+        groups_per_cu = tune.groups_per_cu;
     std::vector<bcc::RaggedTile> tiles;
     for (int e = 0; e < E; ++e)
         for (int64_t b = env_offsets[e]; b < env_offsets[e + 1]; b += tile_pairs)
@@ -727,6 +752,8 @@ int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env
     p->scene = scene;
     p->num_tiles = (int)tiles.size();
     p->num_pairs = total;
+    p->tile_pairs = tile_pairs;
+    p->groups_per_cu = groups_per_cu;
     hipError_t err = hipMalloc((void**)&p->d_tiles, sizeof(bcc::RaggedTile) * std::max<size_t>(tiles.size(), 1));
     if (err == hipSuccess)
         err = hipMalloc((void**)&p->d_counter, sizeof(int));
@@ -737,7 +764,7 @@ int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env
         return hip_fail(err, "ragged plan upload");
     }
     p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
-                              resolution, persistent_grid(robot), d_results};
+                              resolution, device_cus() * groups_per_cu, d_results};
     // capture reset + persistent kernel once; launches replay the graph
     hipStream_t cap;
     err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
@@ -787,19 +814,38 @@ int bcc_ragged_plan_num_tiles(const bcc_ragged_plan* plan)
     return plan ? plan->num_tiles : -1;
 }
 
-int bcc_ragged_plan_time(bcc_ragged_plan* plan, int warmup, int iters, void* stream, double* avg_ms)
+int bcc_ragged_plan_config(const bcc_ragged_plan* plan, int* tile_pairs, int* groups_per_cu)
 {
-    if (!plan || !avg_ms || iters < 1 || warmup < 0)
+    if (!plan)
+        return fail(BCC_ERR_INVALID, "plan is NULL");
+    if (tile_pairs)
+        *tile_pairs = plan->tile_pairs;
+    if (groups_per_cu)
+        *groups_per_cu = plan->groups_per_cu;
+    return BCC_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Average time of `iters` replays of an instantiated graph, after `warmup`.
+int time_graph(hipGraphExec_t exec, int warmup, int iters, hipStream_t s, double* avg_ms)
+{
+    if (!avg_ms || iters < 1 || warmup < 0)
         return fail(BCC_ERR_INVALID, "bad timing arguments");
-    hipStream_t s = (hipStream_t)stream;
     for (int i = 0; i < warmup; ++i)
-        BCC_HIP(hipGraphLaunch(plan->exec, s), "hipGraphLaunch");
+        BCC_HIP(hipGraphLaunch(exec, s), "hipGraphLaunch");
     hipEvent_t t0, t1;
     BCC_HIP(hipEventCreate(&t0), "hipEventCreate");
-    BCC_HIP(hipEventCreate(&t1), "hipEventCreate");
-    hipError_t e = hipEventRecord(t0, s);
+    hipError_t e = hipEventCreate(&t1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    e = hipEventRecord(t0, s);
     for (int i = 0; i < iters && e == hipSuccess; ++i)
-        e = hipGraphLaunch(plan->exec, s);
+        e = hipGraphLaunch(exec, s);
     if (e == hipSuccess)
         e = hipEventRecord(t1, s);
     if (e == hipSuccess)
@@ -810,15 +856,83 @@ int bcc_ragged_plan_time(bcc_ragged_plan* plan, int warmup, int iters, void* str
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (e != hipSuccess)
-        return hip_fail(e, "ragged timing");
+        return hip_fail(e, "graph timing");
     *avg_ms = (double)ms / iters;
     return BCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcc_ragged_plan_time(bcc_ragged_plan* plan, int warmup, int iters, void* stream, double* avg_ms)
+{
+    if (!plan)
+        return fail(BCC_ERR_INVALID, "plan is NULL");
+    return time_graph(plan->exec, warmup, iters, (hipStream_t)stream, avg_ms);
 }
 
 int bcc_ragged_plan_destroy(bcc_ragged_plan* plan)
 {
     bcc_ragged_plan_destroy_impl(plan);
     return BCC_OK;
+}
+
+int bcc_ragged_group_destroy(bcc_ragged_group* group)
+{
+    if (!group)
+        return BCC_OK;
+    if (group->exec)
+        (void)hipGraphExecDestroy(group->exec);
+    if (group->graph)
+        (void)hipGraphDestroy(group->graph);
+    delete group;
+    return BCC_OK;
+}
+
+int bcc_ragged_group_create(bcc_ragged_plan* const* plans, int num_plans, bcc_ragged_group** out)
+{
+    if (!out)
+        return fail(BCC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!plans || num_plans < 1)
+        return fail(BCC_ERR_INVALID, "a group needs at least one plan");
+    for (int i = 0; i < num_plans; ++i) {
+        if (!plans[i] || !plans[i]->graph)
+            return fail(BCC_ERR_INVALID, "plan " + std::to_string(i) + " is NULL");
+        for (int j = 0; j < i; ++j)
+            if (plans[j] == plans[i])
+                return fail(BCC_ERR_INVALID, "a plan appears twice in one group (its launches would overlap)");
+    }
+    bcc_ragged_group* g = new bcc_ragged_group;
+    hipError_t e = hipGraphCreate(&g->graph, 0);
+    for (int i = 0; i < num_plans && e == hipSuccess; ++i) {
+        hipGraphNode_t node;
+        e = hipGraphAddChildGraphNode(&node, g->graph, nullptr, 0, plans[i]->graph);
+    }
+    if (e == hipSuccess)
+        e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        bcc_ragged_group_destroy(g);
+        return hip_fail(e, "ragged group graph");
+    }
+    *out = g;
+    return BCC_OK;
+}
+
+int bcc_ragged_group_launch(bcc_ragged_group* group, void* stream)
+{
+    if (!group)
+        return fail(BCC_ERR_INVALID, "group is NULL");
+    BCC_HIP(hipGraphLaunch(group->exec, (hipStream_t)stream), "hipGraphLaunch");
+    return BCC_OK;
+}
+
+int bcc_ragged_group_time(bcc_ragged_group* group, int warmup, int iters, void* stream, double* avg_ms)
+{
+    if (!group)
+        return fail(BCC_ERR_INVALID, "group is NULL");
+    return time_graph(group->exec, warmup, iters, (hipStream_t)stream, avg_ms);
 }
 
 int bcc_check_ragged(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* edges,

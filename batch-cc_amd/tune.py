@@ -1,0 +1,143 @@
+"""Autotune the ragged-batch launch table on the GPU and write csrc/bcc_tuning.h.
+
+For every robot and every batch shape bucket (mean pairs per environment up
+to 8, 64, 512, unbounded) a seeded ragged batch is built (U{32..256}
+obstacles of the mixed kinds away from the robot base, lognormal edges,
+U{m/2..3m/2} pairs per environment), and each candidate (tile size,
+workgroups per CU) is timed as a replayed hipGraph.  Verdicts must be
+identical for every candidate.  The fastest candidate of each bucket goes into
+the table; the library reads it when a plan is created with tile_pairs = 0 /
+groups_per_cu = 0.
+
+    python batch-cc_amd/tune.py [--out batch-cc_amd/csrc/bcc_tuning.h] [--json timings.json] [--quick]
+
+Rebuild the library (batch-cc_amd/build.py) after writing the table.
+"""
+from __future__ import annotations
+
+import argparse
+import datetime
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUCKETS = (8, 64, 512, None)          # upper bound on mean pairs per environment
+BUCKET_MEAN = (4, 32, 256, 2048)      # the mean used to time each bucket
+TILES = (2, 4, 8, 16, 32, 64, 128, 256)
+GROUPS = (2, 4, 6, 8)
+ROBOTS = ("panda", "fetch", "baxter")
+PAIRS = {"panda": 400_000, "fetch": 400_000, "baxter": 200_000}
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("batch_cc_amd", os.path.join(HERE, "__init__.py"),
+                                                  submodule_search_locations=[HERE])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["batch_cc_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def batch(W, robot: str, mean: int, pairs: int, seed: int):
+    rs = np.random.default_rng([seed, 11])
+    E = max(8, min(16384, pairs // mean))
+    counts = rs.integers(max(1, mean // 2), max(2, 3 * mean // 2 + 1), E).astype(np.int64)
+    envs = []
+    for _ in range(E):
+        n = int(rs.integers(32, 257))
+        envs.append(W.scaled_env(rs, n, (32.0 / n) ** (1.0 / 3.0)))
+    off = np.zeros(E + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    edges = W.lognormal_edges(robot, int(off[-1]), rs, median=W.MIXED_EDGE_MEDIAN)
+    from batch_cc_amd import PackedScene
+    return PackedScene.from_envs(envs), off, edges
+
+
+def header(table: dict, source: str) -> str:
+    rid = {r: i for i, r in enumerate(ROBOTS)}
+    lines = [
+        "// Ragged-batch launch table: per robot, by mean pairs per environment, the",
+        "// tile size (pairs per tile) and the persistent grid (workgroups per CU).",
+        "// Written by batch-cc_amd/tune.py; an entry applies when the batch's mean",
+        "// pairs per environment is at most max_mean_pairs (the last entry of each",
+        "// robot has no bound).",
+        "#pragma once",
+        "",
+        "struct bcc_tune_entry {",
+        "    int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER",
+        "    long long max_mean_pairs;",
+        "    int tile_pairs;",
+        "    int groups_per_cu;",
+        "};",
+        "",
+        f"// source: {source}",
+        "static const bcc_tune_entry kTuneTable[] = {",
+    ]
+    for robot in ROBOTS:
+        for bound, (tile, groups, ms) in zip(BUCKETS, table[robot]):
+            b = "0x7fffffffffffffffLL" if bound is None else f"{bound}LL"
+            lines.append(f"    {{{rid[robot]}, {b}, {tile}, {groups}}},   // {robot}: {ms:.3f} ms")
+    lines.append("};")
+    return "\n".join(lines) + "\n"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "csrc", "bcc_tuning.h"))
+    ap.add_argument("--json", default=None, help="write every timing here")
+    ap.add_argument("--quick", action="store_true", help="fewer candidates and smaller batches")
+    ap.add_argument("--seed", type=int, default=0)
+    a = ap.parse_args()
+    bcc = load_pkg()
+    from batch_cc_amd import workloads as W
+    tiles = (8, 32, 128) if a.quick else TILES
+    groups = (4, 8) if a.quick else GROUPS
+    scale = 0.25 if a.quick else 1.0
+    table, timings = {}, []
+    for robot in ROBOTS:
+        table[robot] = []
+        for mean in BUCKET_MEAN:
+            t0 = time.perf_counter()
+            sc, off, edges = batch(W, robot, mean, int(PAIRS[robot] * scale), a.seed)
+            n = edges.shape[0]
+            best, ref = None, None
+            with bcc.Scene(sc) as scene:
+                de = bcc.DeviceBuffer.from_array(edges)
+                do = bcc.DeviceBuffer(n)
+                for tile in tiles:
+                    if tile > 4 * mean:
+                        continue
+                    for g in groups:
+                        plan = bcc.RaggedPlan(robot, scene, off, de, do, tile_pairs=tile, groups_per_cu=g)
+                        ms = plan.time(2, 5)
+                        v = do.to_array(np.uint8, (n,))
+                        plan.close()
+                        if ref is None:
+                            ref = v
+                        elif not np.array_equal(v, ref):
+                            raise SystemExit(f"tune: verdicts differ for {robot} tile {tile} groups {g}")
+                        timings.append({"robot": robot, "mean_pairs": mean, "pairs": n, "tile": tile,
+                                        "groups_per_cu": g, "ms": ms})
+                        if best is None or ms < best[2]:
+                            best = (tile, g, ms)
+            table[robot].append(best)
+            print(f"[tune] {robot} mean {mean}: {n} pairs, {sc.num_envs} envs, best tile {best[0]} "
+                  f"groups {best[1]} {best[2]:.3f} ms ({n / best[2] / 1e3:.3g} Mpairs/s, "
+                  f"{time.perf_counter() - t0:.1f} s)", flush=True)
+    props = bcc.device_name() if hasattr(bcc, "device_name") else "gfx950"
+    src = f"tune.py on {props}, {datetime.date.today().isoformat()}, seed {a.seed}" + (" (quick)" if a.quick else "")
+    with open(a.out, "w") as f:
+        f.write(header(table, src))
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(timings, f)
+    print(f"[tune] wrote {a.out}")
+
+
+if __name__ == "__main__":
+    main()

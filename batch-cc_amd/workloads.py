@@ -20,6 +20,8 @@ Configurations (sizes from the benchmark definition):
   c2  baxter 4096 envs, Zipf(1.2) edges per env on 1..4096, U{8..256}
              obstacles with 20% capsules (ragged)
   c3  panda  1000 envs x 100000 edges, c0 distributions
+  c4  panda/fetch/baxter environments interleaved, 2^24 pairs, U{32..512}
+      obstacles, about 40% colliding pairs (make_mixed)
 The ragged batch c2 comes from make_ragged().
 """
 from __future__ import annotations
@@ -153,6 +155,110 @@ def make_ragged(name: str, seed: int = 0, scale: float = 1.0):
     off[1:] = np.cumsum(counts)
     edges = uniform_edges("baxter", int(off[-1]), re)
     return "baxter", PackedScene.from_envs(envs), off, edges
+
+
+MIXED_ROBOTS = ("panda", "fetch", "baxter")
+MIXED_SIZE_SCALE = 1.0
+MIXED_EDGE_MEDIAN = 0.3
+MIXED_LO = np.array([-2.5, -2.5, 0.0], np.float32)
+MIXED_HI = np.array([2.5, 2.5, 2.0], np.float32)
+MIXED_CLEAR_RADIUS = 0.7
+
+
+def _centres_clear(rng, n: int) -> np.ndarray:
+    """n points uniform in MIXED_LO..MIXED_HI outside the vertical cylinder of
+    radius MIXED_CLEAR_RADIUS around the z axis, where the robots' bases stand."""
+    out = np.zeros((0, 3), np.float32)
+    while out.shape[0] < n:
+        c = rng.uniform(MIXED_LO, MIXED_HI, (2 * n, 3)).astype(np.float32)
+        c = c[c[:, 0] * c[:, 0] + c[:, 1] * c[:, 1] >= np.float32(MIXED_CLEAR_RADIUS ** 2)]
+        out = np.concatenate([out, c])
+    return out[:n]
+
+
+def scaled_env(rng, n: int, k: float) -> dict:
+    """n obstacles of the mixed kinds (cuboid, sphere, cylinder, capsule) with
+    every size multiplied by k, centred away from the robot bases."""
+    kinds = rng.integers(0, 4, n)
+    cen = _centres_clear(rng, n)
+    nc, ns = int((kinds == 0).sum()), int((kinds == 1).sum())
+    cub = np.zeros((nc, 15), np.float32)
+    cub[:, :3] = cen[kinds == 0]
+    cub[:, 3], cub[:, 7], cub[:, 11] = 1, 1, 1
+    cub[:, 12:] = rng.uniform(0.05, 0.3, (nc, 3)) * k
+    sph = np.zeros((ns, 4), np.float32)
+    sph[:, :3] = cen[kinds == 1]
+    sph[:, 3] = rng.uniform(0.05, 0.2, ns) * k
+    cc = cen[kinds >= 2]
+    r = rng.uniform(0.03, 0.15, len(cc)) * k
+    h = rng.uniform(0.1, 0.6, len(cc)) * k
+    return {"cuboids": cub, "spheres": sph, "capsules": cylinders_as_capsules(cc, r, h)}
+
+
+def cylinders_as_capsules(c, r, h) -> np.ndarray:
+    """cylinder_as_capsule over arrays (elementwise float32, same values)."""
+    c = np.asarray(c, np.float32).reshape(-1, 3)
+    half = np.asarray(h, np.float32) / np.float32(2.0)
+    out = np.zeros((c.shape[0], 8), np.float32)
+    out[:, :2] = c[:, :2]
+    out[:, 2] = c[:, 2] + half
+    vz = (c[:, 2] - half) - out[:, 2]
+    out[:, 5] = vz
+    dot = (np.float32(0) * np.float32(0) + np.float32(0) * np.float32(0)) + vz * vz
+    with np.errstate(divide="ignore"):
+        out[:, 7] = (1.0 / dot.astype(np.float64)).astype(np.float32)
+    out[:, 6] = np.asarray(r, np.float32)
+    return out
+
+
+def make_mixed(seed: int = 0, scale: float = 1.0, total_pairs: int = 1 << 24, size_scale: float = None):
+    """c4: panda, fetch and baxter environments interleaved (env g belongs to
+    robot g % 3), 1024 of each at full size, U{32..512} obstacles of mixed
+    kinds centred in [-2.5,2.5]^2 x [0,2] outside a 0.7 m column around the
+    robot bases, with sizes
+    shrinking as (32/n)^(1/3) (the occupied volume does not grow with the
+    count) times a constant chosen for about 40% colliding pairs; 2^24 pairs in
+    all, U{m/2..3m/2} per environment (m the mean).  Edges follow c1's law
+    (uniform start, lognormal length, median 0.3 rad): with uniform endpoints
+    the arms' sweep crosses the whole workspace and nearly every pair collides
+    (baxter self-collides on 83% of uniform edges with no obstacles at all).
+
+    Returns (parts, env_robot, global_off): parts is one tuple per robot
+    (robot, PackedScene, env_off, edges, global_index) where global_index[i]
+    is pair i's position in the interleaved batch; global_off is the CSR over
+    all environments in interleaved order."""
+    k0 = MIXED_SIZE_SCALE if size_scale is None else size_scale
+    rs = np.random.default_rng([seed, 1])
+    re = np.random.default_rng([seed, 2])
+    G = max(3, 3 * int(round(1024 * scale)))
+    total = max(G, int(total_pairs * scale))
+    m = total / G
+    counts = re.integers(max(1, int(m // 2)), max(2, int(3 * m // 2) + 1), G).astype(np.int64)
+    # hit the exact total: spread the difference over the environments
+    diff = total - int(counts.sum())
+    sgn = 1 if diff >= 0 else -1
+    q, r = divmod(abs(diff), G)
+    counts += sgn * q
+    counts[:r] += sgn
+    assert counts.min() >= 0 and int(counts.sum()) == total
+    global_off = np.zeros(G + 1, np.int64)
+    global_off[1:] = np.cumsum(counts)
+    env_robot = np.arange(G) % 3
+    envs = []
+    for g in range(G):
+        n = int(rs.integers(32, 513))
+        envs.append(scaled_env(rs, n, k0 * (32.0 / n) ** (1.0 / 3.0)))
+    parts = []
+    for r, robot in enumerate(MIXED_ROBOTS):
+        ids = np.nonzero(env_robot == r)[0]
+        c = counts[ids]
+        off = np.zeros(len(ids) + 1, np.int64)
+        off[1:] = np.cumsum(c)
+        # global position of each pair: its env's global start + rank in the env
+        gidx = np.repeat(global_off[ids] - off[:-1], c) + np.arange(int(off[-1]), dtype=np.int64)
+        edges = lognormal_edges(robot, int(off[-1]), re, median=MIXED_EDGE_MEDIAN)
+        parts.append((robot, PackedScene.from_envs([envs[g] for g in ids]), off, edges, gidx))
+    return parts, env_robot, global_off
 
 
 def make_config(name: str, seed: int = 0, scale: float = 1.0):

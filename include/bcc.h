@@ -157,11 +157,33 @@ int bcc_check_ragged(int robot, const bcc_scene *scene, const int64_t *env_offse
 typedef struct bcc_ragged_plan bcc_ragged_plan;
 int bcc_ragged_plan_create(int robot, const bcc_scene *scene, const int64_t *env_offsets, const float *d_edges,
                            int resolution, uint8_t *d_results, int tile_pairs, bcc_ragged_plan **out);
+/* Same, with the persistent grid given as workgroups per CU (one wave per
+ * SIMD each); 0 for either knob takes the tuning table's entry for this
+ * robot and the batch's mean pairs per environment. */
+int bcc_ragged_plan_create_tuned(int robot, const bcc_scene *scene, const int64_t *env_offsets,
+                                 const float *d_edges, int resolution, uint8_t *d_results, int tile_pairs,
+                                 int groups_per_cu, bcc_ragged_plan **out);
 int bcc_ragged_plan_launch(bcc_ragged_plan *plan, void *stream);        /* graph replay */
 int bcc_ragged_plan_launch_eager(bcc_ragged_plan *plan, void *stream);  /* same kernels, no graph */
 int bcc_ragged_plan_num_tiles(const bcc_ragged_plan *plan);
+/* tile size and grid the plan was built with */
+int bcc_ragged_plan_config(const bcc_ragged_plan *plan, int *tile_pairs, int *groups_per_cu);
 int bcc_ragged_plan_time(bcc_ragged_plan *plan, int warmup, int iters, void *stream, double *avg_ms);
 int bcc_ragged_plan_destroy(bcc_ragged_plan *plan);
+
+/*
+ * Mixed-robot batches (no reference counterpart: the reference checks one
+ * robot type per call, src/batch_cc.cu:244-261,340-342).  A group replays
+ * several ragged plans - typically one per robot, each over its own scene and
+ * buffers - as one hipGraph whose plans are independent branches.  The plans
+ * must outlive the group and must not be launched while a group launch that
+ * contains them runs.
+ */
+typedef struct bcc_ragged_group bcc_ragged_group;
+int bcc_ragged_group_create(bcc_ragged_plan *const *plans, int num_plans, bcc_ragged_group **out);
+int bcc_ragged_group_launch(bcc_ragged_group *group, void *stream);
+int bcc_ragged_group_time(bcc_ragged_group *group, int warmup, int iters, void *stream, double *avg_ms);
+int bcc_ragged_group_destroy(bcc_ragged_group *group);
 
 /*
  * Dense check with options (device buffers):

@@ -96,3 +96,50 @@ def test_workloads_follow_reference_conventions(bcc):
     assert robot == "fetch" and edges.shape[2] == 8
     n = sc.counts[:, [0, 1, 4]].sum(1)
     assert np.all((n >= 4) & (n <= 64))
+
+
+def test_ragged_plan_and_group_reject_bad_arguments(bcc):
+    import ctypes
+    L = bcc.lib()
+    h = ctypes.c_void_p()
+    off = np.zeros(2, np.int64)
+    assert L.bcc_ragged_plan_create_tuned(0, None, off.ctypes.data, None, 32, None, 0, 0, ctypes.byref(h)) == 1
+    assert b"scene is NULL" in L.bcc_last_error()
+    assert L.bcc_ragged_group_create(None, 0, ctypes.byref(h)) == 1
+    assert b"at least one plan" in L.bcc_last_error()
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert L.bcc_ragged_group_create(arr, 2, ctypes.byref(h)) == 1
+    assert b"is NULL" in L.bcc_last_error()
+    assert L.bcc_ragged_plan_config(None, None, None) == 1
+    assert L.bcc_ragged_group_launch(None, None) == 1
+    assert L.bcc_ragged_group_destroy(None) == 0
+
+
+def test_tuning_table_covers_every_robot():
+    text = open(os.path.join(REPO, "batch-cc_amd", "csrc", "bcc_tuning.h")).read()
+    rows = re.findall(r"\{\s*(\d+),\s*(0x[0-9a-fA-F]+LL|\d+LL|\d+),\s*(\d+),\s*(\d+)\s*\}", text)
+    by_robot = {}
+    for r, bound, tile, groups in rows:
+        by_robot.setdefault(int(r), []).append((int(bound.rstrip("L"), 0), int(tile), int(groups)))
+    assert sorted(by_robot) == [0, 1, 2]
+    for r, ent in by_robot.items():
+        bounds = [b for b, _, _ in ent]
+        assert bounds == sorted(bounds) and bounds[-1] == 0x7fffffffffffffff, r
+        assert all(t >= 1 and 1 <= g <= 32 for _, t, g in ent), r
+
+
+def test_mixed_workload_layout(bcc):
+    from batch_cc_amd import workloads as W
+    parts, env_robot, goff = W.make_mixed(seed=1, scale=0.01, total_pairs=1 << 20)
+    assert [p[0] for p in parts] == ["panda", "fetch", "baxter"]
+    assert int(goff[-1]) == int((1 << 20) * 0.01)
+    allg = np.concatenate([p[4] for p in parts])
+    assert np.array_equal(np.sort(allg), np.arange(goff[-1]))
+    for r, (robot, sc, off, edges, gidx) in enumerate(parts):
+        ids = np.nonzero(env_robot == r)[0]
+        assert sc.num_envs == len(ids) and edges.shape == (int(off[-1]), 2, bcc.DIMS[robot])
+        assert np.array_equal(np.diff(off), np.diff(goff)[ids])
+        n = sc.counts.sum(1)
+        assert np.all((n >= 32) & (n <= 512))
+        c = np.concatenate([sc.env(e)["cuboids"][:, :3] for e in range(sc.num_envs)])
+        assert np.all(c[:, 0] ** 2 + c[:, 1] ** 2 >= np.float32(W.MIXED_CLEAR_RADIUS ** 2))
