@@ -2,7 +2,8 @@
 MotionBenchMaker problem files (scripts/panda_problems.json,
 scripts/fetch_problems.json) in their own structure, holding only the fields
 the README driver reads (scripts/batch_cc_mbm.cpp:19-93) plus the generator's
-quaternion, which pins the Euler-angle conversion.
+quaternion, which pins the Euler-angle conversion, and each problem's start
+and goal configurations (valid in the generator's mesh model).
 
 Kept per robot, in the file's family order:
   bookshelf_small  every problem (the README run, `batch_cc panda 100 1000`,
@@ -44,7 +45,7 @@ def main():
             probs = data["problems"][fam]
             n = KEEP[fam] or len(probs)
             out["problems"][fam] = [
-                {"index": p["index"], "valid": p["valid"],
+                {"index": p["index"], "valid": p["valid"], "start": p["start"], "goals": p["goals"],
                  **{k: [obstacle(o, k) for o in p[k]] for k in ("sphere", "cylinder", "box")}}
                 for p in probs[:n]]
         dst = os.path.join(here, f"mbm_{robot}.json")

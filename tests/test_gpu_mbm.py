@@ -59,3 +59,29 @@ def test_readme_run_through_the_ragged_plan(bcc, gpu):
         ragged = bcc.check_ragged("panda", scene, off, edges[pick])
     env_of = np.repeat(np.arange(sc.num_envs), counts)
     assert np.array_equal(ragged, dense[pick, env_of])
+
+
+@pytest.mark.parametrize("robot,rid", [("panda", 0), ("fetch", 1)])
+def test_start_and_goal_configurations_on_the_gpu(bcc, oracle, gpu, robot, rid):
+    """Every problem's start and goals as zero-length edges against all 100
+    scenes: bit-exact against the oracle; on its own scene each lies outside
+    the obstacles up to the 5 mm padding band (FP64 margin > -5 mm)."""
+    from batch_cc_amd import mbm
+    problems = _problems(robot)
+    sel = mbm.select_problems(problems, 100)
+    sc = mbm.mbm_scene(problems, 100)
+    qs, own = [], []
+    for e, (fam, p) in enumerate(sel):
+        for q in [p["start"]] + p["goals"]:
+            qs.append(np.array(q, np.float32))
+            own.append(e)
+    q = np.stack(qs)
+    edges = np.stack([q, q], 1)
+    with bcc.Scene(sc) as scene:
+        got = bcc.check_dense(robot, scene, edges)
+    assert np.array_equal(got, oracle.check_dense(rid, sc, edges, threads=16))
+    _, mg = oracle.check_dense64(rid, sc, edges, threads=16)
+    own_margin = mg[np.arange(len(own)), own]
+    print(f"{robot}: {len(own)} configurations, {int(got[np.arange(len(own)), own].sum())} in contact with "
+          f"their own scene, min margin {own_margin.min():.4f} m; {got.mean():.3f} of all (config, scene) pairs collide")
+    assert own_margin.min() > -0.005

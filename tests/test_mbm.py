@@ -173,6 +173,40 @@ def test_box_family_cylinders_become_cuboids(mbm, panda_problems):
             assert np.array_equal(env["cuboids"][k][:3], np.array(o["position"], np.float32))
 
 
+def _start_goal_margins(mbm, oracle, problems, rid, families):
+    from batch_cc_amd import PackedScene
+    margins = []
+    for fam, p in mbm.select_problems(problems, 1000, families=families):
+        sc = PackedScene.from_envs([mbm.problem_dict_to_env(p, fam)])
+        for q in [p["start"]] + p["goals"]:
+            q = np.array(q, np.float32)
+            _, mg = oracle.check_dense64(rid, sc, np.stack([q, q])[None])
+            margins.append(mg[0, 0])
+    return np.array(margins)
+
+
+@pytest.mark.parametrize("robot,rid", [("panda", 0), ("fetch", 1)])
+def test_start_and_goal_configurations_are_free_up_to_the_padding(mbm, oracle, robot, rid):
+    """Every start and goal was valid in the problem generator's mesh model.
+    The sphere model encloses the meshes, so a sphere-model collision at such
+    a configuration can only come from the 5 mm padding band: its FP64 margin
+    stays above -5 mm.  A wrong Euler convention or cylinder rule rotates the
+    shelves and pushes valid configurations deep into them."""
+    problems = mbm.load_problems(os.path.join(GOLDEN, f"mbm_{robot}.json"))
+    m = _start_goal_margins(mbm, oracle, problems, rid, {"bookshelf_small", "box"})
+    print(f"{robot}: {len(m)} configurations, {(m < 0).sum()} in contact, min margin {m.min():.4f} m")
+    assert len(m) > 200
+    assert m.min() > -0.005
+    assert (m < 0).mean() < 0.05
+
+
+def test_start_goal_invariant_detects_transposed_axes(mbm, oracle, panda_problems, monkeypatch):
+    orig = mbm.quat_rotate
+    monkeypatch.setattr(mbm, "quat_rotate", lambda q, v: orig((-q[0], -q[1], -q[2], q[3]), v))
+    m = _start_goal_margins(mbm, oracle, panda_problems, 0, {"bookshelf_small"})
+    assert m.min() < -0.005
+
+
 def test_oracle_fp32_agrees_with_fp64_on_the_readme_scenes(mbm, panda_problems, oracle):
     sc = mbm.mbm_scene(panda_problems, 20)
     edges = mbm.halton_edges("panda", 100)
