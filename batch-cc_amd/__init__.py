@@ -98,6 +98,7 @@ def lib():
     _sig(L, "bcc_model_train", I, [P, P, P, I, I, I, ctypes.c_float, P])
     _sig(L, "bcc_model_logits", I, [P, P, ctypes.c_int64, P])
     _sig(L, "bcc_model_order_dense", I, [P, P, P, I, P, P])
+    _sig(L, "bcc_model_logits_dense", I, [P, P, P, I, P, P, P])
     _sig(L, "bcc_debug_gemm", I, [I, I, P, I, P, I, P, I, I, I, I, ctypes.c_float, ctypes.c_float])
     _sig(L, "bcc_malloc", I, [ctypes.POINTER(P), ctypes.c_size_t])
     _sig(L, "bcc_free", I, [P])
@@ -553,6 +554,15 @@ class Model:
         _check(lib().bcc_model_order_dense(self.handle, scene.handle, d_edges.ptr, int(num_edges), d.ptr, None))
         synchronize()
         return d
+
+    def logits_dense(self, scene: Scene, d_edges: DeviceBuffer, num_edges: int):
+        """(order uint32 [M, E], logits float32 [M, E, 32]) from the fused kernel."""
+        M, E = int(num_edges), scene.num_envs
+        d = DeviceBuffer(M * E * 4)
+        dl = DeviceBuffer(M * E * self.OUT * 4)
+        _check(lib().bcc_model_logits_dense(self.handle, scene.handle, d_edges.ptr, M, d.ptr, dl.ptr, None))
+        synchronize()
+        return d.to_array(np.uint32, (M, E)), dl.to_array(np.float32, (M, E, self.OUT))
 
     def close(self):
         if self.handle:

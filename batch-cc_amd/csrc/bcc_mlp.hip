@@ -209,14 +209,174 @@ __global__ void to_bf16_kernel(const float* __restrict__ a, long long n, uint16_
         b[i] = f2bf(a[i]);
 }
 
-// Top-4 bins of each row of logits, packed one byte per rank (rank 0 in the
-// low byte); ties go to the lower bin.
-__global__ void top4_kernel(const float* __restrict__ logits, long long rows, uint32_t* __restrict__ order)
+// ------------------------------------------------------------ fused inference
+// Inference over a dense batch in one kernel, computed transposed,
+// Y^T = W^T X^T, so that an MFMA result tile (lane l: rows 4 (l/16) + v,
+// column l % 16) is already the next layer's B operand (lane l: k rows
+// 4 (l/16) + v, column l % 16) and activations never leave registers.
+// Columns are pairs, 16 per MFMA tile, 64 per wave.  Weights are read as
+// prepacked A fragments: fragment (rt, kt) of a layer with W [K][N] holds for
+// lane l the four values W[16 kt + 4 (l/16) + i][16 rt + l % 16], i = 0..3.
+// Every product, its k order and the bias/ReLU/rounding steps are those of
+// the layer-by-layer path (gemm_bf16_kernel + bias_act_kernel), so logits
+// and orders are identical to it.
+constexpr int kKT1 = kIn / 16;                   // k blocks of layer 1
+constexpr int kKT2 = kHid / 16;                  // k blocks of layers 2 and 3
+constexpr int kRT12 = kHid / 16;                 // row tiles of layers 1 and 2
+constexpr int kRT3 = kOut / 16;                  // row tiles of layer 3
+constexpr int kFragL1 = 0;
+constexpr int kFragL2 = kFragL1 + kRT12 * kKT1;
+constexpr int kFragL3 = kFragL2 + kRT12 * kKT2;
+constexpr int kFrags = kFragL3 + kRT3 * kKT2;
+constexpr int kFuseCols = 4;                     // 16-pair column tiles per wave
+
+__global__ void pack_fragments_kernel(const uint16_t* __restrict__ W1, const uint16_t* __restrict__ W2,
+                                      const uint16_t* __restrict__ W3, uint16_t* __restrict__ frag)
 {
-    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows)
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;   // (fragment, lane, i)
+    if (idx >= kFrags * 64 * 4)
         return;
-    const float* z = logits + r * kOut;
+    const int i = idx & 3;
+    const int lane = (idx >> 2) & 63;
+    const int f = idx >> 8;
+    const uint16_t* W;
+    int N, KT, f0;
+    if (f < kFragL2) {
+        W = W1, N = kHid, KT = kKT1, f0 = kFragL1;
+    } else if (f < kFragL3) {
+        W = W2, N = kHid, KT = kKT2, f0 = kFragL2;
+    } else {
+        W = W3, N = kOut, KT = kKT2, f0 = kFragL3;
+    }
+    const int rt = (f - f0) / KT, kt = (f - f0) % KT;
+    frag[idx] = W[(size_t)(16 * kt + 4 * (lane >> 4) + i) * N + 16 * rt + (lane & 15)];
+}
+
+__device__ __forceinline__ float dense_feature(const float* __restrict__ enc, const float* __restrict__ edges,
+                                               int dim, const float* __restrict__ lo,
+                                               const float* __restrict__ range, int env, long long edge, int c)
+{
+    if (c < kEnc)
+        return enc[(size_t)env * kEnc + c];
+    if (c < kEnc + 2 * dim) {
+        const int j = (c - kEnc) % dim;
+        const int which = (c - kEnc) / dim;
+        const float q = edges[(size_t)edge * 2 * dim + which * dim + j];
+        return 2.0f * (q - lo[j]) / range[j] - 1.0f;
+    }
+    return 0.0f;
+}
+
+__global__ __launch_bounds__(256) void fused_order_kernel(
+    const bf16x4* __restrict__ frag, const float* __restrict__ b1, const float* __restrict__ b2,
+    const float* __restrict__ b3, const float* __restrict__ enc, const float* __restrict__ edges, int dim,
+    const float* __restrict__ lo, const float* __restrict__ range, int num_envs, long long num_pairs,
+    uint32_t* __restrict__ order, float* __restrict__ logits)
+{
+    __shared__ float zs[4][64][kOut + 1];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int col = lane & 15;
+    const int kq = 4 * (lane >> 4);
+    const long long base = ((long long)blockIdx.x * 4 + wave) * 64;
+
+    // layer-1 B operands: X^T blocks (6 k blocks x 4 column tiles)
+    bf16x4 xb[kKT1][kFuseCols];
+#pragma unroll
+    for (int ct = 0; ct < kFuseCols; ++ct) {
+        const long long p = base + 16 * ct + col;
+        const bool ok = p < num_pairs;
+        const int env = ok ? (int)(p % num_envs) : 0;
+        const long long edge = ok ? p / num_envs : 0;
+#pragma unroll
+        for (int kt = 0; kt < kKT1; ++kt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float v = ok ? dense_feature(enc, edges, dim, lo, range, env, edge, 16 * kt + kq + i) : 0.0f;
+                xb[kt][ct][i] = (short)f2bf(v);
+            }
+    }
+
+    // layer 1: Y1^T (256 x 64) kept as BF16 B operands of layer 2
+    bf16x4 y1[kKT2][kFuseCols];
+#pragma unroll
+    for (int rt = 0; rt < kRT12; ++rt) {
+        f32x4 acc[kFuseCols];
+#pragma unroll
+        for (int ct = 0; ct < kFuseCols; ++ct)
+            acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < kKT1; ++kt) {
+            const bf16x4 a = frag[(kFragL1 + rt * kKT1 + kt) * 64 + lane];
+#pragma unroll
+            for (int ct = 0; ct < kFuseCols; ++ct)
+                acc[ct] = mfma16(a, xb[kt][ct], acc[ct]);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float bb = b1[16 * rt + kq + v];
+#pragma unroll
+            for (int ct = 0; ct < kFuseCols; ++ct) {
+                const float z = 1.0f * acc[ct][v] + 0.0f;
+                y1[rt][ct][v] = (short)f2bf(fmaxf(z + bb, 0.0f));
+            }
+        }
+    }
+
+    // layers 2 and 3: each layer-2 row tile is layer 3's next k block
+    f32x4 acc3[kRT3][kFuseCols];
+#pragma unroll
+    for (int r3 = 0; r3 < kRT3; ++r3)
+#pragma unroll
+        for (int ct = 0; ct < kFuseCols; ++ct)
+            acc3[r3][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int rt = 0; rt < kRT12; ++rt) {
+        f32x4 acc[kFuseCols];
+#pragma unroll
+        for (int ct = 0; ct < kFuseCols; ++ct)
+            acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < kKT2; ++kt) {
+            const bf16x4 a = frag[(kFragL2 + rt * kKT2 + kt) * 64 + lane];
+#pragma unroll
+            for (int ct = 0; ct < kFuseCols; ++ct)
+                acc[ct] = mfma16(a, y1[kt][ct], acc[ct]);
+        }
+        bf16x4 y2[kFuseCols];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float bb = b2[16 * rt + kq + v];
+#pragma unroll
+            for (int ct = 0; ct < kFuseCols; ++ct) {
+                const float z = 1.0f * acc[ct][v] + 0.0f;
+                y2[ct][v] = (short)f2bf(fmaxf(z + bb, 0.0f));
+            }
+        }
+#pragma unroll
+        for (int r3 = 0; r3 < kRT3; ++r3) {
+            const bf16x4 a = frag[(kFragL3 + r3 * kKT2 + rt) * 64 + lane];
+#pragma unroll
+            for (int ct = 0; ct < kFuseCols; ++ct)
+                acc3[r3][ct] = mfma16(a, y2[ct], acc3[r3][ct]);
+        }
+    }
+
+    // logits to LDS (pair-major), then one lane per pair picks its top 4
+#pragma unroll
+    for (int r3 = 0; r3 < kRT3; ++r3)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int bin = 16 * r3 + kq + v;
+            const float bb = b3[bin];
+#pragma unroll
+            for (int ct = 0; ct < kFuseCols; ++ct)
+                zs[wave][16 * ct + col][bin] = (1.0f * acc3[r3][ct][v] + 0.0f) + bb;
+        }
+    __syncthreads();
+    const long long p = base + lane;
+    if (p >= num_pairs)
+        return;
+    const float* z = zs[wave][lane];
     uint32_t taken = 0, packed = 0;
     for (int k = 0; k < 4; ++k) {
         int best = -1;
@@ -232,7 +392,10 @@ __global__ void top4_kernel(const float* __restrict__ logits, long long rows, ui
         taken |= 1u << best;
         packed |= (uint32_t)best << (8 * k);
     }
-    order[r] = packed;
+    order[p] = packed;
+    if (logits)
+        for (int c = 0; c < kOut; ++c)
+            logits[p * kOut + c] = z[c];
 }
 
 }  // namespace bcc
@@ -327,6 +490,7 @@ struct bcc_model {
     float* v = nullptr;       // Adam second moment
     float* lo = nullptr;      // joint lower limits [dim]
     float* range = nullptr;   // joint ranges [dim]
+    uint16_t* frag = nullptr; // MFMA A fragments of the BF16 weights (fused inference)
     int cap = 0;              // scratch rows
     uint16_t *h1 = nullptr, *h2 = nullptr, *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;
     float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *d1 = nullptr, *d2 = nullptr, *loss = nullptr;
@@ -441,7 +605,7 @@ void destroy_model(bcc_model* M)
         return;
     free_scratch(M);
     for (void* p : {(void*)M->w, (void*)M->wb, (void*)M->g, (void*)M->m, (void*)M->v, (void*)M->lo,
-                    (void*)M->range, (void*)M->loss})
+                    (void*)M->range, (void*)M->loss, (void*)M->frag})
         (void)hipFree(p);
     delete M;
 }
@@ -462,6 +626,7 @@ int alloc_model(int robot, bcc_model** out)
     if (e == hipSuccess) e = hipMalloc((void**)&M->lo, sizeof(float) * 16);
     if (e == hipSuccess) e = hipMalloc((void**)&M->range, sizeof(float) * 16);
     if (e == hipSuccess) e = hipMalloc((void**)&M->loss, sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&M->frag, sizeof(uint16_t) * kFrags * 64 * 4);
     if (e == hipSuccess) e = hipMemset(M->m, 0, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMemset(M->v, 0, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMemcpy(M->lo, kLo[robot], sizeof(float) * 16, hipMemcpyHostToDevice);
@@ -719,47 +884,55 @@ int bcc_model_logits(bcc_model* M, const uint16_t* d_x, int64_t rows, float* d_l
     return BCC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Fused inference over every (edge, env) pair of a dense batch: pack the
+// weight fragments, encode the environments, one kernel for all layers and
+// the top-4 selection.  d_logits (optional) receives [pairs][32].
+int order_dense_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges, int num_edges, uint32_t* d_order,
+                      float* d_logits, hipStream_t s)
+{
+    const long long pairs = (long long)num_edges * scene->num_envs;
+    if (pairs == 0)
+        return BCC_OK;
+    float* enc = nullptr;
+    BCC_HIP(hipMalloc((void**)&enc, sizeof(float) * kEnc * scene->num_envs), "order scratch");
+    hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks_for((long long)kFrags * 64 * 4)), dim3(256), 0, s,
+                       M->wb + P_W1, M->wb + P_W2, M->wb + P_W3, M->frag);
+    hipLaunchKernelGGL(encode_env_kernel, dim3(scene->num_envs), dim3(64), 0, s, scene->d_ref_obs, scene->d_ref_meta,
+                       scene->num_envs, enc);
+    hipLaunchKernelGGL(fused_order_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s,
+                       (const bf16x4*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, enc, d_edges, M->dim, M->lo,
+                       M->range, scene->num_envs, pairs, d_order, d_logits);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(s);
+    (void)hipFree(enc);
+    if (e != hipSuccess)
+        return hip_fail(e, "fused order");
+    return BCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int bcc_model_order_dense(bcc_model* M, const bcc_scene* scene, const float* d_edges, int num_edges,
                           uint32_t* d_order, void* stream)
 {
     if (!M || !scene || num_edges < 0 || (num_edges > 0 && (!d_edges || !d_order)))
         return fail(BCC_ERR_INVALID, "bad arguments");
-    const long long pairs = (long long)num_edges * scene->num_envs;
-    if (pairs == 0)
-        return BCC_OK;
-    hipStream_t s = (hipStream_t)stream;
-    const int chunk = 131072;
-    if (int rc = ensure_scratch(M, (int)std::min<long long>(pairs, chunk)))
-        return rc;
-    float* enc = nullptr;
-    uint16_t* x = nullptr;
-    hipError_t e = hipMalloc((void**)&enc, sizeof(float) * kEnc * scene->num_envs);
-    if (e == hipSuccess)
-        e = hipMalloc((void**)&x, (size_t)2 * kIn * std::min<long long>(pairs, chunk));
-    if (e != hipSuccess) {
-        (void)hipFree(enc);
-        return hip_fail(e, "order scratch");
-    }
-    hipLaunchKernelGGL(encode_env_kernel, dim3(scene->num_envs), dim3(64), 0, s, scene->d_ref_obs, scene->d_ref_meta,
-                       scene->num_envs, enc);
-    int rc = BCC_OK;
-    for (long long p0 = 0; p0 < pairs && rc == BCC_OK; p0 += chunk) {
-        const int n = (int)std::min<long long>(chunk, pairs - p0);
-        hipLaunchKernelGGL(features_kernel, dim3(blocks_for((long long)n * kIn)), dim3(256), 0, s, enc, d_edges,
-                           M->dim, M->lo, M->range, (const int*)nullptr, (const int*)nullptr, scene->num_envs, p0,
-                           (long long)n, x);
-        rc = forward(M, x, n, s);
-        if (rc == BCC_OK)
-            hipLaunchKernelGGL(top4_kernel, dim3(blocks_for(n)), dim3(256), 0, s, M->z3, (long long)n, d_order + p0);
-    }
-    e = hipStreamSynchronize(s);
-    (void)hipFree(enc);
-    (void)hipFree(x);
-    if (rc != BCC_OK)
-        return rc;
-    if (e != hipSuccess)
-        return hip_fail(e, "order");
-    return BCC_OK;
+    return order_dense_fused(M, scene, d_edges, num_edges, d_order, nullptr, (hipStream_t)stream);
+}
+
+int bcc_model_logits_dense(bcc_model* M, const bcc_scene* scene, const float* d_edges, int num_edges,
+                           uint32_t* d_order, float* d_logits, void* stream)
+{
+    if (!M || !scene || num_edges < 0 || (num_edges > 0 && (!d_edges || !d_order || !d_logits)))
+        return fail(BCC_ERR_INVALID, "bad arguments");
+    return order_dense_fused(M, scene, d_edges, num_edges, d_order, d_logits, (hipStream_t)stream);
 }
 
 int bcc_debug_gemm(int ta, int tb, const uint16_t* d_a, int lda, const uint16_t* d_b, int ldb, float* d_c, int ldc,

@@ -233,9 +233,14 @@ int bcc_model_grads(bcc_model *model, const uint16_t *d_x, const uint32_t *d_lab
 int bcc_model_train(bcc_model *model, const uint16_t *d_x, const uint32_t *d_labels, int num_rows, int epochs,
                     int batch, float lr, float *epoch_loss);
 int bcc_model_logits(bcc_model *model, const uint16_t *d_x, int64_t rows, float *d_logits);
-/* top-4 bins of every (edge, env) pair of a dense batch into d_order [num_edges][num_envs] */
+/* top-4 bins of every (edge, env) pair of a dense batch into d_order [num_edges][num_envs]
+ * (one fused kernel: features, three MFMA layers and the selection; same
+ * values as bcc_model_logits on bcc_model_features rows) */
 int bcc_model_order_dense(bcc_model *model, const bcc_scene *scene, const float *d_edges, int num_edges,
                           uint32_t *d_order, void *stream);
+/* the same, also writing every pair's 32 logits into d_logits [num_edges][num_envs][32] */
+int bcc_model_logits_dense(bcc_model *model, const bcc_scene *scene, const float *d_edges, int num_edges,
+                           uint32_t *d_order, float *d_logits, void *stream);
 /* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
 int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,
                    int m, int n, int k, float alpha, float beta);

@@ -214,3 +214,37 @@ def test_trained_model_gain_over_untrained_and_plain(bcc, gpu):
     # step 0 is shared by every order; later rounds follow the prediction
     assert res["trained"][2] == res["plain"][2]
     assert res["trained"][1] <= res["untrained"][1] + 1e-3
+
+
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+def test_fused_inference_matches_layer_by_layer(bcc, gpu, robot):
+    """The fused kernel (features, three MFMA layers, top-4 in one launch)
+    gives the same logits, bit for bit, as the layer-by-layer GEMM path on the
+    same rows, and the top-4 of those logits with ties to the lower bin."""
+    from batch_cc_amd import workloads as W
+    rng = np.random.default_rng(12)
+    envs = [W.c0_env(rng, n_cub=3, n_sph=2, n_cyl=2) for _ in range(7)]
+    sc = bcc.PackedScene.from_envs(envs)
+    edges = W.uniform_edges(robot, 53, rng)          # 371 pairs: the last wave tile is partial
+    M, E = edges.shape[0], sc.num_envs
+    model = bcc.Model(robot, seed=4)
+    p = model.params()
+    o = 0
+    for name, shape in mlp_ref.SIZES:
+        n = int(np.prod(shape))
+        if name.startswith("b"):
+            p[o:o + n] = rng.standard_normal(n).astype(np.float32) * 0.1
+        o += n
+    model.set_params(p)
+    with bcc.Scene(sc) as scene:
+        de = bcc.DeviceBuffer.from_array(edges)
+        order, logits = model.logits_dense(scene, de, M)
+        pair = np.arange(M * E)
+        x = model.features(scene, de, (pair % E).astype(np.int32), (pair // E).astype(np.int32))
+        ref = model.logits(x, M * E).reshape(M, E, 32)
+        order2 = model.order_dense(scene, de, M).to_array(np.uint32, (M, E))
+    assert np.array_equal(logits.view(np.uint32), ref.view(np.uint32))
+    top = np.argsort(-ref, axis=2, kind="stable")[:, :, :4].astype(np.uint32)
+    exp = top[..., 0] | top[..., 1] << 8 | top[..., 2] << 16 | top[..., 3] << 24
+    assert np.array_equal(order, exp)
+    assert np.array_equal(order2, order)
