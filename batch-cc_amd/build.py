@@ -41,10 +41,11 @@ def run(cmd, **kw):
     return out
 
 
-def generate():
+def generate(order: str = "tree", gate_chunk: int = 4, fine_chunk: int = 8):
     gen = os.path.join(HERE, "robots", "gen_robot_code.py")
     for r in ROBOTS:
-        run([sys.executable, gen, os.path.join(HERE, "robots", f"{r}.json"), os.path.join(CSRC, "robots")])
+        run([sys.executable, gen, os.path.join(HERE, "robots", f"{r}.json"), os.path.join(CSRC, "robots"),
+             "--order", order, "--gate-chunk", str(gate_chunk), "--fine-chunk", str(fine_chunk)])
 
 
 def compile_one(src, objdir=OBJDIR, defines=()):
@@ -57,11 +58,13 @@ def build_oracle():
     run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
 
 
-def build(gen: bool = True, jobs: int = 4, defines=(), out: str = LIB) -> str:
+def build(gen: bool = True, jobs: int = 4, defines=(), out: str = LIB, order: str = "tree", gate_chunk: int = 4,
+          fine_chunk: int = 8) -> str:
     """Compile and link; `defines` adds -D macros (tuning variants), `out`
-    writes the library elsewhere (the in-tree libbcc.so is the product)."""
+    writes the library elsewhere (the in-tree libbcc.so is the product),
+    `order` picks the generated check order (see gen_robot_code.schedule)."""
     if gen:
-        generate()
+        generate(order, gate_chunk, fine_chunk)
     objdir = OBJDIR if out == LIB else out + ".objs"
     os.makedirs(objdir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -80,8 +83,12 @@ def main():
     ap.add_argument("-j", type=int, default=4)
     ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor macro")
     ap.add_argument("--out", default=LIB, help="library path (default: the in-tree libbcc.so)")
+    ap.add_argument("--order", default="tree", choices=["tree", "program"], help="generated check order")
+    ap.add_argument("--gate-chunk", type=int, default=4, help="gate spheres per obstacle sweep")
+    ap.add_argument("--fine-chunk", type=int, default=8, help="fine spheres per gated sweep")
     a = ap.parse_args()
-    print(build(gen=not a.no_gen, jobs=a.j, defines=a.defines, out=os.path.abspath(a.out)))
+    print(build(gen=not a.no_gen, jobs=a.j, defines=a.defines, out=os.path.abspath(a.out), order=a.order,
+                gate_chunk=a.gate_chunk, fine_chunk=a.fine_chunk))
 
 
 if __name__ == "__main__":

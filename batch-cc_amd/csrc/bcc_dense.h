@@ -131,16 +131,40 @@ __device__ __forceinline__ int setup_pair_sd(const float* __restrict__ se, int r
     return n;
 }
 
-// order (optional): learned top-4 bins per pair, [edge][env]; when given the
-// half visits samples in ordered_sample order, otherwise in plain order.
+// Per-thread sample configuration (and, in plain order, the per-step
+// increment) live in LDS, [joint][thread], not in registers: the check
+// program reads q[j] once per joint (for its sin/cos) and the step loop adds
+// d once per step, so neither needs to hold VGPRs across the whole program,
+// which leaves the registers to the frames and sweep temporaries.
+struct LdsRow {
+    float* p;   // element j at p[j * kBlock]
+    __device__ __forceinline__ float operator[](int j) const { return p[j * kBlock]; }
+    __device__ __forceinline__ float& at(int j) const { return p[j * kBlock]; }
+};
+
+// Plain-order pair setup into LDS rows (same arithmetic as setup_pair).
+template <int D>
+__device__ __forceinline__ int setup_pair_lds(const float* __restrict__ se, int resolution, int t, const LdsRow& q,
+                                              const LdsRow& d)
+{
+    float qq[D], dd[D];
+    const int n = setup_pair<D>(se, resolution, t, qq, dd);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+        q.at(j) = qq[j];
+        d.at(j) = dd[j];
+    }
+    return n;
+}
+
+// kOrdered: a learned top-4 order per pair, order[edge][env], decides which
+// samples each round visits (ordered_sample); otherwise plain order.
 // rounds (optional): number of rounds (steps) each pair ran, [edge][env].
-template <class Robot>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENSE_WAVES, 8))) void dense_kernel(const float* __restrict__ obs, const int* __restrict__ meta,
-                                                       const float* __restrict__ edges, int num_edges, int num_envs,
-                                                       int resolution, int chunk, int chunks_per_env,
-                                                       const uint32_t* __restrict__ order,
-                                                       uint8_t* __restrict__ rounds,
-                                                       uint8_t* __restrict__ out)
+template <class Robot, bool kOrdered>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENSE_WAVES, 8))) void dense_kernel(
+    const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges, int num_edges,
+    int num_envs, int resolution, int chunk, int chunks_per_env, const uint32_t* __restrict__ order,
+    uint8_t* __restrict__ rounds, uint8_t* __restrict__ out)
 {
     constexpr int D = Robot::dim;
     const int env = blockIdx.x / chunks_per_env;
@@ -149,6 +173,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
     const int e_end = min(num_edges, e_begin + chunk);
 
     __shared__ int next_edge;
+    __shared__ float qs[D][kBlock];
+    __shared__ float ds[kOrdered ? 1 : D][kBlock];
     if (threadIdx.x == 0)
         next_edge = e_begin;
     __syncthreads();
@@ -157,13 +183,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
     const int lane = threadIdx.x & 63;
     const int t = lane & 31;
     const int half0 = lane & 32;
+    const LdsRow Q{&qs[0][threadIdx.x]};
+    const LdsRow Dl{&ds[0][threadIdx.x]};
 
-    float q[D], d[D], s0[D];
+    float s0[kOrdered ? D : 1], d[kOrdered ? D : 1];
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-        q[j] = 0.0f;
-        d[j] = 0.0f;
+    for (int j = 0; j < (kOrdered ? D : 1); ++j) {
         s0[j] = 0.0f;
+        d[j] = 0.0f;
     }
     int cur = -1, step = 0, n = 0;
     uint32_t top = 0;
@@ -179,16 +206,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
             if (g < e_end) {
                 cur = g;
                 step = 0;
-                if (order) {
+                if constexpr (kOrdered) {
                     n = setup_pair_sd<D>(edges + (size_t)g * 2 * D, resolution, s0, d);
                     top = order[(size_t)g * num_envs + env];
                     const uint32_t taken = (1u << (top & 31)) | (1u << ((top >> 8) & 31)) |
                                            (1u << ((top >> 16) & 31)) | (1u << ((top >> 24) & 31));
                     if (__popc(taken) != 4 || (top & 0xe0e0e0e0u))
                         top = 0xffffffffu;
-                    ordered_sample<D>(s0, d, n, top, 0, t, q);
+                    float qq[D];
+                    ordered_sample<D>(s0, d, n, top, 0, t, qq);
+#pragma unroll
+                    for (int j = 0; j < D; ++j)
+                        Q.at(j) = qq[j];
                 } else {
-                    n = setup_pair<D>(edges + (size_t)g * 2 * D, resolution, t, q, d);
+                    n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
                 }
             } else {
                 exhausted = true;
@@ -197,7 +228,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
         const uint64_t live = __ballot(cur >= 0);
         if (live == 0)
             break;
-        const uint64_t hit = Robot::check(q, E, live);
+        const uint64_t hit = Robot::check(Q, E, live);
         if (cur >= 0) {
             const bool h = (hit >> lane) & 1ull;
             bool fin = h;
@@ -205,12 +236,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
                 ++step;
                 fin = (step == n);
                 if (!fin) {
-                    if (order) {
-                        ordered_sample<D>(s0, d, n, top, step, t, q);
+                    if constexpr (kOrdered) {
+                        float qq[D];
+                        ordered_sample<D>(s0, d, n, top, step, t, qq);
+#pragma unroll
+                        for (int j = 0; j < D; ++j)
+                            Q.at(j) = qq[j];
                     } else {
 #pragma unroll
                         for (int j = 0; j < D; ++j)
-                            q[j] = q[j] + d[j];
+                            Q.at(j) = Q[j] + Dl[j];
                     }
                 }
             }
@@ -251,17 +286,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
     constexpr int D = Robot::dim;
     constexpr int kWaves = kBlock / 64;
     __shared__ int next_pair[kWaves];
+    __shared__ float qs[D][kBlock];
+    __shared__ float ds[D][kBlock];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int t = lane & 31;
     const int half0 = lane & 32;
+    const LdsRow Q{&qs[0][threadIdx.x]};
+    const LdsRow Dl{&ds[0][threadIdx.x]};
 
-    float q[D], d[D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-        q[j] = 0.0f;
-        d[j] = 0.0f;
-    }
     for (;;) {
         // -- take a tile (wave-uniform)
         int tile = 0;
@@ -290,7 +323,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
                 if (g < p_end) {
                     cur = g;
                     step = 0;
-                    n = setup_pair<D>(edges + (size_t)g * 2 * D, resolution, t, q, d);
+                    n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
                 } else {
                     exhausted = true;
                 }
@@ -298,7 +331,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
             const uint64_t live = __ballot(cur >= 0);
             if (live == 0)
                 break;
-            const uint64_t hit = Robot::check(q, E, live);
+            const uint64_t hit = Robot::check(Q, E, live);
             if (cur >= 0) {
                 const bool h = (hit >> lane) & 1ull;
                 bool fin = h;
@@ -308,7 +341,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
                     if (!fin) {
 #pragma unroll
                         for (int j = 0; j < D; ++j)
-                            q[j] = q[j] + d[j];
+                            Q.at(j) = Q[j] + Dl[j];
                     }
                 }
                 if (fin) {
@@ -629,8 +662,14 @@ hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
     const long long blocks = (long long)chunks_per_env * a.num_envs;
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(dense_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
-                       a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order, a.rounds, a.out);
+    if (a.order)
+        hipLaunchKernelGGL((dense_kernel<Robot, true>), dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.obs, a.meta,
+                           a.edges, a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order, a.rounds,
+                           a.out);
+    else
+        hipLaunchKernelGGL((dense_kernel<Robot, false>), dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.obs,
+                           a.meta, a.edges, a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order,
+                           a.rounds, a.out);
     return hipGetLastError();
 }
 

@@ -275,11 +275,19 @@ __device__ __forceinline__ bool bcc_ref_env_hit(const RefEnvView& E, float x, fl
 // (N independent tests per obstacle keep the VALU pipeline full).  `rp` are
 // the padded radii float(double(r) + 0.005) (src/utils.cuh:215).
 
-#define BCC_SWEEP(count, stride, base, TEST)                     \
-    _Pragma("unroll 2") for (int k = 0; k < (count); ++k) {      \
-        const cfptr o = (base) + (stride) * k;                   \
-        _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; }  \
-        SETTLED_CHECK;                                           \
+// Obstacle loop unroll factor (a tuning knob: more unrolling overlaps scalar
+// loads with tests, but keeps more temporaries live).
+#ifndef BCC_SWEEP_UNROLL
+#define BCC_SWEEP_UNROLL 2
+#endif
+#define BCC_PRAGMA_(x) _Pragma(#x)
+#define BCC_PRAGMA(x) BCC_PRAGMA_(x)
+
+#define BCC_SWEEP(count, stride, base, TEST)                           \
+    BCC_PRAGMA(unroll BCC_SWEEP_UNROLL) for (int k = 0; k < (count); ++k) { \
+        const cfptr o = (base) + (stride) * k;                         \
+        _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; }        \
+        SETTLED_CHECK;                                                 \
     }
 
 // Per-lane, per-sphere hit flags; no early exit (gate spheres and lone

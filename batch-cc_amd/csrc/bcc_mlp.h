@@ -39,14 +39,17 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
 //   op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
 // 256 threads compute a 64x64 tile of C; each wave a 32x32 sub-tile
 // (2x2 MFMA tiles).  K advances 16 per step through LDS; out-of-range
-// elements are read as zero.
+// elements are read as zero.  Each operand is staged with consecutive
+// threads on its contiguous dimension.  Split K: block z covers
+// k in [z*kchunk, (z+1)*kchunk) and writes its tile to C + z*cstride (the
+// caller reduces the partial tiles).
 constexpr int kGemmTile = 64;
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restrict__ A, int lda,
                                                        const uint16_t* __restrict__ B, int ldb,
                                                        float* __restrict__ C, int ldc, int M, int N, int K,
-                                                       float alpha, float beta)
+                                                       float alpha, float beta, int kchunk, long long cstride)
 {
     __shared__ uint16_t As[kGemmTile][16 + 4];   // [m][k]
     __shared__ uint16_t Bs[kGemmTile][16 + 4];   // [n][k]
@@ -57,6 +60,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
     const int n0 = blockIdx.x * kGemmTile;
     const int wm = (wave >> 1) * 32;
     const int wn = (wave & 1) * 32;
+    const int kbeg = blockIdx.z * kchunk;
+    const int kend = min(K, kbeg + kchunk);
+    C += blockIdx.z * cstride;
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -64,22 +70,29 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
         for (int j = 0; j < 2; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
         // 64x16 elements of each operand, 4 per thread
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int idx = tid + 256 * r;   // 0..1023
-            const int mm = idx >> 4, kk = idx & 15;
-            const int gm = m0 + mm, gk = k0 + kk;
-            uint16_t va = 0;
-            if (gm < M && gk < K)
-                va = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-            As[mm][kk] = va;
-            const int gn = n0 + mm;
-            uint16_t vb = 0;
-            if (gn < N && gk < K)
-                vb = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-            Bs[mm][kk] = vb;
+            {
+                const int mm = TA ? (idx & 63) : (idx >> 4);
+                const int kk = TA ? (idx >> 6) : (idx & 15);
+                const int gm = m0 + mm, gk = k0 + kk;
+                uint16_t va = 0;
+                if (gm < M && gk < kend)
+                    va = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+                As[mm][kk] = va;
+            }
+            {
+                const int nn = TB ? (idx >> 4) : (idx & 63);
+                const int kk = TB ? (idx & 15) : (idx >> 6);
+                const int gn = n0 + nn, gk = k0 + kk;
+                uint16_t vb = 0;
+                if (gn < N && gk < kend)
+                    vb = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+                Bs[nn][kk] = vb;
+            }
         }
         __syncthreads();
         const int kq = 4 * (lane >> 4);
@@ -109,6 +122,21 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
                     *c = alpha * acc[i][j][v] + (beta != 0.0f ? beta * *c : 0.0f);
                 }
             }
+}
+
+// Sum of split-K partial tiles in split order, then C = alpha * sum + beta * C.
+__global__ void splitk_reduce_kernel(const float* __restrict__ part, int splits, long long cstride, int M, int N,
+                                     float* __restrict__ C, int ldc, float alpha, float beta)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)M * N)
+        return;
+    const int m = (int)(idx / N), n = (int)(idx - (long long)m * N);
+    float s = 0.0f;
+    for (int z = 0; z < splits; ++z)
+        s += part[z * cstride + idx];
+    float* c = C + (size_t)m * ldc + n;
+    *c = alpha * s + (beta != 0.0f ? beta * *c : 0.0f);
 }
 
 }  // namespace bcc

@@ -429,24 +429,46 @@ unsigned blocks_for(long long n, int bs = 256)
     return (unsigned)((n + bs - 1) / bs);
 }
 
+// Split-K workspace: enough for 32 partial 256 x 256 tiles.
+constexpr long long kSplitWs = 32LL * kHid * kHid;
+
+// C = alpha op(A) op(B) + beta C.  Long, thin products (K >= 1024 and fewer
+// than 64 output tiles: the weight gradients, K = batch rows) are split along
+// K into the workspace `ws` (kSplitWs floats) and reduced in split order, so
+// results do not depend on scheduling.
 hipError_t gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B, int ldb, float* C, int ldc, int M,
-                int N, int K, float alpha, float beta, hipStream_t s)
+                int N, int K, float alpha, float beta, hipStream_t s, float* ws = nullptr)
 {
     if (M <= 0 || N <= 0)
         return hipSuccess;
-    const dim3 grid((N + kGemmTile - 1) / kGemmTile, (M + kGemmTile - 1) / kGemmTile);
+    const int tiles = ((N + kGemmTile - 1) / kGemmTile) * ((M + kGemmTile - 1) / kGemmTile);
+    int splits = 1;
+    if (ws && K >= 1024 && tiles < 64) {
+        splits = std::min<long long>(std::min(32, K / 256), kSplitWs / ((long long)M * N));
+        splits = std::max(splits, 1);
+    }
+    const int kchunk = splits > 1 ? ((K + splits - 1) / splits + 15) / 16 * 16 : std::max(K, 1);
+    splits = (K + kchunk - 1) / kchunk;
+    float* out = splits > 1 ? ws : C;
+    const int ldo = splits > 1 ? N : ldc;
+    const float a = splits > 1 ? 1.0f : alpha, b = splits > 1 ? 0.0f : beta;
+    const long long cstride = splits > 1 ? (long long)M * N : 0;
+    const dim3 grid((N + kGemmTile - 1) / kGemmTile, (M + kGemmTile - 1) / kGemmTile, std::max(splits, 1));
     if (!ta && !tb)
-        hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
-                           alpha, beta);
+        hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, dim3(256), 0, s, A, lda, B, ldb, out, ldo, M, N, K,
+                           a, b, kchunk, cstride);
     else if (ta && !tb)
-        hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
-                           alpha, beta);
+        hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, dim3(256), 0, s, A, lda, B, ldb, out, ldo, M, N, K,
+                           a, b, kchunk, cstride);
     else if (!ta && tb)
-        hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
-                           alpha, beta);
+        hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, dim3(256), 0, s, A, lda, B, ldb, out, ldo, M, N, K,
+                           a, b, kchunk, cstride);
     else
-        hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
-                           alpha, beta);
+        hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, dim3(256), 0, s, A, lda, B, ldb, out, ldo, M, N, K,
+                           a, b, kchunk, cstride);
+    if (splits > 1)
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks_for((long long)M * N)), dim3(256), 0, s, ws, splits,
+                           cstride, M, N, C, ldc, alpha, beta);
     return hipGetLastError();
 }
 
@@ -491,6 +513,7 @@ struct bcc_model {
     float* lo = nullptr;      // joint lower limits [dim]
     float* range = nullptr;   // joint ranges [dim]
     uint16_t* frag = nullptr; // MFMA A fragments of the BF16 weights (fused inference)
+    float* ws = nullptr;      // split-K partial tiles (weight gradients)
     int cap = 0;              // scratch rows
     uint16_t *h1 = nullptr, *h2 = nullptr, *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;
     float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *d1 = nullptr, *d2 = nullptr, *loss = nullptr;
@@ -574,17 +597,17 @@ int backward(bcc_model* M, const uint16_t* x, const uint32_t* labels, int rows, 
     const uint16_t* W3 = M->wb + P_W3;
     hipLaunchKernelGGL(bce_grad_kernel, dim3(blocks_for((long long)rows * kOut)), dim3(256), 0, s, M->z3, labels,
                        rows, M->g3, M->loss);
-    BCC_HIP(gemm(true, false, M->h2, kHid, M->g3, kOut, grad + P_W3, kOut, kHid, kOut, rows, 1.f, 0.f, s), "dW3");
+    BCC_HIP(gemm(true, false, M->h2, kHid, M->g3, kOut, grad + P_W3, kOut, kHid, kOut, rows, 1.f, 0.f, s, M->ws), "dW3");
     hipLaunchKernelGGL(colsum_kernel, dim3(kOut), dim3(256), 0, s, M->g3, rows, kOut, grad + P_B3);
     BCC_HIP(gemm(false, true, M->g3, kOut, W3, kOut, M->d2, kHid, rows, kHid, kOut, 1.f, 0.f, s), "dH2");
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for((long long)rows * kHid)), dim3(256), 0, s, M->d2, M->z2,
                        (long long)rows * kHid, M->g2);
-    BCC_HIP(gemm(true, false, M->h1, kHid, M->g2, kHid, grad + P_W2, kHid, kHid, kHid, rows, 1.f, 0.f, s), "dW2");
+    BCC_HIP(gemm(true, false, M->h1, kHid, M->g2, kHid, grad + P_W2, kHid, kHid, kHid, rows, 1.f, 0.f, s, M->ws), "dW2");
     hipLaunchKernelGGL(colsum_kernel, dim3(kHid), dim3(256), 0, s, M->g2, rows, kHid, grad + P_B2);
     BCC_HIP(gemm(false, true, M->g2, kHid, W2, kHid, M->d1, kHid, rows, kHid, kHid, 1.f, 0.f, s), "dH1");
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for((long long)rows * kHid)), dim3(256), 0, s, M->d1, M->z1,
                        (long long)rows * kHid, M->g1);
-    BCC_HIP(gemm(true, false, x, kIn, M->g1, kHid, grad + P_W1, kHid, kIn, kHid, rows, 1.f, 0.f, s), "dW1");
+    BCC_HIP(gemm(true, false, x, kIn, M->g1, kHid, grad + P_W1, kHid, kIn, kHid, rows, 1.f, 0.f, s, M->ws), "dW1");
     hipLaunchKernelGGL(colsum_kernel, dim3(kHid), dim3(256), 0, s, M->g1, rows, kHid, grad + P_B1);
     BCC_HIP(hipGetLastError(), "backward");
     return BCC_OK;
@@ -605,7 +628,7 @@ void destroy_model(bcc_model* M)
         return;
     free_scratch(M);
     for (void* p : {(void*)M->w, (void*)M->wb, (void*)M->g, (void*)M->m, (void*)M->v, (void*)M->lo,
-                    (void*)M->range, (void*)M->loss, (void*)M->frag})
+                    (void*)M->range, (void*)M->loss, (void*)M->frag, (void*)M->ws})
         (void)hipFree(p);
     delete M;
 }
@@ -627,6 +650,7 @@ int alloc_model(int robot, bcc_model** out)
     if (e == hipSuccess) e = hipMalloc((void**)&M->range, sizeof(float) * 16);
     if (e == hipSuccess) e = hipMalloc((void**)&M->loss, sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void**)&M->frag, sizeof(uint16_t) * kFrags * 64 * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&M->ws, sizeof(float) * kSplitWs);
     if (e == hipSuccess) e = hipMemset(M->m, 0, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMemset(M->v, 0, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMemcpy(M->lo, kLo[robot], sizeof(float) * 16, hipMemcpyHostToDevice);
@@ -940,8 +964,14 @@ int bcc_debug_gemm(int ta, int tb, const uint16_t* d_a, int lda, const uint16_t*
 {
     if (m < 0 || n < 0 || k < 0 || (m > 0 && n > 0 && (!d_a || !d_b || !d_c)))
         return fail(BCC_ERR_INVALID, "bad gemm arguments");
-    BCC_HIP(gemm(ta != 0, tb != 0, d_a, lda, d_b, ldb, d_c, ldc, m, n, k, alpha, beta, nullptr), "gemm");
-    BCC_HIP(hipDeviceSynchronize(), "gemm");
+    float* ws = nullptr;   // split-K workspace, as the model's weight gradients use
+    BCC_HIP(hipMalloc((void**)&ws, sizeof(float) * kSplitWs), "gemm workspace");
+    hipError_t e = gemm(ta != 0, tb != 0, d_a, lda, d_b, ldb, d_c, ldc, m, n, k, alpha, beta, nullptr, ws);
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    (void)hipFree(ws);
+    if (e != hipSuccess)
+        return hip_fail(e, "gemm");
     return BCC_OK;
 }
 

@@ -32,7 +32,8 @@ The generator folds operations on the constants 0, 1 and -1 only where the
 folded form is IEEE-identical for finite operands (up to the sign of zero,
 which no collision predicate can observe).
 
-Usage: python gen_robot_code.py <model.json> <out_dir>
+Usage: python gen_robot_code.py <model.json> <out_dir> [--order tree|program]
+                                 [--gate-chunk N] [--fine-chunk N]
 """
 from __future__ import annotations
 
@@ -219,20 +220,78 @@ def padded_radius(r: float) -> float:
 
 GATE_CHUNK = 4      # gate/lone spheres per environment sweep
 FINE_CHUNK = 8      # fine spheres per gated sweep
+ORDER = "tree"      # "tree": kinematic-tree order; "program": the reference's order
 
 
-def gen_check(model: dict) -> str:
-    """check(): environment groups in the reference's program order, GATE_CHUNK
-    gate (or lone) spheres per obstacle sweep, each chunk followed by one
-    sweep per gated group over its fine spheres; then the self-collision
-    program (all sphere-pair gates, then the gated fine cross products).
-    The evaluation order differs from the reference's interleaving; the
-    verdict (an OR over gated tests) does not depend on it."""
+def tree_rank(model: dict) -> dict:
+    """Depth-first rank of every joint frame (children in index order); the
+    base frame -1 ranks first."""
+    children: dict[int, list[int]] = {}
+    for j, jt in enumerate(model["joints"]):
+        children.setdefault(jt["parent"], []).append(j)
+    order: list[int] = []
+
+    def visit(p):
+        for c in children.get(p, []):
+            order.append(c)
+            visit(c)
+
+    visit(-1)
+    rank = {-1: -1}
+    for i, j in enumerate(order):
+        rank[j] = i
+    return rank
+
+
+def schedule(model: dict, order: str):
+    """Emission order: a list of ("env", [group indices]) chunks of up to
+    GATE_CHUNK environment groups and ("self", group index) items.
+
+    "program": environment groups in the reference's program order, then the
+    whole self-collision program.  "tree": every item keyed by the deepest
+    frame (in depth-first rank) it reads, environment chunks in key order and
+    each self group right after the first chunk whose key reaches its own, so
+    a frame's registers die once the items on it are done instead of living
+    until the self program at the end."""
+    sph = model["spheres"]
+    envp = model["env_program"]
+    prog = [it["index"] for it in model["program"] if it["kind"] == "env"]
+    nself = len(model["self_program"])
+    if order == "program":
+        return ([("env", prog[c:c + GATE_CHUNK]) for c in range(0, len(prog), GATE_CHUNK)]
+                + [("self", i) for i in range(nself)])
+    rank = tree_rank(model)
+
+    def key(spheres):
+        return max(rank[sph[s]["frame"]] for s in spheres)
+
+    env_sorted = sorted(prog, key=lambda gi: (key([envp[gi]["gate"]] + envp[gi]["fine"]), prog.index(gi)))
+    chunks = [env_sorted[c:c + GATE_CHUNK] for c in range(0, len(env_sorted), GATE_CHUNK)]
+    selfs = sorted(range(nself), key=lambda i: (key(model["self_program"][i]["gate"]), i))
+    out, si = [], 0
+    for ch in chunks:
+        k = max(key([envp[gi]["gate"]] + envp[gi]["fine"]) for gi in ch)
+        out.append(("env", ch))
+        while si < len(selfs) and key(model["self_program"][selfs[si]]["gate"]) <= k:
+            out.append(("self", selfs[si]))
+            si += 1
+    out += [("self", i) for i in selfs[si:]]
+    return out
+
+
+def gen_check(model: dict, order: str | None = None) -> str:
+    """check(): environment groups in chunks of GATE_CHUNK gate (or lone)
+    spheres per obstacle sweep, each chunk followed by one sweep per gated
+    group over its fine spheres, and the self-collision groups (sphere-pair
+    gate, then the gated fine cross product), in the order schedule()
+    gives.  The evaluation order differs from the reference's interleaving;
+    the verdict (an OR over gated tests) does not depend on it."""
+    order = order or ORDER
     robot = model["robot"]
     dim = model["dimension"]
     sph = model["spheres"]
     em = Emitter()
-    em.line(f"// generated by gen_robot_code.py from the {robot} model; do not edit")
+    em.line(f"// generated by gen_robot_code.py from the {robot} model ({order} order); do not edit")
     frames = LazyFK(em, model)
     em.line("uint64_t hit = 0;")
 
@@ -250,114 +309,109 @@ def gen_check(model: dict) -> str:
             cache[idx] = pts
         return pts
 
-    def emit_env(gate_chunk):
+    def emit_env_chunk(chunk):
         envp = model["env_program"]
-        order = [it["index"] for it in model["program"] if it["kind"] == "env"]
-        for c0 in range(0, len(order), gate_chunk):
-            chunk = order[c0:c0 + gate_chunk]
-            ng = len(chunk)
-            need_frames([envp[gi]["gate"] for gi in chunk] + [f for gi in chunk for f in envp[gi]["fine"]])
-            em.line("{")
-            em.indent += 1
-            gx, gy, gz, grp = [], [], [], []
-            for gi in chunk:
-                g = envp[gi]
-                c = c3(g["gate"], f"s{g['gate']}", reuse=False)
-                gx.append(c[0]); gy.append(c[1]); gz.append(c[2])
-                grp.append(flit(padded_radius(sph[g["gate"]]["radius"])))
-            em.line(f"const float gx[{ng}] = {{{', '.join(gx)}}};")
-            em.line(f"const float gy[{ng}] = {{{', '.join(gy)}}};")
-            em.line(f"const float gz[{ng}] = {{{', '.join(gz)}}};")
-            em.line(f"const float grp[{ng}] = {{{', '.join(grp)}}};")
-            em.line(f"bool gh[{ng}];")
-            em.line(f"bcc_env_sweep<{ng}>(env, gx, gy, gz, grp, gh);")
-            for k, gi in enumerate(chunk):
-                g = envp[gi]
-                em.line(f"// env group {gi}" + (f" ({g['label']})" if g["label"] else ""))
-                if g["lone"]:
-                    em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot(gh[{k}]) & live); hit |= d; live &= ~d; }}")
-                    continue
-                em.line(f"if (uint64_t need = __ballot(gh[{k}]) & live) {{")
-                em.indent += 1
-                fines = g["fine"]
-                for f0 in range(0, len(fines), FINE_CHUNK):
-                    sub = fines[f0:f0 + FINE_CHUNK]
-                    nf = len(sub)
-                    em.line("if (need) {")
-                    em.indent += 1
-                    fx, fy, fz, frp = [], [], [], []
-                    for fi, f in enumerate(sub):
-                        c = c3(f, f"f{gi}_{f0 + fi}", reuse=False)
-                        fx.append(c[0]); fy.append(c[1]); fz.append(c[2])
-                        frp.append(flit(padded_radius(sph[f]["radius"])))
-                    em.line(f"const float fx[{nf}] = {{{', '.join(fx)}}};")
-                    em.line(f"const float fy[{nf}] = {{{', '.join(fy)}}};")
-                    em.line(f"const float fz[{nf}] = {{{', '.join(fz)}}};")
-                    em.line(f"const float frp[{nf}] = {{{', '.join(frp)}}};")
-                    em.line(f"const bool h = bcc_env_any<{nf}>(env, fx, fy, fz, frp, need);")
-                    em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
-                    em.line("hit |= d; live &= ~d; need &= ~d;")
-                    em.indent -= 1
-                    em.line("}")
-                em.indent -= 1
-                em.line("}")
-            em.indent -= 1
-            em.line("}")
-            em.line("if (!live) return hit;")
-
-    def emit_self():
-        selfp = model["self_program"]
-        gates = []
-        for g in selfp:
-            for s in g["gate"]:
-                if s not in gates:
-                    gates.append(s)
-        need_frames(gates)
-        gc = {s: c3(s, f"sg{s}", reuse=False) for s in gates}
-        for gi, g in enumerate(selfp):
-            a, b = g["gate"]
-            ra = flit(sph[a]["radius"])
-            rb = flit(sph[b]["radius"])
-            em.line(f"// self group {gi}" + (f" ({g['label']})" if g["label"] else ""))
-            gate = (f"bcc_self_hit({gc[a][0]}, {gc[a][1]}, {gc[a][2]}, {ra}, "
-                    f"{gc[b][0]}, {gc[b][1]}, {gc[b][2]}, {rb})")
+        ng = len(chunk)
+        need_frames([envp[gi]["gate"] for gi in chunk] + [f for gi in chunk for f in envp[gi]["fine"]])
+        em.line("{")
+        em.indent += 1
+        gx, gy, gz, grp = [], [], [], []
+        for gi in chunk:
+            g = envp[gi]
+            c = c3(g["gate"], f"s{g['gate']}", reuse=False)
+            gx.append(c[0]); gy.append(c[1]); gz.append(c[2])
+            grp.append(flit(padded_radius(sph[g["gate"]]["radius"])))
+        em.line(f"const float gx[{ng}] = {{{', '.join(gx)}}};")
+        em.line(f"const float gy[{ng}] = {{{', '.join(gy)}}};")
+        em.line(f"const float gz[{ng}] = {{{', '.join(gz)}}};")
+        em.line(f"const float grp[{ng}] = {{{', '.join(grp)}}};")
+        em.line(f"bool gh[{ng}];")
+        em.line(f"bcc_env_sweep<{ng}>(env, gx, gy, gz, grp, gh);")
+        for k, gi in enumerate(chunk):
+            g = envp[gi]
+            em.line(f"// env group {gi}" + (f" ({g['label']})" if g["label"] else ""))
             if g["lone"]:
-                em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot({gate}) & live); hit |= d; live &= ~d; }}")
+                em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot(gh[{k}]) & live); hit |= d; live &= ~d; }}")
                 continue
-            em.line(f"if (uint64_t need = __ballot({gate}) & live) {{")
+            em.line(f"if (uint64_t need = __ballot(gh[{k}]) & live) {{")
             em.indent += 1
-            # frames for the fine spheres are recomputed in this (rarely taken)
-            # block instead of being kept live through the whole section
-            local = LazyFK(em, model, prefix=f"L{gi}_")
-            A = []
-            B = []
-            for x, y in g["fine"]:
-                if x not in A:
-                    A.append(x)
-                if y not in B:
-                    B.append(y)
-            assert set(map(tuple, g["fine"])) == {(x, y) for x in A for y in B}, "self group is not a cross product"
-            for s in A + B:
-                local[sph[s]["frame"]]
-            acs = [([p.code() for p in centre(em, local, sph[x], f"fa{gi}_{ai}")], flit(sph[x]["radius"]))
-                   for ai, x in enumerate(A)]
-            for bi, y in enumerate(B):
+            fines = g["fine"]
+            for f0 in range(0, len(fines), FINE_CHUNK):
+                sub = fines[f0:f0 + FINE_CHUNK]
+                nf = len(sub)
                 em.line("if (need) {")
                 em.indent += 1
-                yc = [p.code() for p in centre(em, local, sph[y], f"fb{gi}_{bi}")]
-                ry = flit(sph[y]["radius"])
-                terms = [f"bcc_self_hit({xc[0]}, {xc[1]}, {xc[2]}, {rx}, {yc[0]}, {yc[1]}, {yc[2]}, {ry})"
-                         for xc, rx in acs]
-                em.line("const bool h = " + "\n                | ".join(terms) + ";")
+                fx, fy, fz, frp = [], [], [], []
+                for fi, f in enumerate(sub):
+                    c = c3(f, f"f{gi}_{f0 + fi}", reuse=False)
+                    fx.append(c[0]); fy.append(c[1]); fz.append(c[2])
+                    frp.append(flit(padded_radius(sph[f]["radius"])))
+                em.line(f"const float fx[{nf}] = {{{', '.join(fx)}}};")
+                em.line(f"const float fy[{nf}] = {{{', '.join(fy)}}};")
+                em.line(f"const float fz[{nf}] = {{{', '.join(fz)}}};")
+                em.line(f"const float frp[{nf}] = {{{', '.join(frp)}}};")
+                em.line(f"const bool h = bcc_env_any<{nf}>(env, fx, fy, fz, frp, need);")
                 em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
                 em.line("hit |= d; live &= ~d; need &= ~d;")
                 em.indent -= 1
                 em.line("}")
             em.indent -= 1
             em.line("}")
+        em.indent -= 1
+        em.line("}")
+        em.line("if (!live) return hit;")
 
-    emit_env(GATE_CHUNK)
-    emit_self()
+    def emit_self_group(gi):
+        g = model["self_program"][gi]
+        a, b = g["gate"]
+        need_frames([a, b])
+        ca = c3(a, f"sg{a}")
+        cb = c3(b, f"sg{b}")
+        ra = flit(sph[a]["radius"])
+        rb = flit(sph[b]["radius"])
+        em.line(f"// self group {gi}" + (f" ({g['label']})" if g["label"] else ""))
+        gate = (f"bcc_self_hit({ca[0]}, {ca[1]}, {ca[2]}, {ra}, "
+                f"{cb[0]}, {cb[1]}, {cb[2]}, {rb})")
+        if g["lone"]:
+            em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot({gate}) & live); hit |= d; live &= ~d; }}")
+            return
+        em.line(f"if (uint64_t need = __ballot({gate}) & live) {{")
+        em.indent += 1
+        # frames for the fine spheres are recomputed in this (rarely taken)
+        # block instead of being kept live through the whole section
+        local = LazyFK(em, model, prefix=f"L{gi}_")
+        A = []
+        B = []
+        for x, y in g["fine"]:
+            if x not in A:
+                A.append(x)
+            if y not in B:
+                B.append(y)
+        assert set(map(tuple, g["fine"])) == {(x, y) for x in A for y in B}, "self group is not a cross product"
+        for s_ in A + B:
+            local[sph[s_]["frame"]]
+        acs = [([p.code() for p in centre(em, local, sph[x], f"fa{gi}_{ai}")], flit(sph[x]["radius"]))
+               for ai, x in enumerate(A)]
+        for bi, y in enumerate(B):
+            em.line("if (need) {")
+            em.indent += 1
+            yc = [p.code() for p in centre(em, local, sph[y], f"fb{gi}_{bi}")]
+            ry = flit(sph[y]["radius"])
+            terms = [f"bcc_self_hit({xc[0]}, {xc[1]}, {xc[2]}, {rx}, {yc[0]}, {yc[1]}, {yc[2]}, {ry})"
+                     for xc, rx in acs]
+            em.line("const bool h = " + "\n                | ".join(terms) + ";")
+            em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
+            em.line("hit |= d; live &= ~d; need &= ~d;")
+            em.indent -= 1
+            em.line("}")
+        em.indent -= 1
+        em.line("}")
+
+    for kind, item in schedule(model, order):
+        if kind == "env":
+            emit_env_chunk(item)
+        else:
+            emit_self_group(item)
     em.line("return hit;")
 
     # all sphere centres (test hook): same FK and centre emission as check()
@@ -382,7 +436,9 @@ def gen_check(model: dict) -> str:
         "    // Collision predicate for one sample per lane.  `live` holds the lanes whose",
         "    // half-wave pair is still undecided; returns the half-spread mask of lanes",
         "    // whose half found a colliding sample (each collision decides its half).",
-        f"    static __device__ __forceinline__ uint64_t check(const float (&q)[{dim}], const EnvView& env, uint64_t live)",
+        "    // q[j] is joint j of this lane's sample (an array or an LDS row view).",
+        "    template <class Q>",
+        "    static __device__ __forceinline__ uint64_t check(const Q& q, const EnvView& env, uint64_t live)",
         "    {",
     ]
     # the check body uses `E` as the env view name in helpers; keep the name `env`
@@ -492,7 +548,22 @@ def gen_model_header(model: dict) -> str:
 
 
 def main(argv):
-    if len(argv) != 3:
+    global GATE_CHUNK, FINE_CHUNK
+    order = ORDER
+    args = list(argv[1:])
+    pos = []
+    while args:
+        a = args.pop(0)
+        if a == "--order":
+            order = args.pop(0)
+        elif a == "--gate-chunk":
+            GATE_CHUNK = int(args.pop(0))
+        elif a == "--fine-chunk":
+            FINE_CHUNK = int(args.pop(0))
+        else:
+            pos.append(a)
+    argv = [argv[0]] + pos
+    if len(argv) != 3 or order not in ("tree", "program") or GATE_CHUNK < 1 or FINE_CHUNK < 1:
         print(__doc__)
         return 2
     model = json.load(open(argv[1]))
@@ -502,7 +573,7 @@ def main(argv):
     with open(os.path.join(out, f"{r}_model.h"), "w") as f:
         f.write(gen_model_header(model))
     with open(os.path.join(out, f"{r}_check.inc"), "w") as f:
-        f.write(gen_check(model))
+        f.write(gen_check(model, order))
     return 0
 
 

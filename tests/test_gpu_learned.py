@@ -22,7 +22,8 @@ def _f(bits):
 @pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
 def test_mfma_gemm_matches_numpy(bcc, gpu, ta, tb):
     rng = np.random.default_rng(0)
-    for (m, n, k) in [(16, 16, 16), (70, 45, 37), (128, 256, 96), (1, 33, 5)]:
+    # the last two shapes take the split-K path (K >= 1024, few output tiles)
+    for (m, n, k) in [(16, 16, 16), (70, 45, 37), (128, 256, 96), (1, 33, 5), (96, 256, 4100), (256, 32, 1024)]:
         a = _bf(rng, (k, m) if ta else (m, k))
         b = _bf(rng, (n, k) if tb else (k, n))
         c0 = rng.standard_normal((m, n)).astype(np.float32)
