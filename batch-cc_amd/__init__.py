@@ -111,6 +111,7 @@ def lib():
     _sig(L, "bcc_debug_samples", I, [I, P, I, I, P, P, P])
     _sig(L, "bcc_debug_prims", I, [P, I, P])
     _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
+    _sig(L, "bcc_debug_force_variant", I, [I])
     _lib = L
     return L
 
@@ -407,6 +408,11 @@ class RaggedGroup:
 
 
 # ------------------------------------------------------------ test hooks
+
+def debug_force_variant(variant: int):
+    """-1: automatic kernel variant per launch; 0 / 1: force low / high occupancy."""
+    _check(lib().bcc_debug_force_variant(int(variant)))
+
 
 def debug_fk(robot, q: np.ndarray) -> np.ndarray:
     q = np.ascontiguousarray(q, np.float32)

@@ -29,7 +29,11 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["bcc_api.hip", "bcc_mlp.hip"] + [f"robot_{r}.hip" for r in ROBOTS]
+# (source, extra macros, object name): each robot's kernels compile as six
+# parts in parallel (see csrc/bcc_robot_parts.h)
+ROBOT_PARTS = 6
+SOURCES = [("bcc_api.hip", (), "bcc_api"), ("bcc_mlp.hip", (), "bcc_mlp")] + [
+    (f"robot_{r}.hip", (f"BCC_PART={k}",), f"robot_{r}_p{k}") for r in ROBOTS for k in range(ROBOT_PARTS)]
 
 
 def run(cmd, **kw):
@@ -48,9 +52,11 @@ def generate(order: str = "tree", gate_chunk: int = 4, fine_chunk: int = 8):
              "--order", order, "--gate-chunk", str(gate_chunk), "--fine-chunk", str(fine_chunk)])
 
 
-def compile_one(src, objdir=OBJDIR, defines=()):
-    obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
-    run([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-c", os.path.join(CSRC, src), "-o", obj], cwd=CSRC)
+def compile_one(item, objdir=OBJDIR, defines=()):
+    src, macros, name = item
+    obj = os.path.join(objdir, name + ".o")
+    run([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in (*macros, *defines)], "-c", os.path.join(CSRC, src), "-o", obj],
+        cwd=CSRC)
     return obj
 
 
@@ -58,7 +64,7 @@ def build_oracle():
     run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
 
 
-def build(gen: bool = True, jobs: int = 4, defines=(), out: str = LIB, order: str = "tree", gate_chunk: int = 4,
+def build(gen: bool = True, jobs: int = 8, defines=(), out: str = LIB, order: str = "tree", gate_chunk: int = 4,
           fine_chunk: int = 8) -> str:
     """Compile and link; `defines` adds -D macros (tuning variants), `out`
     writes the library elsewhere (the in-tree libbcc.so is the product),
@@ -68,7 +74,9 @@ def build(gen: bool = True, jobs: int = 4, defines=(), out: str = LIB, order: st
     objdir = OBJDIR if out == LIB else out + ".objs"
     os.makedirs(objdir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda src: compile_one(src, objdir, defines), SOURCES))
+        # largest translation units first
+        order_ = sorted(SOURCES, key=lambda it: ("baxter" not in it[0], "fetch" not in it[0]))
+        objs = list(ex.map(lambda it: compile_one(it, objdir, defines), order_))
     tmp = out + ".tmp"
     run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)
@@ -80,7 +88,7 @@ def build(gen: bool = True, jobs: int = 4, defines=(), out: str = LIB, order: st
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--no-gen", action="store_true")
-    ap.add_argument("-j", type=int, default=4)
+    ap.add_argument("-j", type=int, default=8)
     ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor macro")
     ap.add_argument("--out", default=LIB, help="library path (default: the in-tree libbcc.so)")
     ap.add_argument("--order", default="tree", choices=["tree", "program"], help="generated check order")

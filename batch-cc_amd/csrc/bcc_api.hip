@@ -15,6 +15,7 @@
 #include <vector>
 
 #include <algorithm>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -198,8 +199,32 @@ int check_env_desc(const bcc_env_desc& d, int e)
     return BCC_OK;
 }
 
-int dispatch_dense(int robot, const bcc::DenseArgs& a, hipStream_t s)
+// Forced kernel variant for tests (-1: chosen from the scene).
+std::atomic<int> g_force_variant{-1};
+
+// High-occupancy variant when the mean number of tested obstacles per
+// environment (weighted by pairs) reaches the robot's threshold.
+int choose_variant(int robot, double mean_obstacles)
 {
+    const int forced = g_force_variant.load();
+    if (forced >= 0)
+        return forced;
+    return mean_obstacles >= (double)kHiVariantMinObstacles[robot] ? 1 : 0;
+}
+
+double mean_obstacles(const bcc_scene* scene)
+{
+    if (scene->num_envs == 0)
+        return 0.0;
+    double sum = 0.0;
+    for (int o : scene->obstacles)
+        sum += o;
+    return sum / scene->num_envs;
+}
+
+int dispatch_dense(int robot, const bcc_scene* scene, bcc::DenseArgs a, hipStream_t s)
+{
+    a.variant = choose_variant(robot, mean_obstacles(scene));
     hipError_t e;
     switch (robot) {
     case BCC_PANDA: e = bcc::launch_dense_panda(a, s); break;
@@ -385,7 +410,7 @@ int bcc_check_dense_async(int robot, const bcc_scene* scene, const float* d_edge
     if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
         return rc;
     bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
-    return dispatch_dense(robot, a, (hipStream_t)stream);
+    return dispatch_dense(robot, scene, a, (hipStream_t)stream);
 }
 
 int bcc_check_dense(int robot, const bcc_scene* scene, const float* edges, int num_edges, int resolution,
@@ -413,7 +438,7 @@ int bcc_check_dense(int robot, const bcc_scene* scene, const float* edges, int n
         rc = hip_fail(e, "dense check setup");
     } else {
         bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_out};
-        rc = dispatch_dense(robot, a, nullptr);
+        rc = dispatch_dense(robot, scene, a, nullptr);
         if (rc == BCC_OK) {
             e = hipMemcpy(results, d_out, pairs, hipMemcpyDeviceToHost);
             if (e != hipSuccess)
@@ -433,7 +458,7 @@ int bcc_check_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, 
     bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
     a.order = d_order;
     a.rounds = d_rounds;
-    return dispatch_dense(robot, a, (hipStream_t)stream);
+    return dispatch_dense(robot, scene, a, (hipStream_t)stream);
 }
 
 int bcc_time_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, int num_edges, int resolution,
@@ -459,7 +484,7 @@ int bcc_time_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, i
     bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
     a.order = d_order;
     for (int i = 0; i < warmup; ++i)
-        if (int rc = dispatch_dense(robot, a, s))
+        if (int rc = dispatch_dense(robot, scene, a, s))
             return rc;
     hipEvent_t t0, t1;
     BCC_HIP(hipEventCreate(&t0), "hipEventCreate");
@@ -467,7 +492,7 @@ int bcc_time_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, i
     int rc = BCC_OK;
     hipError_t e = hipEventRecord(t0, s);
     for (int i = 0; i < iters && e == hipSuccess && rc == BCC_OK; ++i)
-        rc = dispatch_dense(robot, a, s);
+        rc = dispatch_dense(robot, scene, a, s);
     if (e == hipSuccess)
         e = hipEventRecord(t1, s);
     if (e == hipSuccess)
@@ -765,6 +790,10 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_
     }
     p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
                               resolution, device_cus() * groups_per_cu, d_results};
+    double weighted = 0.0;   // mean obstacles per environment, weighted by pairs
+    for (int e = 0; e < E; ++e)
+        weighted += (double)scene->obstacles[e] * (double)(env_offsets[e + 1] - env_offsets[e]);
+    p->args.variant = choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
     // capture reset + persistent kernel once; launches replay the graph
     hipStream_t cap;
     err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
@@ -977,6 +1006,14 @@ int bcc_check_ragged(int robot, const bcc_scene* scene, const int64_t* env_offse
     (void)hipFree(d_edges);
     (void)hipFree(d_out);
     return rc;
+}
+
+int bcc_debug_force_variant(int variant)
+{
+    if (variant < -1 || variant > 1)
+        return fail(BCC_ERR_INVALID, "variant must be -1 (automatic), 0 or 1");
+    g_force_variant.store(variant);
+    return BCC_OK;
 }
 
 int bcc_malloc(void** d_ptr, size_t bytes)

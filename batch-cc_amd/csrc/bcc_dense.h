@@ -50,11 +50,13 @@ __device__ __forceinline__ int setup_pair(const float* __restrict__ se, int reso
     return n;
 }
 
-// Minimum waves per SIMD the register allocator must allow for the dense
-// kernel (occupancy target); set per build, see build.py.
-#ifndef BCC_DENSE_WAVES
-#define BCC_DENSE_WAVES 1
-#endif
+// Kernel variants: kWaves is the occupancy target (waves per SIMD the
+// register allocator must allow), kUnroll the obstacle-loop unroll factor.
+// Each robot builds a high-occupancy variant for scenes with many obstacles
+// per environment (the obstacle sweeps dominate and occupancy hides the
+// scalar-load latency) and a low-occupancy one for few obstacles (forward
+// kinematics and the self program dominate and want registers); the host
+// picks one per launch from the scene's mean obstacle count.
 
 // Sample of round r, lane t under a learned order.  Round 0 is plain step 0
 // (the first sample of every bin, i.e. one sample every 1/31 of the edge);
@@ -160,8 +162,8 @@ __device__ __forceinline__ int setup_pair_lds(const float* __restrict__ se, int 
 // kOrdered: a learned top-4 order per pair, order[edge][env], decides which
 // samples each round visits (ordered_sample); otherwise plain order.
 // rounds (optional): number of rounds (steps) each pair ran, [edge][env].
-template <class Robot, bool kOrdered>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENSE_WAVES, 8))) void dense_kernel(
+template <class Robot, bool kOrdered, int kWaves, int kUnroll>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void dense_kernel(
     const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges, int num_edges,
     int num_envs, int resolution, int chunk, int chunks_per_env, const uint32_t* __restrict__ order,
     uint8_t* __restrict__ rounds, uint8_t* __restrict__ out)
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
         const uint64_t live = __ballot(cur >= 0);
         if (live == 0)
             break;
-        const uint64_t hit = Robot::check(Q, E, live);
+        const uint64_t hit = Robot::template check<kUnroll>(Q, E, live);
         if (cur >= 0) {
             const bool h = (hit >> lane) & 1ull;
             bool fin = h;
@@ -277,15 +279,15 @@ struct RaggedTile {
     int pad;
 };
 
-template <class Robot>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENSE_WAVES, 8))) void ragged_kernel(
+template <class Robot, int kWaves, int kUnroll>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void ragged_kernel(
     const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges,
     const RaggedTile* __restrict__ tiles, int num_tiles, int* __restrict__ tile_counter, int resolution,
     uint8_t* __restrict__ out)
 {
     constexpr int D = Robot::dim;
-    constexpr int kWaves = kBlock / 64;
-    __shared__ int next_pair[kWaves];
+    constexpr int kBlockWaves = kBlock / 64;
+    __shared__ int next_pair[kBlockWaves];
     __shared__ float qs[D][kBlock];
     __shared__ float ds[D][kBlock];
     const int lane = threadIdx.x & 63;
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(BCC_DENS
             const uint64_t live = __ballot(cur >= 0);
             if (live == 0)
                 break;
-            const uint64_t hit = Robot::check(Q, E, live);
+            const uint64_t hit = Robot::template check<kUnroll>(Q, E, live);
             if (cur >= 0) {
                 const bool h = (hit >> lane) & 1ull;
                 bool fin = h;
@@ -372,17 +374,34 @@ struct RaggedArgs {
     int resolution;
     int grid;        // persistent workgroups
     uint8_t* out;
+    int variant = 0; // 1: high-occupancy kernel variant, 0: low
 };
 
-template <class Robot>
+// Kernel variant knobs of a robot: (waves per SIMD, obstacle-loop unroll) of
+// its high- and low-occupancy variants.
+struct VariantKnobs {
+    int hi_waves, hi_unroll, lo_waves, lo_unroll;
+};
+
+// One persistent launch of one kernel variant (instantiated in its own
+// translation unit, see bcc_robot_parts.h).
+template <class Robot, int W, int U>
+hipError_t launch_ragged_variant(const RaggedArgs& a, hipStream_t stream)
+{
+    hipLaunchKernelGGL((ragged_kernel<Robot, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
+                       a.tiles, a.num_tiles, a.tile_counter, a.resolution, a.out);
+    return hipGetLastError();
+}
+
+template <class Robot, int HW, int HU, int LW, int LU>
 hipError_t launch_ragged(const RaggedArgs& a, hipStream_t stream)
 {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.tile_counter);
     if (a.num_tiles == 0)
         return hipGetLastError();
-    hipLaunchKernelGGL(ragged_kernel<Robot>, dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges, a.tiles,
-                       a.num_tiles, a.tile_counter, a.resolution, a.out);
-    return hipGetLastError();
+    if (a.variant)
+        return launch_ragged_variant<Robot, HW, HU>(a, stream);
+    return launch_ragged_variant<Robot, LW, LU>(a, stream);
 }
 
 // ------------------------------------------------------------ test hooks
@@ -650,9 +669,21 @@ struct DenseArgs {
     uint8_t* out;
     const uint32_t* order = nullptr;   // learned top-4 bins per pair, or null (plain order)
     uint8_t* rounds = nullptr;         // rounds per pair, or null
+    int variant = 0;                   // 1: high-occupancy kernel variant, 0: low
 };
 
-template <class Robot>
+// One launch of one dense kernel variant (instantiated in its own
+// translation unit, see bcc_robot_parts.h).
+template <class Robot, bool kOrdered, int W, int U>
+hipError_t launch_dense_variant(const DenseArgs& a, unsigned blocks, int chunk, int chunks_per_env,
+                                hipStream_t stream)
+{
+    hipLaunchKernelGGL((dense_kernel<Robot, kOrdered, W, U>), dim3(blocks), dim3(kBlock), 0, stream, a.obs, a.meta,
+                       a.edges, a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order, a.rounds, a.out);
+    return hipGetLastError();
+}
+
+template <class Robot, int HW, int HU, int LW, int LU>
 hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
 {
     if (a.num_edges == 0 || a.num_envs == 0)
@@ -662,15 +693,11 @@ hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
     const long long blocks = (long long)chunks_per_env * a.num_envs;
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    if (a.order)
-        hipLaunchKernelGGL((dense_kernel<Robot, true>), dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.obs, a.meta,
-                           a.edges, a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order, a.rounds,
-                           a.out);
-    else
-        hipLaunchKernelGGL((dense_kernel<Robot, false>), dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.obs,
-                           a.meta, a.edges, a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order,
-                           a.rounds, a.out);
-    return hipGetLastError();
+    if (a.order)   // the ordered path keeps its pair state in registers: low-occupancy variant
+        return launch_dense_variant<Robot, true, LW, LU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
+    if (a.variant)
+        return launch_dense_variant<Robot, false, HW, HU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
+    return launch_dense_variant<Robot, false, LW, LU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
 }
 
 template <class Robot>
@@ -702,34 +729,8 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
                                      float* cfg_last, hipStream_t stream);                                     \
     hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream);                                     \
     hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream);                                 \
-    hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream);
-
-#define BCC_DEFINE_ROBOT(name, Robot)                                                                          \
-    hipError_t launch_dense_##name(const DenseArgs& a, hipStream_t stream)                                     \
-    {                                                                                                          \
-        return launch_dense<Robot>(a, stream);                                                                 \
-    }                                                                                                          \
-    hipError_t launch_fk_##name(const float* qs, int n, float* out, hipStream_t stream)                        \
-    {                                                                                                          \
-        return launch_fk<Robot>(qs, n, out, stream);                                                           \
-    }                                                                                                          \
-    hipError_t launch_samples_##name(const float* edges, int m, int resolution, int* n_out, float* cfg0,       \
-                                     float* cfg_last, hipStream_t stream)                                      \
-    {                                                                                                          \
-        return launch_samples<Robot>(edges, m, resolution, n_out, cfg0, cfg_last, stream);                     \
-    }                                                                                                          \
-    hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream)                                       \
-    {                                                                                                          \
-        return launch_work<Robot>(a, stream);                                                                  \
-    }                                                                                                          \
-    hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream)                                   \
-    {                                                                                                          \
-        return launch_ragged<Robot>(a, stream);                                                                \
-    }                                                                                                          \
-    hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream)                                     \
-    {                                                                                                          \
-        return launch_label<Robot>(a, stream);                                                                 \
-    }
+    hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream);                                   \
+    extern const VariantKnobs knobs_##name;
 
 BCC_DECLARE_ROBOT(panda)
 BCC_DECLARE_ROBOT(fetch)

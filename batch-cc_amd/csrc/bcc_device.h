@@ -275,24 +275,18 @@ __device__ __forceinline__ bool bcc_ref_env_hit(const RefEnvView& E, float x, fl
 // (N independent tests per obstacle keep the VALU pipeline full).  `rp` are
 // the padded radii float(double(r) + 0.005) (src/utils.cuh:215).
 
-// Obstacle loop unroll factor (a tuning knob: more unrolling overlaps scalar
-// loads with tests, but keeps more temporaries live).
-#ifndef BCC_SWEEP_UNROLL
-#define BCC_SWEEP_UNROLL 2
-#endif
-#define BCC_PRAGMA_(x) _Pragma(#x)
-#define BCC_PRAGMA(x) BCC_PRAGMA_(x)
-
-#define BCC_SWEEP(count, stride, base, TEST)                           \
-    BCC_PRAGMA(unroll BCC_SWEEP_UNROLL) for (int k = 0; k < (count); ++k) { \
-        const cfptr o = (base) + (stride) * k;                         \
-        _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; }        \
-        SETTLED_CHECK;                                                 \
+// U: obstacle loop unroll factor, a tuning knob of each kernel variant (more
+// unrolling overlaps scalar loads with tests but keeps more temporaries live).
+#define BCC_SWEEP(count, stride, base, TEST)                    \
+    _Pragma("unroll U") for (int k = 0; k < (count); ++k) {     \
+        const cfptr o = (base) + (stride) * k;                  \
+        _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; } \
+        SETTLED_CHECK;                                          \
     }
 
 // Per-lane, per-sphere hit flags; no early exit (gate spheres and lone
 // spheres: each lane needs its own answer for every sphere).
-template <int N>
+template <int N, int U>
 __device__ __forceinline__ void bcc_env_sweep(const EnvView& E, const float (&x)[N], const float (&y)[N],
                                               const float (&z)[N], const float (&rp)[N], bool (&h)[N])
 {
@@ -315,7 +309,7 @@ __device__ __forceinline__ void bcc_env_sweep(const EnvView& E, const float (&x)
 // Does any of the N spheres touch any obstacle?  Per-lane OR.  Only lanes in
 // `need` matter; the sweep stops once every half-wave present in `need` has a
 // hit (one colliding sample decides that half's pair).
-template <int N>
+template <int N, int U>
 __device__ __forceinline__ bool bcc_env_any(const EnvView& E, const float (&x)[N], const float (&y)[N],
                                             const float (&z)[N], const float (&rp)[N], uint64_t need)
 {

@@ -12,18 +12,23 @@ struct bcc_tune_entry {
     int groups_per_cu;
 };
 
+// Kernel variant per launch: the high-occupancy variant when the mean number
+// of tested obstacles per environment (weighted by pairs) is at least this,
+// per robot (panda, fetch, baxter).
+static const int kHiVariantMinObstacles[3] = {32, 32, 64};
+
 // source: tune.py on gfx950, 2026-12-03, seed 0
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 8LL, 4, 6},   // panda: 2.648 ms
-    {0, 64LL, 4, 4},   // panda: 7.817 ms
-    {0, 512LL, 4, 6},   // panda: 7.448 ms
-    {0, 0x7fffffffffffffffLL, 4, 4},   // panda: 6.839 ms
-    {1, 8LL, 4, 6},   // fetch: 1.967 ms
-    {1, 64LL, 8, 8},   // fetch: 8.943 ms
-    {1, 512LL, 4, 6},   // fetch: 8.667 ms
-    {1, 0x7fffffffffffffffLL, 4, 6},   // fetch: 8.294 ms
-    {2, 8LL, 16, 6},   // baxter: 5.154 ms
-    {2, 64LL, 4, 8},   // baxter: 13.452 ms
-    {2, 512LL, 4, 6},   // baxter: 12.489 ms
-    {2, 0x7fffffffffffffffLL, 4, 6},   // baxter: 11.833 ms
+    {0, 8LL, 2, 4},   // panda: 2.459 ms
+    {0, 64LL, 4, 8},   // panda: 7.052 ms
+    {0, 512LL, 4, 6},   // panda: 6.983 ms
+    {0, 0x7fffffffffffffffLL, 2, 6},   // panda: 6.513 ms
+    {1, 8LL, 4, 6},   // fetch: 1.822 ms
+    {1, 64LL, 2, 8},   // fetch: 8.045 ms
+    {1, 512LL, 4, 8},   // fetch: 7.915 ms
+    {1, 0x7fffffffffffffffLL, 4, 6},   // fetch: 7.565 ms
+    {2, 8LL, 4, 8},   // baxter: 3.921 ms
+    {2, 64LL, 2, 6},   // baxter: 9.504 ms
+    {2, 512LL, 2, 8},   // baxter: 9.196 ms
+    {2, 0x7fffffffffffffffLL, 2, 6},   // baxter: 9.126 ms
 };

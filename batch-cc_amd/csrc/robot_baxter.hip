@@ -1,13 +1,18 @@
 // Kernels specialised for the baxter model: the generated collision program
-// (robots/baxter_check.inc) instantiated into the dense kernel template.
-// Occupancy target of the dense kernel for this robot (waves per SIMD),
-// chosen by measurement on MI355X; see build.py / tuning.
-#ifndef BCC_DENSE_WAVES
-#define BCC_DENSE_WAVES 4
-#endif
+// (robots/baxter_check.inc) instantiated into the kernel templates, split into
+// parts compiled separately (bcc_robot_parts.h).
 #include "bcc_dense.h"
 
 namespace bcc {
 #include "robots/baxter_check.inc"
-BCC_DEFINE_ROBOT(baxter, Robot_baxter)
 }  // namespace bcc
+
+// Kernel variants (waves per SIMD, obstacle-loop unroll), measured on MI355X:
+// high occupancy for scenes with many obstacles per environment, low for few.
+#ifndef BCC_BAXTER_VARIANTS
+#define BCC_BAXTER_VARIANTS 5, 2, 4, 1
+#endif
+#define BCC_ROBOT baxter
+#define BCC_ROBOT_T Robot_baxter
+#define BCC_KNOBS BCC_BAXTER_VARIANTS
+#include "bcc_robot_parts.h"

@@ -1,13 +1,18 @@
 // Kernels specialised for the fetch model: the generated collision program
-// (robots/fetch_check.inc) instantiated into the dense kernel template.
-// Occupancy target of the dense kernel for this robot (waves per SIMD),
-// chosen by measurement on MI355X; see build.py / tuning.
-#ifndef BCC_DENSE_WAVES
-#define BCC_DENSE_WAVES 6
-#endif
+// (robots/fetch_check.inc) instantiated into the kernel templates, split into
+// parts compiled separately (bcc_robot_parts.h).
 #include "bcc_dense.h"
 
 namespace bcc {
 #include "robots/fetch_check.inc"
-BCC_DEFINE_ROBOT(fetch, Robot_fetch)
 }  // namespace bcc
+
+// Kernel variants (waves per SIMD, obstacle-loop unroll), measured on MI355X:
+// high occupancy for scenes with many obstacles per environment, low for few.
+#ifndef BCC_FETCH_VARIANTS
+#define BCC_FETCH_VARIANTS 6, 1, 4, 1
+#endif
+#define BCC_ROBOT fetch
+#define BCC_ROBOT_T Robot_fetch
+#define BCC_KNOBS BCC_FETCH_VARIANTS
+#include "bcc_robot_parts.h"

@@ -1,13 +1,18 @@
 // Kernels specialised for the panda model: the generated collision program
-// (robots/panda_check.inc) instantiated into the dense kernel template.
-// Occupancy target of the dense kernel for this robot (waves per SIMD),
-// chosen by measurement on MI355X; see build.py / tuning.
-#ifndef BCC_DENSE_WAVES
-#define BCC_DENSE_WAVES 4
-#endif
+// (robots/panda_check.inc) instantiated into the kernel templates, split into
+// parts compiled separately (bcc_robot_parts.h).
 #include "bcc_dense.h"
 
 namespace bcc {
 #include "robots/panda_check.inc"
-BCC_DEFINE_ROBOT(panda, Robot_panda)
 }  // namespace bcc
+
+// Kernel variants (waves per SIMD, obstacle-loop unroll), measured on MI355X:
+// high occupancy for scenes with many obstacles per environment, low for few.
+#ifndef BCC_PANDA_VARIANTS
+#define BCC_PANDA_VARIANTS 6, 1, 4, 1
+#endif
+#define BCC_ROBOT panda
+#define BCC_ROBOT_T Robot_panda
+#define BCC_KNOBS BCC_PANDA_VARIANTS
+#include "bcc_robot_parts.h"

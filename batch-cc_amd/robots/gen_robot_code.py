@@ -326,7 +326,7 @@ def gen_check(model: dict, order: str | None = None) -> str:
         em.line(f"const float gz[{ng}] = {{{', '.join(gz)}}};")
         em.line(f"const float grp[{ng}] = {{{', '.join(grp)}}};")
         em.line(f"bool gh[{ng}];")
-        em.line(f"bcc_env_sweep<{ng}>(env, gx, gy, gz, grp, gh);")
+        em.line(f"bcc_env_sweep<{ng}, U>(env, gx, gy, gz, grp, gh);")
         for k, gi in enumerate(chunk):
             g = envp[gi]
             em.line(f"// env group {gi}" + (f" ({g['label']})" if g["label"] else ""))
@@ -350,7 +350,7 @@ def gen_check(model: dict, order: str | None = None) -> str:
                 em.line(f"const float fy[{nf}] = {{{', '.join(fy)}}};")
                 em.line(f"const float fz[{nf}] = {{{', '.join(fz)}}};")
                 em.line(f"const float frp[{nf}] = {{{', '.join(frp)}}};")
-                em.line(f"const bool h = bcc_env_any<{nf}>(env, fx, fy, fz, frp, need);")
+                em.line(f"const bool h = bcc_env_any<{nf}, U>(env, fx, fy, fz, frp, need);")
                 em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
                 em.line("hit |= d; live &= ~d; need &= ~d;")
                 em.indent -= 1
@@ -436,8 +436,9 @@ def gen_check(model: dict, order: str | None = None) -> str:
         "    // Collision predicate for one sample per lane.  `live` holds the lanes whose",
         "    // half-wave pair is still undecided; returns the half-spread mask of lanes",
         "    // whose half found a colliding sample (each collision decides its half).",
-        "    // q[j] is joint j of this lane's sample (an array or an LDS row view).",
-        "    template <class Q>",
+        "    // q[j] is joint j of this lane's sample (an array or an LDS row view);",
+        "    // U is the obstacle-loop unroll factor of the calling kernel variant.",
+        "    template <int U, class Q>",
         "    static __device__ __forceinline__ uint64_t check(const Q& q, const EnvView& env, uint64_t live)",
         "    {",
     ]

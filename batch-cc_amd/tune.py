@@ -1,5 +1,11 @@
 """Autotune the ragged-batch launch table on the GPU and write csrc/bcc_tuning.h.
 
+Then, per robot, dense batches with 4 .. 128 obstacles per environment (the
+benchmark scenes' mixed obstacles spread over the workspace) time
+the two compiled kernel variants (high and low occupancy) against each other;
+the obstacle count from which the high-occupancy variant wins becomes the
+robot's variant threshold.
+
 For every robot and every batch shape bucket (mean pairs per environment up
 to 8, 64, 512, unbounded) a seeded ragged batch is built (U{32..256}
 obstacles of the mixed kinds away from the robot base, lognormal edges,
@@ -58,7 +64,48 @@ def batch(W, robot: str, mean: int, pairs: int, seed: int):
     return PackedScene.from_envs(envs), off, edges
 
 
-def header(table: dict, source: str) -> str:
+OBSTACLE_COUNTS = (4, 8, 16, 32, 64, 128)
+
+
+def variant_threshold(bcc, W, robot: str, seed: int, quick: bool):
+    """Smallest mean obstacle count from which the high-occupancy kernel
+    variant is at least as fast as the low one on a dense batch of the
+    benchmark scenes' kind (each count: 128 environments of that many mixed
+    obstacles spread over the workspace, 2000 lognormal edges as in c1);
+    returns (threshold, timings)."""
+    rs = np.random.default_rng([seed, 12])
+    rows, thr = [], None
+    for n_obs in OBSTACLE_COUNTS:
+        envs = [W.mixed_env(rs, n_obs) for _ in range(128)]
+        from batch_cc_amd import PackedScene
+        sc = PackedScene.from_envs(envs)
+        edges = W.lognormal_edges(robot, 500 if quick else 2000, rs)
+        ms = {}
+        with bcc.Scene(sc) as scene:
+            de = bcc.DeviceBuffer.from_array(edges)
+            do = bcc.DeviceBuffer(edges.shape[0] * sc.num_envs)
+            ref = None
+            for v in (0, 1):
+                bcc.debug_force_variant(v)
+                ms[v] = bcc.time_dense(robot, scene, de, edges.shape[0], 32, do, 2, 5)
+                got = do.to_array(np.uint8, (edges.shape[0], sc.num_envs))
+                if ref is None:
+                    ref = got
+                elif not np.array_equal(got, ref):
+                    bcc.debug_force_variant(-1)
+                    raise SystemExit(f"tune: kernel variants disagree for {robot} at {n_obs} obstacles")
+        bcc.debug_force_variant(-1)
+        rows.append({"robot": robot, "obstacles": n_obs, "ms_low": ms[0], "ms_high": ms[1]})
+        print(f"[tune] {robot} {n_obs} obstacles/env: low {ms[0]:.3f} ms, high {ms[1]:.3f} ms", flush=True)
+    # the first count from which high wins at every larger count
+    for i, n_obs in enumerate(OBSTACLE_COUNTS):
+        if all(r["ms_high"] <= r["ms_low"] for r in rows[i:]):
+            thr = n_obs
+            break
+    return (thr if thr is not None else 1 << 30), rows
+
+
+def header(table: dict, source: str, thresholds: dict) -> str:
     rid = {r: i for i, r in enumerate(ROBOTS)}
     lines = [
         "// Ragged-batch launch table: per robot, by mean pairs per environment, the",
@@ -74,6 +121,12 @@ def header(table: dict, source: str) -> str:
         "    int tile_pairs;",
         "    int groups_per_cu;",
         "};",
+        "",
+        "// Kernel variant per launch: the high-occupancy variant when the mean number",
+        "// of tested obstacles per environment (weighted by pairs) is at least this,",
+        "// per robot (panda, fetch, baxter).",
+        "static const int kHiVariantMinObstacles[3] = {"
+        + ", ".join(str(thresholds[r]) for r in ROBOTS) + "};",
         "",
         f"// source: {source}",
         "static const bcc_tune_entry kTuneTable[] = {",
@@ -129,10 +182,16 @@ def main():
             print(f"[tune] {robot} mean {mean}: {n} pairs, {sc.num_envs} envs, best tile {best[0]} "
                   f"groups {best[1]} {best[2]:.3f} ms ({n / best[2] / 1e3:.3g} Mpairs/s, "
                   f"{time.perf_counter() - t0:.1f} s)", flush=True)
+    thresholds = {}
+    for robot in ROBOTS:
+        thresholds[robot], rows = variant_threshold(bcc, W, robot, a.seed, a.quick)
+        timings += rows
+        print(f"[tune] {robot}: high-occupancy variant from {thresholds[robot]} obstacles per environment",
+              flush=True)
     props = bcc.device_name() if hasattr(bcc, "device_name") else "gfx950"
     src = f"tune.py on {props}, {datetime.date.today().isoformat()}, seed {a.seed}" + (" (quick)" if a.quick else "")
     with open(a.out, "w") as f:
-        f.write(header(table, src))
+        f.write(header(table, src, thresholds))
     if a.json:
         with open(a.json, "w") as f:
             json.dump(timings, f)

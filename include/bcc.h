@@ -245,6 +245,11 @@ int bcc_model_logits_dense(bcc_model *model, const bcc_scene *scene, const float
 int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,
                    int m, int n, int k, float alpha, float beta);
 
+/* Test hook (no reference counterpart): force the dense/ragged kernel
+ * variant, 0 = low occupancy, 1 = high occupancy, -1 = chosen per launch from
+ * the scene's mean obstacle count (the default).  Verdicts never depend on it. */
+int bcc_debug_force_variant(int variant);
+
 /* Device memory helpers for bindings that do not bring their own allocator. */
 int bcc_malloc(void **d_ptr, size_t bytes);
 int bcc_free(void *d_ptr);

@@ -232,3 +232,29 @@ def test_full_size_fetch_batch_is_deterministic_and_matches_sampled_pairs(bcc, o
     pg = rng.integers(0, edges.shape[0], 4000).astype(np.int32)
     exp, _, _, _ = oracle.check_pairs(1, sc, edges, pe, pg)
     assert np.array_equal(a[pg, pe], exp)
+
+
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+def test_both_kernel_variants_match_oracle(bcc, oracle, gpu, robot):
+    """The high- and low-occupancy kernel variants (dense and ragged), forced
+    in turn, give the oracle's verdicts; the automatic choice gives the same."""
+    from batch_cc_amd import workloads as W
+    rng = np.random.default_rng(31 + ROBOTS[robot])
+    envs = [W.mixed_env(rng, int(n)) for n in rng.integers(1, 48, 12)]
+    sc = bcc.PackedScene.from_envs(envs)
+    edges = W.lognormal_edges(robot, 120, rng)
+    exp = oracle.check_dense(ROBOTS[robot], sc, edges, threads=16)
+    counts = rng.integers(0, 20, sc.num_envs)
+    off = np.zeros(sc.num_envs + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    pick = rng.integers(0, edges.shape[0], int(off[-1]))
+    env_of = np.repeat(np.arange(sc.num_envs), counts)
+    try:
+        with bcc.Scene(sc) as scene:
+            for v in (0, 1, -1):
+                bcc.debug_force_variant(v)
+                print(f"{robot}: variant {v}", flush=True)
+                assert np.array_equal(bcc.check_dense(robot, scene, edges), exp), v
+                assert np.array_equal(bcc.check_ragged(robot, scene, off, edges[pick]), exp[pick, env_of]), v
+    finally:
+        bcc.debug_force_variant(-1)

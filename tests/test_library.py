@@ -143,3 +143,10 @@ def test_mixed_workload_layout(bcc):
         assert np.all((n >= 32) & (n <= 512))
         c = np.concatenate([sc.env(e)["cuboids"][:, :3] for e in range(sc.num_envs)])
         assert np.all(c[:, 0] ** 2 + c[:, 1] ** 2 >= np.float32(W.MIXED_CLEAR_RADIUS ** 2))
+
+
+def test_force_variant_rejects_out_of_range(bcc):
+    L = bcc.lib()
+    assert L.bcc_debug_force_variant(2) == 1
+    assert b"variant" in L.bcc_last_error()
+    assert L.bcc_debug_force_variant(-1) == 0
