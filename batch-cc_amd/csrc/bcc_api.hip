@@ -212,6 +212,15 @@ int choose_variant(int robot, double mean_obstacles)
     return mean_obstacles >= (double)kHiVariantMinObstacles[robot] ? 1 : 0;
 }
 
+const bcc::VariantKnobs& knobs(int robot)
+{
+    switch (robot) {
+    case BCC_FETCH: return bcc::knobs_fetch;
+    case BCC_BAXTER: return bcc::knobs_baxter;
+    default: return bcc::knobs_panda;
+    }
+}
+
 double mean_obstacles(const bcc_scene* scene)
 {
     if (scene->num_envs == 0)
@@ -685,20 +694,28 @@ int dispatch_ragged(int robot, const bcc::RaggedArgs& a, hipStream_t s)
     return BCC_OK;
 }
 
-// Tile size and persistent grid from the tuning table (bcc_tuning.h, written
-// by batch-cc_amd/tune.py from timings on the GPU): the first entry of the
-// robot whose bound is at least the batch's mean pairs per environment.
-const bcc_tune_entry& tuned(int robot, double mean_pairs_per_env)
+// Launch shape from the tuning table (bcc_tuning.h, written by
+// batch-cc_amd/tune.py from timings on the GPU): per robot, workgroups per CU
+// of the persistent grid and a target number of tiles per wave; the tile
+// size follows from the batch, pairs per wave / tiles per wave, rounded down
+// to a power of two in [1, 256], so short batches get small tiles (load
+// balance at the end of the launch) and long ones large tiles (fewer
+// tile fetches).
+const bcc_tune_entry& tuned(int robot)
 {
-    const bcc_tune_entry* last = nullptr;
-    for (const bcc_tune_entry& t : kTuneTable) {
-        if (t.robot != robot)
-            continue;
-        last = &t;
-        if (mean_pairs_per_env <= (double)t.max_mean_pairs)
+    for (const bcc_tune_entry& t : kTuneTable)
+        if (t.robot == robot)
             return t;
-    }
-    return *last;   // every robot has entries (checked by the library tests)
+    return kTuneTable[0];   // every robot has an entry (checked by the library tests)
+}
+
+int tile_for(int64_t total_pairs, int grid_waves, int tiles_per_wave)
+{
+    const double per = (double)total_pairs / ((double)std::max(grid_waves, 1) * std::max(tiles_per_wave, 1));
+    int tile = 1;
+    while (tile < 256 && 2.0 * tile <= per)
+        tile *= 2;
+    return tile;
 }
 
 int device_cus()
@@ -755,13 +772,22 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_
         return fail(BCC_ERR_INVALID, "more than 2^31 pairs in one ragged batch");
     if (total > 0 && (!d_edges || !d_results))
         return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
-    if (tile_pairs < 0 || groups_per_cu < 0 || groups_per_cu > 32)
-        return fail(BCC_ERR_INVALID, "tile_pairs must be >= 0 and groups_per_cu in 0..32");
-    const bcc_tune_entry& tune = tuned(robot, E > 0 ? (double)total / E : 0.0);
-    if (tile_pairs == 0)
-        tile_pairs = tune.tile_pairs;
+    if (tile_pairs < -1024 || groups_per_cu < 0 || groups_per_cu > 32)
+        return fail(BCC_ERR_INVALID, "tile_pairs must be >= -1024 and groups_per_cu in 0..32");
+    double weighted = 0.0;   // mean obstacles per environment, weighted by pairs
+    for (int e = 0; e < E; ++e)
+        weighted += (double)scene->obstacles[e] * (double)(env_offsets[e + 1] - env_offsets[e]);
+    const int variant = choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
+    const bcc_tune_entry& tune = tuned(robot);
     if (groups_per_cu == 0)
         groups_per_cu = tune.groups_per_cu;
+    if (tile_pairs <= 0) {
+        // waves that can be resident at once: the grid, capped by the variant's occupancy
+        const bcc::VariantKnobs& k = knobs(robot);
+        const int resident = std::min(groups_per_cu, variant ? k.hi_waves : k.lo_waves);
+        const int per_wave = tile_pairs < 0 ? -tile_pairs : tune.tiles_per_wave;
+        tile_pairs = tile_for(total, device_cus() * resident * (bcc::kBlock / 64), per_wave);
+    }
     std::vector<bcc::RaggedTile> tiles;
     for (int e = 0; e < E; ++e)
         for (int64_t b = env_offsets[e]; b < env_offsets[e + 1]; b += tile_pairs)
@@ -790,10 +816,7 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_
     }
     p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
                               resolution, device_cus() * groups_per_cu, d_results};
-    double weighted = 0.0;   // mean obstacles per environment, weighted by pairs
-    for (int e = 0; e < E; ++e)
-        weighted += (double)scene->obstacles[e] * (double)(env_offsets[e + 1] - env_offsets[e]);
-    p->args.variant = choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
+    p->args.variant = variant;
     // capture reset + persistent kernel once; launches replay the graph
     hipStream_t cap;
     err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);

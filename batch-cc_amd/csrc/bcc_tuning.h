@@ -1,34 +1,22 @@
-// Ragged-batch launch table: per robot, by mean pairs per environment, the
-// tile size (pairs per tile) and the persistent grid (workgroups per CU).
-// Written by batch-cc_amd/tune.py; an entry applies when the batch's mean
-// pairs per environment is at most max_mean_pairs (the last entry of each
-// robot has no bound).
+// Ragged-batch launch table: per robot, the persistent grid (workgroups per
+// CU) and the target number of tiles per wave, from which a plan sizes its
+// tiles; and the kernel-variant thresholds.  Written by batch-cc_amd/tune.py.
 #pragma once
 
 struct bcc_tune_entry {
     int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER
-    long long max_mean_pairs;
-    int tile_pairs;
-    int groups_per_cu;
+    int tiles_per_wave;   // target tiles per persistent wave
+    int groups_per_cu;    // persistent workgroups per CU
 };
 
 // Kernel variant per launch: the high-occupancy variant when the mean number
 // of tested obstacles per environment (weighted by pairs) is at least this,
 // per robot (panda, fetch, baxter).
-static const int kHiVariantMinObstacles[3] = {32, 32, 64};
+static const int kHiVariantMinObstacles[3] = {32, 32, 32};
 
 // source: tune.py on gfx950, 2026-12-03, seed 0
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 8LL, 2, 4},   // panda: 2.459 ms
-    {0, 64LL, 4, 8},   // panda: 7.052 ms
-    {0, 512LL, 4, 6},   // panda: 6.983 ms
-    {0, 0x7fffffffffffffffLL, 2, 6},   // panda: 6.513 ms
-    {1, 8LL, 4, 6},   // fetch: 1.822 ms
-    {1, 64LL, 2, 8},   // fetch: 8.045 ms
-    {1, 512LL, 4, 8},   // fetch: 7.915 ms
-    {1, 0x7fffffffffffffffLL, 4, 6},   // fetch: 7.565 ms
-    {2, 8LL, 4, 8},   // baxter: 3.921 ms
-    {2, 64LL, 2, 6},   // baxter: 9.504 ms
-    {2, 512LL, 2, 8},   // baxter: 9.196 ms
-    {2, 0x7fffffffffffffffLL, 2, 6},   // baxter: 9.126 ms
+    {0, 8, 8},   // panda: score 1.000
+    {1, 8, 8},   // fetch: score 1.014
+    {2, 4, 6},   // baxter: score 1.021
 };

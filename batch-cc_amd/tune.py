@@ -6,14 +6,12 @@ the two compiled kernel variants (high and low occupancy) against each other;
 the obstacle count from which the high-occupancy variant wins becomes the
 robot's variant threshold.
 
-For every robot and every batch shape bucket (mean pairs per environment up
-to 8, 64, 512, unbounded) a seeded ragged batch is built (U{32..256}
-obstacles of the mixed kinds away from the robot base, lognormal edges,
-U{m/2..3m/2} pairs per environment), and each candidate (tile size,
-workgroups per CU) is timed as a replayed hipGraph.  Verdicts must be
-identical for every candidate.  The fastest candidate of each bucket goes into
-the table; the library reads it when a plan is created with tile_pairs = 0 /
-groups_per_cu = 0.
+For every robot, three seeded ragged batches (long pairs, few environments,
+short pairs: see batch()) are timed as replayed hipGraphs for every
+candidate (target tiles per wave, workgroups per CU); verdicts must be
+identical for every candidate.  The candidate with the lowest mean of
+time / best time over the three batches goes into the table; the library
+reads it when a plan is created with tile_pairs = 0 / groups_per_cu = 0.
 
     python batch-cc_amd/tune.py [--out batch-cc_amd/csrc/bcc_tuning.h] [--json timings.json] [--quick]
 
@@ -32,10 +30,8 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-BUCKETS = (8, 64, 512, None)          # upper bound on mean pairs per environment
-BUCKET_MEAN = (4, 32, 256, 2048)      # the mean used to time each bucket
-TILES = (2, 4, 8, 16, 32, 64, 128, 256)
-GROUPS = (2, 4, 6, 8)
+TILES_PER_WAVE = (4, 8, 16, 32, 64)
+GROUPS = (4, 5, 6, 8)
 ROBOTS = ("panda", "fetch", "baxter")
 PAIRS = {"panda": 400_000, "fetch": 400_000, "baxter": 200_000}
 
@@ -49,18 +45,33 @@ def load_pkg():
     return mod
 
 
-def batch(W, robot: str, mean: int, pairs: int, seed: int):
+def batch(W, robot: str, kind: str, pairs: int, seed: int):
+    """Ragged tuning batches.  "long": U{32..256} small obstacles away from the
+    base, lognormal edges (about 40% colliding, free pairs run every step),
+    U{16..48} pairs per environment; "few-envs": the same with about 2048 pairs
+    per environment; "short": workspace-filling mixed obstacles (U{8..256}),
+    uniform edges, Zipf(1.2) pairs per environment (nearly every pair collides
+    in its first step, as in c2)."""
+    from batch_cc_amd import PackedScene
     rs = np.random.default_rng([seed, 11])
-    E = max(8, min(16384, pairs // mean))
-    counts = rs.integers(max(1, mean // 2), max(2, 3 * mean // 2 + 1), E).astype(np.int64)
-    envs = []
-    for _ in range(E):
-        n = int(rs.integers(32, 257))
-        envs.append(W.scaled_env(rs, n, (32.0 / n) ** (1.0 / 3.0)))
+    if kind == "short":
+        E = max(8, pairs // 200)
+        counts = W.zipf_counts(rs, E)
+        envs = [W.mixed_env(rs, int(rs.integers(8, 257)), capsule_frac=0.2) for _ in range(E)]
+    else:
+        mean = 32 if kind == "long" else 2048
+        E = max(8, min(16384, pairs // mean))
+        counts = rs.integers(max(1, mean // 2), max(2, 3 * mean // 2 + 1), E).astype(np.int64)
+        envs = []
+        for _ in range(E):
+            n = int(rs.integers(32, 257))
+            envs.append(W.scaled_env(rs, n, (32.0 / n) ** (1.0 / 3.0)))
     off = np.zeros(E + 1, np.int64)
     off[1:] = np.cumsum(counts)
-    edges = W.lognormal_edges(robot, int(off[-1]), rs, median=W.MIXED_EDGE_MEDIAN)
-    from batch_cc_amd import PackedScene
+    if kind == "short":
+        edges = W.uniform_edges(robot, int(off[-1]), rs)
+    else:
+        edges = W.lognormal_edges(robot, int(off[-1]), rs, median=W.MIXED_EDGE_MEDIAN)
     return PackedScene.from_envs(envs), off, edges
 
 
@@ -108,18 +119,15 @@ def variant_threshold(bcc, W, robot: str, seed: int, quick: bool):
 def header(table: dict, source: str, thresholds: dict) -> str:
     rid = {r: i for i, r in enumerate(ROBOTS)}
     lines = [
-        "// Ragged-batch launch table: per robot, by mean pairs per environment, the",
-        "// tile size (pairs per tile) and the persistent grid (workgroups per CU).",
-        "// Written by batch-cc_amd/tune.py; an entry applies when the batch's mean",
-        "// pairs per environment is at most max_mean_pairs (the last entry of each",
-        "// robot has no bound).",
+        "// Ragged-batch launch table: per robot, the persistent grid (workgroups per",
+        "// CU) and the target number of tiles per wave, from which a plan sizes its",
+        "// tiles; and the kernel-variant thresholds.  Written by batch-cc_amd/tune.py.",
         "#pragma once",
         "",
         "struct bcc_tune_entry {",
         "    int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER",
-        "    long long max_mean_pairs;",
-        "    int tile_pairs;",
-        "    int groups_per_cu;",
+        "    int tiles_per_wave;   // target tiles per persistent wave",
+        "    int groups_per_cu;    // persistent workgroups per CU",
         "};",
         "",
         "// Kernel variant per launch: the high-occupancy variant when the mean number",
@@ -132,9 +140,8 @@ def header(table: dict, source: str, thresholds: dict) -> str:
         "static const bcc_tune_entry kTuneTable[] = {",
     ]
     for robot in ROBOTS:
-        for bound, (tile, groups, ms) in zip(BUCKETS, table[robot]):
-            b = "0x7fffffffffffffffLL" if bound is None else f"{bound}LL"
-            lines.append(f"    {{{rid[robot]}, {b}, {tile}, {groups}}},   // {robot}: {ms:.3f} ms")
+        per_wave, groups, score = table[robot]
+        lines.append(f"    {{{rid[robot]}, {per_wave}, {groups}}},   // {robot}: score {score:.3f}")
     lines.append("};")
     return "\n".join(lines) + "\n"
 
@@ -148,40 +155,44 @@ def main():
     a = ap.parse_args()
     bcc = load_pkg()
     from batch_cc_amd import workloads as W
-    tiles = (8, 32, 128) if a.quick else TILES
-    groups = (4, 8) if a.quick else GROUPS
+    per_waves = (8, 16, 32) if a.quick else TILES_PER_WAVE
+    groups = (4, 6) if a.quick else GROUPS
     scale = 0.25 if a.quick else 1.0
     table, timings = {}, []
     for robot in ROBOTS:
-        table[robot] = []
-        for mean in BUCKET_MEAN:
+        scores = {}
+        for kind in ("long", "few-envs", "short"):
             t0 = time.perf_counter()
-            sc, off, edges = batch(W, robot, mean, int(PAIRS[robot] * scale), a.seed)
+            sc, off, edges = batch(W, robot, kind, int(PAIRS[robot] * scale), a.seed)
             n = edges.shape[0]
-            best, ref = None, None
+            ms_of, ref = {}, None
             with bcc.Scene(sc) as scene:
                 de = bcc.DeviceBuffer.from_array(edges)
                 do = bcc.DeviceBuffer(n)
-                for tile in tiles:
-                    if tile > 4 * mean:
-                        continue
+                for T in per_waves:
                     for g in groups:
-                        plan = bcc.RaggedPlan(robot, scene, off, de, do, tile_pairs=tile, groups_per_cu=g)
+                        plan = bcc.RaggedPlan(robot, scene, off, de, do, tile_pairs=-T, groups_per_cu=g)
+                        tile = plan.config[0]
                         ms = plan.time(2, 5)
                         v = do.to_array(np.uint8, (n,))
                         plan.close()
                         if ref is None:
                             ref = v
                         elif not np.array_equal(v, ref):
-                            raise SystemExit(f"tune: verdicts differ for {robot} tile {tile} groups {g}")
-                        timings.append({"robot": robot, "mean_pairs": mean, "pairs": n, "tile": tile,
-                                        "groups_per_cu": g, "ms": ms})
-                        if best is None or ms < best[2]:
-                            best = (tile, g, ms)
-            table[robot].append(best)
-            print(f"[tune] {robot} mean {mean}: {n} pairs, {sc.num_envs} envs, best tile {best[0]} "
-                  f"groups {best[1]} {best[2]:.3f} ms ({n / best[2] / 1e3:.3g} Mpairs/s, "
-                  f"{time.perf_counter() - t0:.1f} s)", flush=True)
+                            raise SystemExit(f"tune: verdicts differ for {robot} {kind} T {T} groups {g}")
+                        ms_of[(T, g)] = ms
+                        timings.append({"robot": robot, "batch": kind, "pairs": n, "tiles_per_wave": T,
+                                        "tile": tile, "groups_per_cu": g, "ms": ms})
+            best = min(ms_of.values())
+            for k, v in ms_of.items():
+                scores[k] = scores.get(k, 0.0) + v / best / 3.0
+            kb = min(ms_of, key=ms_of.get)
+            print(f"[tune] {robot} {kind}: {n} pairs, {sc.num_envs} envs, best tiles/wave {kb[0]} groups {kb[1]} "
+                  f"{best:.3f} ms ({time.perf_counter() - t0:.1f} s)", flush=True)
+        k = min(scores, key=scores.get)
+        table[robot] = (k[0], k[1], scores[k])
+        print(f"[tune] {robot}: tiles per wave {k[0]}, groups per CU {k[1]} (mean time / best {scores[k]:.3f})",
+              flush=True)
     thresholds = {}
     for robot in ROBOTS:
         thresholds[robot], rows = variant_threshold(bcc, W, robot, a.seed, a.quick)

@@ -150,7 +150,7 @@ int bcc_check_ragged(int robot, const bcc_scene *scene, const int64_t *env_offse
  * A ragged plan owns the tile list (env, pair range) of one CSR layout and a
  * captured hipGraph (tile-counter reset + one persistent kernel launch) over
  * caller-owned device buffers d_edges / d_results.  tile_pairs = 0 picks the
- * tuned default.  The plan keeps pointers to the scene and buffers: they
+ * tuned size.  The plan keeps pointers to the scene and buffers: they
  * must outlive it.  Launches of one plan must not overlap (they share the
  * tile counter).
  */
@@ -158,8 +158,10 @@ typedef struct bcc_ragged_plan bcc_ragged_plan;
 int bcc_ragged_plan_create(int robot, const bcc_scene *scene, const int64_t *env_offsets, const float *d_edges,
                            int resolution, uint8_t *d_results, int tile_pairs, bcc_ragged_plan **out);
 /* Same, with the persistent grid given as workgroups per CU (one wave per
- * SIMD each); 0 for either knob takes the tuning table's entry for this
- * robot and the batch's mean pairs per environment. */
+ * SIMD each).  tile_pairs > 0 is the tile size; 0 sizes tiles from the tuning
+ * table's target tiles per wave for this robot; -T targets T tiles per wave
+ * (tile = pairs per resident wave / T, a power of two in [1, 256]).
+ * groups_per_cu = 0 takes the table's grid. */
 int bcc_ragged_plan_create_tuned(int robot, const bcc_scene *scene, const int64_t *env_offsets,
                                  const float *d_edges, int resolution, uint8_t *d_results, int tile_pairs,
                                  int groups_per_cu, bcc_ragged_plan **out);

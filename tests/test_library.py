@@ -117,15 +117,12 @@ def test_ragged_plan_and_group_reject_bad_arguments(bcc):
 
 def test_tuning_table_covers_every_robot():
     text = open(os.path.join(REPO, "batch-cc_amd", "csrc", "bcc_tuning.h")).read()
-    rows = re.findall(r"\{\s*(\d+),\s*(0x[0-9a-fA-F]+LL|\d+LL|\d+),\s*(\d+),\s*(\d+)\s*\}", text)
-    by_robot = {}
-    for r, bound, tile, groups in rows:
-        by_robot.setdefault(int(r), []).append((int(bound.rstrip("L"), 0), int(tile), int(groups)))
-    assert sorted(by_robot) == [0, 1, 2]
-    for r, ent in by_robot.items():
-        bounds = [b for b, _, _ in ent]
-        assert bounds == sorted(bounds) and bounds[-1] == 0x7fffffffffffffff, r
-        assert all(t >= 1 and 1 <= g <= 32 for _, t, g in ent), r
+    body = text[text.index("kTuneTable[]"):]
+    rows = re.findall(r"\{\s*(\d+),\s*(\d+),\s*(\d+)\s*\}", body)
+    assert sorted(int(r) for r, _, _ in rows) == [0, 1, 2]
+    assert all(1 <= int(t) <= 1024 and 1 <= int(g) <= 32 for _, t, g in rows)
+    thr = re.search(r"kHiVariantMinObstacles\[3\] = \{(\d+), (\d+), (\d+)\}", text)
+    assert thr and all(int(x) >= 0 for x in thr.groups())
 
 
 def test_mixed_workload_layout(bcc):
