@@ -49,13 +49,13 @@ def batch(W, robot: str, kind: str, pairs: int, seed: int):
     """Ragged tuning batches.  "long": U{32..256} small obstacles away from the
     base, lognormal edges (about 40% colliding, free pairs run every step),
     U{16..48} pairs per environment; "few-envs": the same with about 2048 pairs
-    per environment; "short": workspace-filling mixed obstacles (U{8..256}),
-    uniform edges, Zipf(1.2) pairs per environment (nearly every pair collides
-    in its first step, as in c2)."""
+    per environment; "short": c2's shape (4096 environments of workspace-filling
+    mixed obstacles, U{8..256}, uniform edges, Zipf(1.2) pairs per environment:
+    nearly every pair collides in its first step)."""
     from batch_cc_amd import PackedScene
     rs = np.random.default_rng([seed, 11])
     if kind == "short":
-        E = max(8, pairs // 200)
+        E = 4096 if pairs >= 100_000 else max(8, pairs // 200)   # c2's shape at full size
         counts = W.zipf_counts(rs, E)
         envs = [W.mixed_env(rs, int(rs.integers(8, 257)), capsule_frac=0.2) for _ in range(E)]
     else:
