@@ -28,6 +28,12 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet)
 HBM_PEAK_GBS = 8000.0
 METRIC = "edge-env checks/sec at bit-exact verdicts, learned-ordered vs plain, 1 GPU"
+# HBM bytes per launch of the dense kernel, measured with rocprofv3 FETCH_SIZE
+# (x2.000) + WRITE_SIZE (x1.000), the scales read off a plain copy kernel;
+# commands and raw counts in profiles/dense_kernel_fetch_c1.md.  Only the
+# default workload is measured; re-measure after kernel changes.
+MEASURED_TRAFFIC = {"c1": (2 * 6.94295e6 * 1024 + 1.67999e7 * 1024,
+                           "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/dense_kernel_fetch_c1.md")}
 
 
 def load_pkg():
@@ -382,7 +388,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC[a.config][0] if a.config in MEASURED_TRAFFIC and a.scale == 1.0 else None,
+            "traffic_source": MEASURED_TRAFFIC[a.config][1] if a.config in MEASURED_TRAFFIC else None,
             "kernel": "ragged_kernel (hipGraph replay)" if ragged else "dense_kernel",
             "algorithmic_flops_per_launch": flops,
             "sincos_per_launch": trigs,
