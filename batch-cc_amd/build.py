@@ -45,11 +45,18 @@ def run(cmd, **kw):
     return out
 
 
-def generate(order: str = "tree", gate_chunk: int = 4, fine_chunk: int = 8):
+def generate(order: str = "tree", gate_chunk: int | None = None, fine_chunk: int | None = None):
+    """Regenerate the robot check code; chunk sizes default to the
+    generator's per-robot table."""
     gen = os.path.join(HERE, "robots", "gen_robot_code.py")
+    extra = []
+    if gate_chunk:
+        extra += ["--gate-chunk", str(gate_chunk)]
+    if fine_chunk:
+        extra += ["--fine-chunk", str(fine_chunk)]
     for r in ROBOTS:
         run([sys.executable, gen, os.path.join(HERE, "robots", f"{r}.json"), os.path.join(CSRC, "robots"),
-             "--order", order, "--gate-chunk", str(gate_chunk), "--fine-chunk", str(fine_chunk)])
+             "--order", order, *extra])
 
 
 def compile_one(item, objdir=OBJDIR, defines=()):
@@ -64,8 +71,8 @@ def build_oracle():
     run(["make", "-s", "-C", os.path.join(REPO, "oracle")])
 
 
-def build(gen: bool = True, jobs: int = 8, defines=(), out: str = LIB, order: str = "tree", gate_chunk: int = 4,
-          fine_chunk: int = 8) -> str:
+def build(gen: bool = True, jobs: int = 8, defines=(), out: str = LIB, order: str = "tree",
+          gate_chunk: int | None = None, fine_chunk: int | None = None) -> str:
     """Compile and link; `defines` adds -D macros (tuning variants), `out`
     writes the library elsewhere (the in-tree libbcc.so is the product),
     `order` picks the generated check order (see gen_robot_code.schedule)."""
@@ -92,8 +99,8 @@ def main():
     ap.add_argument("-D", dest="defines", action="append", default=[], help="extra preprocessor macro")
     ap.add_argument("--out", default=LIB, help="library path (default: the in-tree libbcc.so)")
     ap.add_argument("--order", default="tree", choices=["tree", "program"], help="generated check order")
-    ap.add_argument("--gate-chunk", type=int, default=4, help="gate spheres per obstacle sweep")
-    ap.add_argument("--fine-chunk", type=int, default=8, help="fine spheres per gated sweep")
+    ap.add_argument("--gate-chunk", type=int, default=None, help="gate spheres per obstacle sweep (default: per robot)")
+    ap.add_argument("--fine-chunk", type=int, default=None, help="fine spheres per gated sweep (default: per robot)")
     a = ap.parse_args()
     print(build(gen=not a.no_gen, jobs=a.j, defines=a.defines, out=os.path.abspath(a.out), order=a.order,
                 gate_chunk=a.gate_chunk, fine_chunk=a.fine_chunk))

@@ -231,9 +231,24 @@ double mean_obstacles(const bcc_scene* scene)
     return sum / scene->num_envs;
 }
 
+int device_cus();
+
+// Edges per dense workgroup: 64, halved (down to 8) while the launch would
+// have fewer than 8 workgroups per CU, so small batches still fill the GPU
+// and end with a short tail.
+int dense_chunk(int num_envs, int num_edges)
+{
+    const long long target = 8LL * device_cus();
+    int chunk = 64;
+    while (chunk > 8 && (long long)num_envs * ((num_edges + chunk - 1) / chunk) < target)
+        chunk /= 2;
+    return chunk;
+}
+
 int dispatch_dense(int robot, const bcc_scene* scene, bcc::DenseArgs a, hipStream_t s)
 {
     a.variant = choose_variant(robot, mean_obstacles(scene));
+    a.chunk = dense_chunk(a.num_envs, a.num_edges);
     hipError_t e;
     switch (robot) {
     case BCC_PANDA: e = bcc::launch_dense_panda(a, s); break;

@@ -670,6 +670,7 @@ struct DenseArgs {
     const uint32_t* order = nullptr;   // learned top-4 bins per pair, or null (plain order)
     uint8_t* rounds = nullptr;         // rounds per pair, or null
     int variant = 0;                   // 1: high-occupancy kernel variant, 0: low
+    int chunk = 64;                    // edges per workgroup
 };
 
 // One launch of one dense kernel variant (instantiated in its own
@@ -688,7 +689,7 @@ hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
 {
     if (a.num_edges == 0 || a.num_envs == 0)
         return hipSuccess;
-    const int chunk = 64;
+    const int chunk = a.chunk > 0 ? a.chunk : 64;
     const int chunks_per_env = (a.num_edges + chunk - 1) / chunk;
     const long long blocks = (long long)chunks_per_env * a.num_envs;
     if (blocks > 0x7fffffffLL)

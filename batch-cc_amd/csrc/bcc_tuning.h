@@ -12,11 +12,11 @@ struct bcc_tune_entry {
 // Kernel variant per launch: the high-occupancy variant when the mean number
 // of tested obstacles per environment (weighted by pairs) is at least this,
 // per robot (panda, fetch, baxter).
-static const int kHiVariantMinObstacles[3] = {32, 32, 64};
+static const int kHiVariantMinObstacles[3] = {32, 16, 32};
 
-// source: tune.py on gfx950, 2026-12-03, seed 0
+// source: tune.py on gfx950, 2026-12-04, seed 0
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 16, 8},   // panda: score 1.003
-    {1, 32, 6},   // fetch: score 1.002
-    {2, 16, 6},   // baxter: score 1.000
+    {0, 16, 6},   // panda: score 1.001
+    {1, 32, 8},   // fetch: score 1.018
+    {2, 16, 6},   // baxter: score 1.001
 };

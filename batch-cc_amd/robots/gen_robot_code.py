@@ -218,9 +218,16 @@ def padded_radius(r: float) -> float:
     return f32(np.float64(np.float32(r)) + 0.005)
 
 
-GATE_CHUNK = 4      # gate/lone spheres per environment sweep
-FINE_CHUNK = 8      # fine spheres per gated sweep
+# Spheres per obstacle sweep, per robot (gate/lone spheres, fine spheres of a
+# gated group): fewer spheres per sweep keep fewer temporaries live (less
+# register spilling) and let a decided half-wave leave sooner; more spheres
+# amortise each obstacle's scalar loads.  Measured on MI355X over the
+# benchmark workloads (gate 1..4 x fine 2..8).
+CHUNKS = {"panda": (2, 4), "fetch": (1, 4), "baxter": (2, 4)}
+GATE_CHUNK = 2      # defaults for robots not listed; set per robot by gen_check
+FINE_CHUNK = 4
 ORDER = "tree"      # "tree": kinematic-tree order; "program": the reference's order
+_CHUNK_OVERRIDE = False   # set by --gate-chunk / --fine-chunk
 
 
 def tree_rank(model: dict) -> dict:
@@ -286,8 +293,11 @@ def gen_check(model: dict, order: str | None = None) -> str:
     gate, then the gated fine cross product), in the order schedule()
     gives.  The evaluation order differs from the reference's interleaving;
     the verdict (an OR over gated tests) does not depend on it."""
+    global GATE_CHUNK, FINE_CHUNK
     order = order or ORDER
     robot = model["robot"]
+    if not _CHUNK_OVERRIDE:
+        GATE_CHUNK, FINE_CHUNK = CHUNKS.get(robot, (GATE_CHUNK, FINE_CHUNK))
     dim = model["dimension"]
     sph = model["spheres"]
     em = Emitter()
@@ -549,7 +559,7 @@ def gen_model_header(model: dict) -> str:
 
 
 def main(argv):
-    global GATE_CHUNK, FINE_CHUNK
+    global GATE_CHUNK, FINE_CHUNK, _CHUNK_OVERRIDE
     order = ORDER
     args = list(argv[1:])
     pos = []
@@ -559,8 +569,10 @@ def main(argv):
             order = args.pop(0)
         elif a == "--gate-chunk":
             GATE_CHUNK = int(args.pop(0))
+            _CHUNK_OVERRIDE = True
         elif a == "--fine-chunk":
             FINE_CHUNK = int(args.pop(0))
+            _CHUNK_OVERRIDE = True
         else:
             pos.append(a)
     argv = [argv[0]] + pos
