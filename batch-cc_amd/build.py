@@ -45,7 +45,8 @@ def run(cmd, **kw):
     return out
 
 
-def generate(order: str = "tree", gate_chunk: int | None = None, fine_chunk: int | None = None):
+def generate(order: str = "tree", gate_chunk: int | None = None, fine_chunk: int | None = None,
+             first_chunk: int | None = None):
     """Regenerate the robot check code; chunk sizes default to the
     generator's per-robot table."""
     gen = os.path.join(HERE, "robots", "gen_robot_code.py")
@@ -54,6 +55,8 @@ def generate(order: str = "tree", gate_chunk: int | None = None, fine_chunk: int
         extra += ["--gate-chunk", str(gate_chunk)]
     if fine_chunk:
         extra += ["--fine-chunk", str(fine_chunk)]
+    if first_chunk:
+        extra += ["--first-chunk", str(first_chunk)]
     for r in ROBOTS:
         run([sys.executable, gen, os.path.join(HERE, "robots", f"{r}.json"), os.path.join(CSRC, "robots"),
              "--order", order, *extra])
@@ -72,12 +75,12 @@ def build_oracle():
 
 
 def build(gen: bool = True, jobs: int = 8, defines=(), out: str = LIB, order: str = "tree",
-          gate_chunk: int | None = None, fine_chunk: int | None = None) -> str:
+          gate_chunk: int | None = None, fine_chunk: int | None = None, first_chunk: int | None = None) -> str:
     """Compile and link; `defines` adds -D macros (tuning variants), `out`
     writes the library elsewhere (the in-tree libbcc.so is the product),
     `order` picks the generated check order (see gen_robot_code.schedule)."""
     if gen:
-        generate(order, gate_chunk, fine_chunk)
+        generate(order, gate_chunk, fine_chunk, first_chunk)
     objdir = OBJDIR if out == LIB else out + ".objs"
     os.makedirs(objdir, exist_ok=True)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
@@ -101,9 +104,10 @@ def main():
     ap.add_argument("--order", default="tree", choices=["tree", "program"], help="generated check order")
     ap.add_argument("--gate-chunk", type=int, default=None, help="gate spheres per obstacle sweep (default: per robot)")
     ap.add_argument("--fine-chunk", type=int, default=None, help="fine spheres per gated sweep (default: per robot)")
+    ap.add_argument("--first-chunk", type=int, default=None, help="gate spheres in the first sweep (default: gate chunk)")
     a = ap.parse_args()
     print(build(gen=not a.no_gen, jobs=a.j, defines=a.defines, out=os.path.abspath(a.out), order=a.order,
-                gate_chunk=a.gate_chunk, fine_chunk=a.fine_chunk))
+                gate_chunk=a.gate_chunk, fine_chunk=a.fine_chunk, first_chunk=a.first_chunk))
 
 
 if __name__ == "__main__":

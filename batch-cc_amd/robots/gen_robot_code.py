@@ -218,14 +218,17 @@ def padded_radius(r: float) -> float:
     return f32(np.float64(np.float32(r)) + 0.005)
 
 
-# Spheres per obstacle sweep, per robot (gate/lone spheres, fine spheres of a
-# gated group): fewer spheres per sweep keep fewer temporaries live (less
-# register spilling) and let a decided half-wave leave sooner; more spheres
-# amortise each obstacle's scalar loads.  Measured on MI355X over the
-# benchmark workloads (gate 1..4 x fine 2..8).
-CHUNKS = {"panda": (2, 4), "fetch": (1, 4), "baxter": (2, 4)}
+# Spheres per obstacle sweep, per robot: (first environment chunk, later
+# environment chunks, fine spheres of a gated group).  Fewer spheres per sweep
+# keep fewer temporaries live (less register spilling) and let a decided
+# half-wave leave sooner; more spheres amortise each obstacle's scalar loads.
+# A first chunk of one gate lets samples that hit the base links leave after
+# one sweep.  Measured on MI355X over the benchmark workloads (gate 1..4,
+# fine 2..8, first chunk 1 or the gate chunk).
+CHUNKS = {"panda": (2, 2, 4), "fetch": (1, 2, 4), "baxter": (1, 2, 4)}
 GATE_CHUNK = 2      # defaults for robots not listed; set per robot by gen_check
 FINE_CHUNK = 4
+FIRST_CHUNK = 0     # size of the first environment chunk (0: GATE_CHUNK)
 ORDER = "tree"      # "tree": kinematic-tree order; "program": the reference's order
 _CHUNK_OVERRIDE = False   # set by --gate-chunk / --fine-chunk
 
@@ -273,7 +276,9 @@ def schedule(model: dict, order: str):
         return max(rank[sph[s]["frame"]] for s in spheres)
 
     env_sorted = sorted(prog, key=lambda gi: (key([envp[gi]["gate"]] + envp[gi]["fine"]), prog.index(gi)))
-    chunks = [env_sorted[c:c + GATE_CHUNK] for c in range(0, len(env_sorted), GATE_CHUNK)]
+    first = FIRST_CHUNK or GATE_CHUNK
+    chunks = [env_sorted[:first]] + [env_sorted[c:c + GATE_CHUNK] for c in range(first, len(env_sorted), GATE_CHUNK)]
+    chunks = [c for c in chunks if c]
     selfs = sorted(range(nself), key=lambda i: (key(model["self_program"][i]["gate"]), i))
     out, si = [], 0
     for ch in chunks:
@@ -293,11 +298,11 @@ def gen_check(model: dict, order: str | None = None) -> str:
     gate, then the gated fine cross product), in the order schedule()
     gives.  The evaluation order differs from the reference's interleaving;
     the verdict (an OR over gated tests) does not depend on it."""
-    global GATE_CHUNK, FINE_CHUNK
+    global GATE_CHUNK, FINE_CHUNK, FIRST_CHUNK
     order = order or ORDER
     robot = model["robot"]
     if not _CHUNK_OVERRIDE:
-        GATE_CHUNK, FINE_CHUNK = CHUNKS.get(robot, (GATE_CHUNK, FINE_CHUNK))
+        FIRST_CHUNK, GATE_CHUNK, FINE_CHUNK = CHUNKS.get(robot, (FIRST_CHUNK, GATE_CHUNK, FINE_CHUNK))
     dim = model["dimension"]
     sph = model["spheres"]
     em = Emitter()
@@ -559,7 +564,7 @@ def gen_model_header(model: dict) -> str:
 
 
 def main(argv):
-    global GATE_CHUNK, FINE_CHUNK, _CHUNK_OVERRIDE
+    global GATE_CHUNK, FINE_CHUNK, FIRST_CHUNK, _CHUNK_OVERRIDE
     order = ORDER
     args = list(argv[1:])
     pos = []
@@ -573,6 +578,8 @@ def main(argv):
         elif a == "--fine-chunk":
             FINE_CHUNK = int(args.pop(0))
             _CHUNK_OVERRIDE = True
+        elif a == "--first-chunk":
+            FIRST_CHUNK = int(args.pop(0))
         else:
             pos.append(a)
     argv = [argv[0]] + pos
