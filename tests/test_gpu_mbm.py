@@ -85,3 +85,22 @@ def test_start_and_goal_configurations_on_the_gpu(bcc, oracle, gpu, robot, rid):
     print(f"{robot}: {len(own)} configurations, {int(got[np.arange(len(own)), own].sum())} in contact with "
           f"their own scene, min margin {own_margin.min():.4f} m; {got.mean():.3f} of all (config, scene) pairs collide")
     assert own_margin.min() > -0.005
+
+
+def test_readme_driver_prints_the_oracle_matrix(bcc, oracle, gpu):
+    """`batch_cc_mbm.py panda 100 200` prints the reference driver's lines; its
+    verdict matrix equals the oracle's on the same inputs."""
+    import subprocess
+    import sys
+    from batch_cc_amd import mbm
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(repo, "batch-cc_amd", "batch_cc_mbm.py"), "panda", "100", "200"],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    assert lines[3] == "Edges checked: 20000" and lines[4].startswith("Throughput: ")
+    rows = [ln for ln in lines if ln.startswith("Edge ") and ":" in ln and not ln.startswith("Edges")]
+    assert len(rows) == 200 and rows[0].startswith("Edge 0: ")
+    got = np.array([[int(v) for v in ln.split(":")[1].split()] for ln in rows], np.uint8)
+    sc, edges = mbm.readme_run(_problems("panda"), "panda", 100, 200)
+    assert np.array_equal(got, oracle.check_dense(0, sc, edges, threads=16))

@@ -216,3 +216,15 @@ def test_oracle_fp32_agrees_with_fp64_on_the_readme_scenes(mbm, panda_problems, 
     assert np.all(np.abs(mg[bad]) < 1e-4), mg[bad]
     print(f"collision fraction {v32.mean():.3f}, FP32/FP64 disagreements {int(bad.sum())}")
     assert 0.0 < v32.mean() < 1.0
+
+
+def test_readme_driver_argument_errors_match_the_reference(capsys):
+    import importlib.util
+    path = os.path.join(os.path.dirname(HERE), "batch-cc_amd", "batch_cc_mbm.py")
+    spec = importlib.util.spec_from_file_location("batch_cc_mbm", path)
+    drv = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(drv)
+    assert drv.main(["panda", "100"]) == 1
+    assert "Usage: ./batch_cc <robot_name> <num_environments> <num_edges>" in capsys.readouterr().out
+    assert drv.main(["xarm", "1", "1"]) == 255
+    assert "Unknown robot name: xarm" in capsys.readouterr().out
