@@ -16,7 +16,7 @@ static const int kHiVariantMinObstacles[3] = {32, 16, 32};
 
 // source: tune.py on gfx950, 2026-12-04, seed 0
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 16, 6},   // panda: score 1.001
-    {1, 32, 8},   // fetch: score 1.018
+    {0, 16, 6},   // panda: score 1.003
+    {1, 8, 8},   // fetch: score 1.016
     {2, 16, 6},   // baxter: score 1.001
 };

@@ -10,7 +10,7 @@ namespace bcc {
 // Kernel variants (waves per SIMD, obstacle-loop unroll), measured on MI355X:
 // high occupancy for scenes with many obstacles per environment, low for few.
 #ifndef BCC_BAXTER_VARIANTS
-#define BCC_BAXTER_VARIANTS 5, 2, 4, 1
+#define BCC_BAXTER_VARIANTS 5, 1, 4, 1
 #endif
 #define BCC_ROBOT baxter
 #define BCC_ROBOT_T Robot_baxter
