@@ -78,6 +78,7 @@ def lib():
     _sig(L, "bcc_ragged_plan_destroy", I, [P])
     _sig(L, "bcc_ragged_plan_create_tuned", I, [I, P, P, P, I, P, I, I, ctypes.POINTER(P)])
     _sig(L, "bcc_ragged_plan_config", I, [P, ctypes.POINTER(I), ctypes.POINTER(I)])
+    _sig(L, "bcc_ragged_plan_create_ordered", I, [I, P, P, P, I, P, P, I, I, ctypes.POINTER(P)])
     _sig(L, "bcc_ragged_group_create", I, [P, I, ctypes.POINTER(P)])
     _sig(L, "bcc_ragged_group_launch", I, [P, P])
     _sig(L, "bcc_ragged_group_time", I, [P, I, I, P, ctypes.POINTER(ctypes.c_double)])
@@ -93,6 +94,7 @@ def lib():
     _sig(L, "bcc_model_load", I, [ctypes.c_char_p, ctypes.POINTER(P)])
     _sig(L, "bcc_encode_envs", I, [P, P])
     _sig(L, "bcc_model_features", I, [P, P, P, ctypes.c_int64, P, P, P, P])
+    _sig(L, "bcc_model_order_pairs", I, [P, P, P, ctypes.c_int64, P, P, P, P, P])
     _sig(L, "bcc_label_pairs", I, [I, P, P, I, P, P, I, P, P])
     _sig(L, "bcc_model_grads", I, [P, P, P, I, P, ctypes.POINTER(ctypes.c_float)])
     _sig(L, "bcc_model_train", I, [P, P, P, I, I, I, ctypes.c_float, P])
@@ -331,17 +333,25 @@ def check_ragged(robot, scene: Scene, env_off, edges, resolution: int = 32) -> n
 
 
 class RaggedPlan:
-    """Tile list + captured hipGraph for repeated ragged launches on device buffers."""
+    """Tile list + captured hipGraph for repeated ragged launches on device buffers.
+    d_order (optional): a learned visiting order per pair (uint32 per pair,
+    e.g. from Model.order_pairs), read at every launch."""
 
     def __init__(self, robot, scene: Scene, env_off, d_edges: DeviceBuffer, d_out: DeviceBuffer,
-                 resolution: int = 32, tile_pairs: int = 0, groups_per_cu: int = 0):
+                 resolution: int = 32, tile_pairs: int = 0, groups_per_cu: int = 0,
+                 d_order: DeviceBuffer | None = None):
         self.off = np.ascontiguousarray(env_off, np.int64)
         self.robot = robot
-        self.scene, self.d_edges, self.d_out = scene, d_edges, d_out
+        self.scene, self.d_edges, self.d_out, self.d_order = scene, d_edges, d_out, d_order
         self.handle = ctypes.c_void_p()
-        _check(lib().bcc_ragged_plan_create_tuned(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
-                                                  int(resolution), d_out.ptr, int(tile_pairs), int(groups_per_cu),
-                                                  ctypes.byref(self.handle)))
+        if d_order is None:
+            _check(lib().bcc_ragged_plan_create_tuned(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
+                                                      int(resolution), d_out.ptr, int(tile_pairs),
+                                                      int(groups_per_cu), ctypes.byref(self.handle)))
+        else:
+            _check(lib().bcc_ragged_plan_create_ordered(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
+                                                        int(resolution), d_order.ptr, d_out.ptr, int(tile_pairs),
+                                                        int(groups_per_cu), ctypes.byref(self.handle)))
 
     @property
     def num_tiles(self) -> int:
@@ -569,6 +579,29 @@ class Model:
         _check(lib().bcc_model_logits_dense(self.handle, scene.handle, d_edges.ptr, M, d.ptr, dl.ptr, None))
         synchronize()
         return d.to_array(np.uint32, (M, E)), dl.to_array(np.float32, (M, E, self.OUT))
+
+    def order_pairs(self, scene: Scene, d_edges: DeviceBuffer, pair_env, pair_edge: np.ndarray | None = None,
+                    logits: bool = False, out: DeviceBuffer | None = None):
+        """Top-4 bins of the pairs (pair_env[p], pair_edge[p] or p) from the
+        fused kernel: a device buffer of uint32 per pair (`out` if given), and
+        with logits=True also the float32 [pairs, 32] logits (host).
+        pair_env: int32 host array, or a DeviceBuffer holding one."""
+        if isinstance(pair_env, DeviceBuffer):
+            d_env, n = pair_env, pair_env.nbytes // 4
+        else:
+            pe = np.ascontiguousarray(pair_env, np.int32)
+            d_env, n = DeviceBuffer.from_array(pe), int(pe.shape[0])
+        d_edge = DeviceBuffer.from_array(np.ascontiguousarray(pair_edge, np.int32)) if pair_edge is not None else None
+        d = out if out is not None else DeviceBuffer(max(n, 1) * 4)
+        if d.nbytes < 4 * n:
+            raise ValueError("order buffer too small")
+        dl = DeviceBuffer(max(n, 1) * self.OUT * 4) if logits else None
+        _check(lib().bcc_model_order_pairs(self.handle, scene.handle, d_edges.ptr, n, d_env.ptr,
+                                           d_edge.ptr if d_edge else None, d.ptr, dl.ptr if dl else None, None))
+        synchronize()
+        if logits:
+            return d, dl.to_array(np.float32, (n, self.OUT))
+        return d
 
     def close(self):
         if self.handle:

@@ -765,9 +765,15 @@ int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env
                                         out);
 }
 
-int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_t* env_offsets,
-                                 const float* d_edges, int resolution, uint8_t* d_results, int tile_pairs,
-                                 int groups_per_cu, bcc_ragged_plan** out)
+}  // extern "C"
+
+namespace {
+
+// Plan over one CSR layout; `ordered` selects the ordered kernel (d_order:
+// one word per pair, on the device), which runs the low-occupancy variant.
+int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* d_edges, int resolution,
+                bool ordered, const uint32_t* d_order, uint8_t* d_results, int tile_pairs, int groups_per_cu,
+                bcc_ragged_plan** out)
 {
     if (!out)
         return fail(BCC_ERR_INVALID, "out is NULL");
@@ -787,12 +793,14 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_
         return fail(BCC_ERR_INVALID, "more than 2^31 pairs in one ragged batch");
     if (total > 0 && (!d_edges || !d_results))
         return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
+    if (ordered && total > 0 && !d_order)
+        return fail(BCC_ERR_INVALID, "d_order is NULL");
     if (tile_pairs < -1024 || groups_per_cu < 0 || groups_per_cu > 32)
         return fail(BCC_ERR_INVALID, "tile_pairs must be >= -1024 and groups_per_cu in 0..32");
     double weighted = 0.0;   // mean obstacles per environment, weighted by pairs
     for (int e = 0; e < E; ++e)
         weighted += (double)scene->obstacles[e] * (double)(env_offsets[e + 1] - env_offsets[e]);
-    const int variant = choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
+    const int variant = ordered ? 0 : choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
     const bcc_tune_entry& tune = tuned(robot);
     if (groups_per_cu == 0)
         groups_per_cu = tune.groups_per_cu;
@@ -832,6 +840,7 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_
     p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
                               resolution, device_cus() * groups_per_cu, d_results};
     p->args.variant = variant;
+    p->args.order = d_order;
     // capture reset + persistent kernel once; launches replay the graph
     hipStream_t cap;
     err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
@@ -859,6 +868,26 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_
     }
     *out = p;
     return BCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_t* env_offsets,
+                                 const float* d_edges, int resolution, uint8_t* d_results, int tile_pairs,
+                                 int groups_per_cu, bcc_ragged_plan** out)
+{
+    return create_plan(robot, scene, env_offsets, d_edges, resolution, false, nullptr, d_results, tile_pairs,
+                       groups_per_cu, out);
+}
+
+int bcc_ragged_plan_create_ordered(int robot, const bcc_scene* scene, const int64_t* env_offsets,
+                                   const float* d_edges, int resolution, const uint32_t* d_order,
+                                   uint8_t* d_results, int tile_pairs, int groups_per_cu, bcc_ragged_plan** out)
+{
+    return create_plan(robot, scene, env_offsets, d_edges, resolution, true, d_order, d_results, tile_pairs,
+                       groups_per_cu, out);
 }
 
 int bcc_ragged_plan_launch(bcc_ragged_plan* plan, void* stream)

@@ -133,6 +133,15 @@ __device__ __forceinline__ int setup_pair_sd(const float* __restrict__ se, int r
     return n;
 }
 
+// A learned order word names four distinct bins below 32, one byte each;
+// anything else means plain order (0xffffffff).
+__device__ __forceinline__ uint32_t valid_order(uint32_t top)
+{
+    const uint32_t taken = (1u << (top & 31)) | (1u << ((top >> 8) & 31)) | (1u << ((top >> 16) & 31)) |
+                           (1u << ((top >> 24) & 31));
+    return (__popc(taken) != 4 || (top & 0xe0e0e0e0u)) ? 0xffffffffu : top;
+}
+
 // Per-thread sample configuration (and, in plain order, the per-step
 // increment) live in LDS, [joint][thread], not in registers: the check
 // program reads q[j] once per joint (for its sin/cos) and the step loop adds
@@ -210,11 +219,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 step = 0;
                 if constexpr (kOrdered) {
                     n = setup_pair_sd<D>(edges + (size_t)g * 2 * D, resolution, s0, d);
-                    top = order[(size_t)g * num_envs + env];
-                    const uint32_t taken = (1u << (top & 31)) | (1u << ((top >> 8) & 31)) |
-                                           (1u << ((top >> 16) & 31)) | (1u << ((top >> 24) & 31));
-                    if (__popc(taken) != 4 || (top & 0xe0e0e0e0u))
-                        top = 0xffffffffu;
+                    top = valid_order(order[(size_t)g * num_envs + env]);
                     float qq[D];
                     ordered_sample<D>(s0, d, n, top, 0, t, qq);
 #pragma unroll
@@ -279,23 +284,32 @@ struct RaggedTile {
     int pad;
 };
 
-template <class Robot, int kWaves, int kUnroll>
+//
+// kOrdered: order[p] holds pair p's learned top-4 bins (as in dense_kernel).
+template <class Robot, bool kOrdered, int kWaves, int kUnroll>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void ragged_kernel(
     const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges,
     const RaggedTile* __restrict__ tiles, int num_tiles, int* __restrict__ tile_counter, int resolution,
-    uint8_t* __restrict__ out)
+    const uint32_t* __restrict__ order, uint8_t* __restrict__ out)
 {
     constexpr int D = Robot::dim;
     constexpr int kBlockWaves = kBlock / 64;
     __shared__ int next_pair[kBlockWaves];
     __shared__ float qs[D][kBlock];
-    __shared__ float ds[D][kBlock];
+    __shared__ float ds[kOrdered ? 1 : D][kBlock];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int t = lane & 31;
     const int half0 = lane & 32;
     const LdsRow Q{&qs[0][threadIdx.x]};
     const LdsRow Dl{&ds[0][threadIdx.x]};
+    float s0[kOrdered ? D : 1], d[kOrdered ? D : 1];
+#pragma unroll
+    for (int j = 0; j < (kOrdered ? D : 1); ++j) {
+        s0[j] = 0.0f;
+        d[j] = 0.0f;
+    }
+    uint32_t top = 0;
 
     for (;;) {
         // -- take a tile (wave-uniform)
@@ -325,7 +339,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 if (g < p_end) {
                     cur = g;
                     step = 0;
-                    n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
+                    if constexpr (kOrdered) {
+                        n = setup_pair_sd<D>(edges + (size_t)g * 2 * D, resolution, s0, d);
+                        top = valid_order(order[g]);
+                        float qq[D];
+                        ordered_sample<D>(s0, d, n, top, 0, t, qq);
+#pragma unroll
+                        for (int j = 0; j < D; ++j)
+                            Q.at(j) = qq[j];
+                    } else {
+                        n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
+                    }
                 } else {
                     exhausted = true;
                 }
@@ -341,9 +365,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                     ++step;
                     fin = (step == n);
                     if (!fin) {
+                        if constexpr (kOrdered) {
+                            float qq[D];
+                            ordered_sample<D>(s0, d, n, top, step, t, qq);
 #pragma unroll
-                        for (int j = 0; j < D; ++j)
-                            Q.at(j) = Q[j] + Dl[j];
+                            for (int j = 0; j < D; ++j)
+                                Q.at(j) = qq[j];
+                        } else {
+#pragma unroll
+                            for (int j = 0; j < D; ++j)
+                                Q.at(j) = Q[j] + Dl[j];
+                        }
                     }
                 }
                 if (fin) {
@@ -374,7 +406,8 @@ struct RaggedArgs {
     int resolution;
     int grid;        // persistent workgroups
     uint8_t* out;
-    int variant = 0; // 1: high-occupancy kernel variant, 0: low
+    int variant = 0;                   // 1: high-occupancy kernel variant, 0: low
+    const uint32_t* order = nullptr;   // learned top-4 bins per pair, or null (plain order)
 };
 
 // Kernel variant knobs of a robot: (waves per SIMD, obstacle-loop unroll) of
@@ -385,11 +418,11 @@ struct VariantKnobs {
 
 // One persistent launch of one kernel variant (instantiated in its own
 // translation unit, see bcc_robot_parts.h).
-template <class Robot, int W, int U>
+template <class Robot, bool kOrdered, int W, int U>
 hipError_t launch_ragged_variant(const RaggedArgs& a, hipStream_t stream)
 {
-    hipLaunchKernelGGL((ragged_kernel<Robot, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
-                       a.tiles, a.num_tiles, a.tile_counter, a.resolution, a.out);
+    hipLaunchKernelGGL((ragged_kernel<Robot, kOrdered, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta,
+                       a.edges, a.tiles, a.num_tiles, a.tile_counter, a.resolution, a.order, a.out);
     return hipGetLastError();
 }
 
@@ -399,9 +432,11 @@ hipError_t launch_ragged(const RaggedArgs& a, hipStream_t stream)
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.tile_counter);
     if (a.num_tiles == 0)
         return hipGetLastError();
+    if (a.order)   // the ordered path keeps its pair state in registers: low-occupancy variant
+        return launch_ragged_variant<Robot, true, LW, LU>(a, stream);
     if (a.variant)
-        return launch_ragged_variant<Robot, HW, HU>(a, stream);
-    return launch_ragged_variant<Robot, LW, LU>(a, stream);
+        return launch_ragged_variant<Robot, false, HW, HU>(a, stream);
+    return launch_ragged_variant<Robot, false, LW, LU>(a, stream);
 }
 
 // ------------------------------------------------------------ test hooks

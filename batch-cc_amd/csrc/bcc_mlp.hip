@@ -270,8 +270,8 @@ __device__ __forceinline__ float dense_feature(const float* __restrict__ enc, co
 __global__ __launch_bounds__(256) void fused_order_kernel(
     const bf16x4* __restrict__ frag, const float* __restrict__ b1, const float* __restrict__ b2,
     const float* __restrict__ b3, const float* __restrict__ enc, const float* __restrict__ edges, int dim,
-    const float* __restrict__ lo, const float* __restrict__ range, int num_envs, long long num_pairs,
-    uint32_t* __restrict__ order, float* __restrict__ logits)
+    const float* __restrict__ lo, const float* __restrict__ range, int num_envs, const int* __restrict__ pair_env,
+    const int* __restrict__ pair_edge, long long num_pairs, uint32_t* __restrict__ order, float* __restrict__ logits)
 {
     __shared__ float zs[4][64][kOut + 1];
     const int lane = threadIdx.x & 63;
@@ -286,8 +286,12 @@ __global__ __launch_bounds__(256) void fused_order_kernel(
     for (int ct = 0; ct < kFuseCols; ++ct) {
         const long long p = base + 16 * ct + col;
         const bool ok = p < num_pairs;
-        const int env = ok ? (int)(p % num_envs) : 0;
-        const long long edge = ok ? p / num_envs : 0;
+        int env = 0;
+        long long edge = 0;
+        if (ok) {   // same pair mapping as features_kernel
+            env = pair_env ? pair_env[p] : (int)(p % num_envs);
+            edge = pair_edge ? (long long)pair_edge[p] : (pair_env ? p : p / num_envs);
+        }
 #pragma unroll
         for (int kt = 0; kt < kKT1; ++kt)
 #pragma unroll
@@ -912,14 +916,15 @@ int bcc_model_logits(bcc_model* M, const uint16_t* d_x, int64_t rows, float* d_l
 
 namespace {
 
-// Fused inference over every (edge, env) pair of a dense batch: pack the
-// weight fragments, encode the environments, one kernel for all layers and
-// the top-4 selection.  d_logits (optional) receives [pairs][32].
-int order_dense_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges, int num_edges, uint32_t* d_order,
-                      float* d_logits, hipStream_t s)
+// Fused inference: pack the weight fragments, encode the environments, one
+// kernel for all layers and the top-4 selection.  Pairs: every (edge, env)
+// pair of a dense batch, edge-major (pair_env null), or the list
+// (pair_env[p], pair_edge ? pair_edge[p] : p).  d_logits (optional) receives
+// [pairs][32].
+int order_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges, long long pairs, const int* pair_env,
+                const int* pair_edge, uint32_t* d_order, float* d_logits, hipStream_t s)
 {
-    const long long pairs = (long long)num_edges * scene->num_envs;
-    if (pairs == 0)
+    if (pairs == 0 || scene->num_envs == 0)
         return BCC_OK;
     float* enc = nullptr;
     BCC_HIP(hipMalloc((void**)&enc, sizeof(float) * kEnc * scene->num_envs), "order scratch");
@@ -929,7 +934,7 @@ int order_dense_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges
                        scene->num_envs, enc);
     hipLaunchKernelGGL(fused_order_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s,
                        (const bf16x4*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, enc, d_edges, M->dim, M->lo,
-                       M->range, scene->num_envs, pairs, d_order, d_logits);
+                       M->range, scene->num_envs, pair_env, pair_edge, pairs, d_order, d_logits);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipStreamSynchronize(s);
@@ -948,7 +953,8 @@ int bcc_model_order_dense(bcc_model* M, const bcc_scene* scene, const float* d_e
 {
     if (!M || !scene || num_edges < 0 || (num_edges > 0 && (!d_edges || !d_order)))
         return fail(BCC_ERR_INVALID, "bad arguments");
-    return order_dense_fused(M, scene, d_edges, num_edges, d_order, nullptr, (hipStream_t)stream);
+    return order_fused(M, scene, d_edges, (long long)num_edges * scene->num_envs, nullptr, nullptr, d_order, nullptr,
+                       (hipStream_t)stream);
 }
 
 int bcc_model_logits_dense(bcc_model* M, const bcc_scene* scene, const float* d_edges, int num_edges,
@@ -956,7 +962,20 @@ int bcc_model_logits_dense(bcc_model* M, const bcc_scene* scene, const float* d_
 {
     if (!M || !scene || num_edges < 0 || (num_edges > 0 && (!d_edges || !d_order || !d_logits)))
         return fail(BCC_ERR_INVALID, "bad arguments");
-    return order_dense_fused(M, scene, d_edges, num_edges, d_order, d_logits, (hipStream_t)stream);
+    return order_fused(M, scene, d_edges, (long long)num_edges * scene->num_envs, nullptr, nullptr, d_order,
+                       d_logits, (hipStream_t)stream);
+}
+
+int bcc_model_order_pairs(bcc_model* M, const bcc_scene* scene, const float* d_edges, int64_t num_pairs,
+                          const int32_t* d_pair_env, const int32_t* d_pair_edge, uint32_t* d_order, float* d_logits,
+                          void* stream)
+{
+    if (!M || !scene || num_pairs < 0 || (num_pairs > 0 && (!d_edges || !d_pair_env || !d_order)))
+        return fail(BCC_ERR_INVALID, "bad arguments");
+    if (num_pairs > 0 && scene->num_envs == 0)
+        return fail(BCC_ERR_INVALID, "pairs given for a scene without environments");
+    return order_fused(M, scene, d_edges, (long long)num_pairs, d_pair_env, d_pair_edge, d_order, d_logits,
+                       (hipStream_t)stream);
 }
 
 int bcc_debug_gemm(int ta, int tb, const uint16_t* d_a, int lda, const uint16_t* d_b, int ldb, float* d_c, int ldc,

@@ -13,6 +13,7 @@
 //   1      ordered dense kernel (low-occupancy knobs)
 //   2, 3   plain dense kernel, high / low occupancy
 //   4, 5   ragged kernel, high / low occupancy
+//   6      ordered ragged kernel (low-occupancy knobs)
 #ifndef BCC_PART
 #define BCC_PART -1
 #endif
@@ -30,16 +31,18 @@ constexpr int HW = kKnobs.hi_waves, HU = kKnobs.hi_unroll, LW = kKnobs.lo_waves,
 #define BCC_CAT2(a, b) BCC_CAT2_(a, b)
 
 #if BCC_PART == 0
-// instantiated in parts 1-5
+// instantiated in parts 1-6
 extern template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
 extern template hipError_t launch_dense_variant<robot_parts::RobotT, false, robot_parts::HW, robot_parts::HU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
 extern template hipError_t launch_dense_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::HW, robot_parts::HU>(
     const RaggedArgs&, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
+    const RaggedArgs&, hipStream_t);
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
     const RaggedArgs&, hipStream_t);
 #endif
 
@@ -93,12 +96,16 @@ template hipError_t launch_dense_variant<robot_parts::RobotT, false, robot_parts
     const DenseArgs&, unsigned, int, int, hipStream_t);
 #endif
 #if BCC_PART == 4
-template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(const RaggedArgs&,
-                                                                                                 hipStream_t);
+template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::HW, robot_parts::HU>(
+    const RaggedArgs&, hipStream_t);
 #endif
 #if BCC_PART == 5
-template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(const RaggedArgs&,
-                                                                                                 hipStream_t);
+template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
+    const RaggedArgs&, hipStream_t);
+#endif
+#if BCC_PART == 6
+template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+    const RaggedArgs&, hipStream_t);
 #endif
 
 }  // namespace bcc

@@ -26,6 +26,14 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.abspath(__file__))
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet)
+BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 MFMA peak (datasheet)
+DIM = {"panda": 7, "fetch": 8, "baxter": 14}
+
+
+def mlp_flops(robot: str) -> int:
+    """Algorithmic flops of one inference: 3 layers over the unpadded input
+    (64-cell encoding + start and end)."""
+    return 2 * ((64 + 2 * DIM[robot]) * 256 + 256 * 256 + 256 * 32)
 HBM_PEAK_GBS = 8000.0
 METRIC = "edge-env checks/sec at bit-exact verdicts, learned-ordered vs plain, 1 GPU"
 # HBM bytes per launch of the dense kernel, measured with rocprofv3 FETCH_SIZE
@@ -158,10 +166,14 @@ def learned_leg(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, plai
     if mismatches:
         print("[bench] learned-ordered verdicts differ from plain order", file=sys.stderr)
         sys.exit(1)
+    mfma = mlp_flops(robot) * E * M / (order_ms / 1e3) / 1e12
     return {
         "ms_per_step": order_ms + check_ms,
         "order_ms": order_ms,
         "ordered_check_ms": check_ms,
+        "inference_mfma": {"achieved": mfma, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": mfma / BF16_PEAK_TFLOPS, "flops_per_pair": mlp_flops(robot),
+                           "note": "wall time of the inference call (fragment packing, encoder, fused kernel)"},
         "speedup_vs_plain": plain_ms / (order_ms + check_ms),
         "check_only_speedup_vs_plain": plain_ms / check_ms,
         "verdict_mismatches_vs_plain": mismatches,
@@ -172,6 +184,85 @@ def learned_leg(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, plai
         "train": {"pairs": n, "epochs": a.train_epochs, "label_s": t_label, "train_s": t_train,
                   "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
                   "edges": "separate seeded edge stream, same environments"},
+    }
+
+
+def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
+    """Learned leg of the mixed-robot batch: one model per robot, trained on
+    labels the exact checker writes for seeded (env, edge) pairs of that
+    robot's environments with a separate lognormal edge stream (the batch's
+    edge law); one learned step = inference for every pair of every robot
+    (bcc_model_order_pairs) + one replay of a group of ordered plans."""
+    n = int(a.train_pairs)
+    for k, r in enumerate(runs):
+        rng = np.random.default_rng([a.seed, 3, k])
+        E = r["sc"].num_envs
+        n_edges = max(1, min(r["edges"].shape[0], 100_000))
+        train = W.lognormal_edges(r["robot"], n_edges, rng, median=W.MIXED_EDGE_MEDIAN)
+        d_train = bcc.DeviceBuffer.from_array(train)
+        pe = rng.integers(0, E, n).astype(np.int32)
+        pg = rng.integers(0, n_edges, n).astype(np.int32)
+        t0 = time.perf_counter()
+        d_labels = bcc.label_pairs(r["robot"], r["scene"], d_train, pe, pg)
+        model = bcc.Model(r["robot"], seed=a.seed)
+        d_x = model.features(r["scene"], d_train, pe, pg)
+        losses = model.train(d_x, d_labels, n, epochs=a.train_epochs, batch=8192, lr=2e-3)
+        bcc.synchronize()
+        del d_x, d_labels, d_train
+        r["model"], r["losses"] = model, losses
+        r["pair_env"] = np.repeat(np.arange(E, dtype=np.int32), np.diff(r["off"]))
+        print(f"[bench] learned {r['robot']}: {n} labels, {a.train_epochs} epochs in "
+              f"{time.perf_counter() - t0:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}",
+              file=sys.stderr, flush=True)
+    # inference time per robot (order every pair of the part)
+    order_ms = {}
+    for r in runs:
+        d_env = bcc.DeviceBuffer.from_array(r["pair_env"])
+        r["d_order"] = r["model"].order_pairs(r["scene"], r["d_edges"], d_env)
+        ts = []
+        for _ in range(max(3, a.steps // 4)):
+            t0 = time.perf_counter()
+            r["model"].order_pairs(r["scene"], r["d_edges"], d_env, out=r["d_order"])
+            ts.append((time.perf_counter() - t0) * 1e3)
+        order_ms[r["robot"]] = float(np.median(ts))
+    for r in runs:
+        r["d_out_l"] = bcc.DeviceBuffer(max(1, r["edges"].shape[0]))
+        r["plan_l"] = bcc.RaggedPlan(r["robot"], r["scene"], r["off"], r["d_edges"], r["d_out_l"],
+                                     d_order=r["d_order"])
+    group = bcc.RaggedGroup([r["plan_l"] for r in runs])
+    check_ms = group.time(a.warmup, a.steps)
+    mismatches = 0
+    for r in runs:
+        mismatches += int((r["d_out_l"].to_array(np.uint8, (r["edges"].shape[0],)) != r["v"]).sum())
+    group.close()
+    for r in runs:
+        r["plan_l"].close()
+        r["model"].close()
+    total_order = sum(order_ms.values())
+    print(f"[bench] learned step: order {total_order:.2f} ms ("
+          + ", ".join(f"{k} {v:.2f}" for k, v in order_ms.items())
+          + f") + ordered group {check_ms:.3f} ms (plain group {plain_ms:.3f} ms); verdict mismatches "
+          f"{mismatches}", file=sys.stderr, flush=True)
+    if mismatches:
+        print("[bench] learned-ordered verdicts differ from plain order", file=sys.stderr)
+        sys.exit(1)
+    flops = sum(mlp_flops(r["robot"]) * r["edges"].shape[0] for r in runs)
+    mfma = flops / (total_order / 1e3) / 1e12
+    return {
+        "ms_per_step": total_order + check_ms,
+        "order_ms": total_order,
+        "order_ms_per_robot": order_ms,
+        "inference_mfma": {"achieved": mfma, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": mfma / BF16_PEAK_TFLOPS,
+                           "note": "wall time of the inference calls (fragment packing, encoder, fused kernel)"},
+        "ordered_check_ms": check_ms,
+        "speedup_vs_plain": plain_ms / (total_order + check_ms),
+        "check_only_speedup_vs_plain": plain_ms / check_ms,
+        "verdict_mismatches_vs_plain": mismatches,
+        "train": {"pairs_per_robot": n, "epochs": a.train_epochs,
+                  "loss_first_epoch": {r["robot"]: float(r["losses"][0]) for r in runs},
+                  "loss_last_epoch": {r["robot"]: float(r["losses"][-1]) for r in runs},
+                  "edges": "separate seeded lognormal edge stream per robot, same environments"},
     }
 
 
@@ -208,6 +299,7 @@ def bench_mixed(bcc, W, a) -> dict:
             sys.exit(1)
     print(f"[bench] {a.steps} timed group launches: {ms:.3f} ms each; plans alone: "
           + ", ".join(f"{k} {v:.3f} ms" for k, v in alone.items()), file=sys.stderr, flush=True)
+    learned = learned_mixed(bcc, W, a, runs, ms) if a.learned else None
     roofline = None
     if not a.no_work:
         flops = trigs = 0.0
@@ -256,6 +348,9 @@ def bench_mixed(bcc, W, a) -> dict:
                                               "groups_per_cu": r["plan"].config[1]} for r in runs}},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if learned is not None:
+        learned["value"] = total / (learned["ms_per_step"] / 1e3)
+        rec["learned"] = learned
     group.close()
     for r in runs:
         r["plan"].close()
@@ -306,6 +401,8 @@ def main():
                "steps": a.steps, "warmup": a.warmup, "ms_per_step": part["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                "config": part["config"], "roofline": part["roofline"], "cpu_baseline": part["cpu_baseline"]}
+        if "learned" in part:
+            rec["learned"] = part["learned"]
         print(json.dumps(rec))
         return
 

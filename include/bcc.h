@@ -165,6 +165,14 @@ int bcc_ragged_plan_create(int robot, const bcc_scene *scene, const int64_t *env
 int bcc_ragged_plan_create_tuned(int robot, const bcc_scene *scene, const int64_t *env_offsets,
                                  const float *d_edges, int resolution, uint8_t *d_results, int tile_pairs,
                                  int groups_per_cu, bcc_ragged_plan **out);
+/* Same, with a learned visiting order per pair: d_order [total pairs] on the
+ * device, uint32 as in bcc_check_dense_ex (d_order[p] for pair p; any value
+ * that does not name four distinct bins below 32 means plain order).  The
+ * plan reads d_order at every launch.  Verdicts equal the plain plan's.  No
+ * reference counterpart (learned ordering). */
+int bcc_ragged_plan_create_ordered(int robot, const bcc_scene *scene, const int64_t *env_offsets,
+                                   const float *d_edges, int resolution, const uint32_t *d_order,
+                                   uint8_t *d_results, int tile_pairs, int groups_per_cu, bcc_ragged_plan **out);
 int bcc_ragged_plan_launch(bcc_ragged_plan *plan, void *stream);        /* graph replay */
 int bcc_ragged_plan_launch_eager(bcc_ragged_plan *plan, void *stream);  /* same kernels, no graph */
 int bcc_ragged_plan_num_tiles(const bcc_ragged_plan *plan);
@@ -243,6 +251,12 @@ int bcc_model_order_dense(bcc_model *model, const bcc_scene *scene, const float 
 /* the same, also writing every pair's 32 logits into d_logits [num_edges][num_envs][32] */
 int bcc_model_logits_dense(bcc_model *model, const bcc_scene *scene, const float *d_edges, int num_edges,
                            uint32_t *d_order, float *d_logits, void *stream);
+/* the same for a list of pairs (d_pair_env[p], d_pair_edge[p]), e.g. a ragged
+ * batch (d_pair_edge = NULL: pair p uses edge p); environment and edge indices
+ * must be in range.  d_order [num_pairs]; d_logits [num_pairs][32] or NULL. */
+int bcc_model_order_pairs(bcc_model *model, const bcc_scene *scene, const float *d_edges, int64_t num_pairs,
+                          const int32_t *d_pair_env, const int32_t *d_pair_edge, uint32_t *d_order,
+                          float *d_logits, void *stream);
 /* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
 int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,
                    int m, int n, int k, float alpha, float beta);

@@ -110,6 +110,11 @@ def test_ragged_plan_and_group_reject_bad_arguments(bcc):
     arr = (ctypes.c_void_p * 2)(None, None)
     assert L.bcc_ragged_group_create(arr, 2, ctypes.byref(h)) == 1
     assert b"is NULL" in L.bcc_last_error()
+    assert L.bcc_ragged_plan_create_ordered(0, None, off.ctypes.data, None, 32, None, None, 0, 0,
+                                            ctypes.byref(h)) == 1
+    assert b"scene is NULL" in L.bcc_last_error()
+    assert L.bcc_model_order_pairs(None, None, None, 1, None, None, None, None, None) == 1
+    assert b"bad arguments" in L.bcc_last_error()
     assert L.bcc_ragged_plan_config(None, None, None) == 1
     assert L.bcc_ragged_group_launch(None, None) == 1
     assert L.bcc_ragged_group_destroy(None) == 0
