@@ -114,6 +114,7 @@ def lib():
     _sig(L, "bcc_debug_prims", I, [P, I, P])
     _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
     _sig(L, "bcc_debug_force_variant", I, [I])
+    _sig(L, "bcc_debug_cull_min", I, [I])
     _lib = L
     return L
 
@@ -422,6 +423,12 @@ class RaggedGroup:
 def debug_force_variant(variant: int):
     """-1: automatic kernel variant per launch; 0 / 1: force low / high occupancy."""
     _check(lib().bcc_debug_force_variant(int(variant)))
+
+
+def debug_cull_min(work: int):
+    """Culling threshold (spheres per sweep x primitives per environment) for
+    scenes created afterwards; -1 restores the tuned value."""
+    _check(lib().bcc_debug_cull_min(int(work)))
 
 
 def debug_fk(robot, q: np.ndarray) -> np.ndarray:

@@ -66,10 +66,149 @@ namespace {
 // (z-cuboids with axes (1,0),(0,1)) become aabbs: for those inputs the
 // special-case formulas are bit-identical to the general ones.  Cylinders
 // are dropped: the reference never tests them (src/utils.cuh:213-245).
+//
+// A seventh list holds one bounding box per tested primitive, in list order
+// ([lo.x lo.y lo.z 0 hi.x hi.y hi.z 0]), for the kernels' exact culling
+// (bcc_device.h): every point where the primitive's test value can be
+// negative for a robot sphere of padded radius rp lies within rp of the box.
+// Boxes follow the formula each list uses, are computed in double, widened
+// by 1e-5 (|centre| + size) and rounded outwards; primitives whose formula
+// is not a Euclidean distance (cuboid axes not orthonormal to 4e-6) get an
+// infinite box, i.e. are never culled.
+namespace {
+
+struct Box {
+    double lo[3], hi[3];
+};
+
+void push_box(std::vector<float>& out, const Box& b, double scale)
+{
+    const double pad = 1e-5 * scale;
+    float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        v[k] = std::nextafter((float)(b.lo[k] - pad), -INFINITY);
+        v[4 + k] = std::nextafter((float)(b.hi[k] + pad), INFINITY);
+    }
+    out.insert(out.end(), v, v + 8);
+}
+
+void push_infinite_box(std::vector<float>& out)
+{
+    const float v[8] = {-INFINITY, -INFINITY, -INFINITY, 0.0f, INFINITY, INFINITY, INFINITY, 0.0f};
+    out.insert(out.end(), v, v + 8);
+}
+
+// r[0..2], r[3..5] (, r[6..8]): rows of an axis matrix; |R R^T - I| <= 4e-6
+bool orthonormal(const float* a, const float* b, const float* c, int n)
+{
+    const float* r[3] = {a, b, c};
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double d = 0.0;
+            for (int k = 0; k < n; ++k)
+                d += (double)r[i][k] * (double)r[j][k];
+            if (!(std::fabs(d - (i == j ? 1.0 : 0.0)) <= 4e-6))
+                return false;
+        }
+    return true;
+}
+
+// list k record -> box (see the list layouts in bcc_device.h)
+void record_box(int k, const float* o, std::vector<float>& out)
+{
+    Box b;
+    double scale = 0.0;
+    switch (k) {
+    case 0: {   // sphere x y z r
+        const double r = std::fabs((double)o[3]);
+        for (int i = 0; i < 3; ++i) {
+            b.lo[i] = (double)o[i] - r;
+            b.hi[i] = (double)o[i] + r;
+            scale += std::fabs((double)o[i]);
+        }
+        scale += r;
+        break;
+    }
+    case 1: {   // capsule: segment p1 .. p1 + v, radius r
+        const double r = std::fabs((double)o[6]);
+        for (int i = 0; i < 3; ++i) {
+            const double p = o[i], q = (double)o[i] + (double)o[3 + i];
+            b.lo[i] = std::min(p, q) - r;
+            b.hi[i] = std::max(p, q) + r;
+            scale += std::fabs(p) + std::fabs((double)o[3 + i]);
+        }
+        scale += r;
+        break;
+    }
+    case 2: {   // z-capsule: segment (x1, y1, z1) .. (x1, y1, z1 + zv)
+        const double r = std::fabs((double)o[6]);
+        for (int i = 0; i < 2; ++i) {
+            b.lo[i] = (double)o[i] - r;
+            b.hi[i] = (double)o[i] + r;
+        }
+        const double p = o[2], q = (double)o[2] + (double)o[5];
+        b.lo[2] = std::min(p, q) - r;
+        b.hi[2] = std::max(p, q) + r;
+        scale = std::fabs((double)o[0]) + std::fabs((double)o[1]) + std::fabs(p) + std::fabs((double)o[5]) + r;
+        break;
+    }
+    case 3: {   // cuboid: centre, rows a1 a2 a3, half-extents
+        if (!orthonormal(o + 3, o + 6, o + 9, 3)) {
+            push_infinite_box(out);
+            return;
+        }
+        for (int i = 0; i < 3; ++i) {
+            double w = 0.0;
+            for (int a = 0; a < 3; ++a)
+                w += std::fabs((double)o[3 + 3 * a + i]) * std::fabs((double)o[12 + a]);
+            b.lo[i] = (double)o[i] - w;
+            b.hi[i] = (double)o[i] + w;
+            scale += std::fabs((double)o[i]) + std::fabs((double)o[12 + i]);
+        }
+        break;
+    }
+    case 4: {   // z-cuboid: 2-D rows (a1x, a1y), (a2x, a2y), half-extents; z is axis-aligned
+        if (!orthonormal(o + 3, o + 6, nullptr, 2)) {
+            push_infinite_box(out);
+            return;
+        }
+        for (int i = 0; i < 2; ++i) {
+            const double w = std::fabs((double)o[3 + i]) * std::fabs((double)o[12]) +
+                             std::fabs((double)o[6 + i]) * std::fabs((double)o[13]);
+            b.lo[i] = (double)o[i] - w;
+            b.hi[i] = (double)o[i] + w;
+        }
+        b.lo[2] = (double)o[2] - std::fabs((double)o[14]);
+        b.hi[2] = (double)o[2] + std::fabs((double)o[14]);
+        for (int i = 0; i < 3; ++i)
+            scale += std::fabs((double)o[i]) + std::fabs((double)o[12 + i]);
+        break;
+    }
+    default: {  // aabb: centre, half-extents
+        for (int i = 0; i < 3; ++i) {
+            const double h = std::fabs((double)o[3 + i]);
+            b.lo[i] = (double)o[i] - h;
+            b.hi[i] = (double)o[i] + h;
+            scale += std::fabs((double)o[i]) + h;
+        }
+        break;
+    }
+    }
+    push_box(out, b, scale);
+}
+
+}  // namespace
+
+// Culling threshold stamped into each environment's record at scene
+// creation: a sweep of N robot spheres culls when N * (primitives in the
+// environment) reaches it (below, the box reduction costs more than the tests
+// it saves).  -1: the tuning table's value.
+std::atomic<int> g_cull_min{-1};
+
 int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene** out)
 {
-    constexpr int kLists = 6;
-    const int stride[kLists] = {4, 8, 8, 16, 16, 8};
+    constexpr int kLists = 7;   // six primitive lists + their boxes
+    const int stride[kLists] = {4, 8, 8, 16, 16, 8, 8};
     std::vector<std::vector<float>> lists((size_t)num_envs * kLists);
     for (int e = 0; e < num_envs; ++e) {
         const bcc_env_desc& d = envs[e];
@@ -106,6 +245,9 @@ int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene**
                 L[4].push_back(0.0f);
             }
         }
+        for (int k = 0; k < 6; ++k)
+            for (size_t i = 0; i < L[k].size(); i += stride[k])
+                record_box(k, &L[k][i], L[6]);
     }
     std::vector<int> meta((size_t)num_envs * bcc::kEnvMetaInts, 0);
     size_t total = 0;
@@ -120,6 +262,8 @@ int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene**
             meta[(size_t)e * bcc::kEnvMetaInts + 2 * k + 1] = (int)(l.size() / stride[k]);
             total += l.size();
         }
+        const int cull_min = g_cull_min.load();
+        meta[(size_t)e * bcc::kEnvMetaInts + 14] = cull_min >= 0 ? cull_min : kCullMinWork;
     }
     total = align16(total) + 16;
     std::vector<float> host(total, 0.0f);
@@ -1073,6 +1217,12 @@ int bcc_check_ragged(int robot, const bcc_scene* scene, const int64_t* env_offse
     (void)hipFree(d_edges);
     (void)hipFree(d_out);
     return rc;
+}
+
+int bcc_debug_cull_min(int work)
+{
+    g_cull_min.store(work < 0 ? -1 : work);
+    return BCC_OK;
 }
 
 int bcc_debug_force_variant(int variant)

@@ -15,6 +15,7 @@
 namespace bcc {
 
 constexpr int kBlock = 256;
+static_assert(kBlock / 64 <= kMaxBlockWaves, "culling keeps one LDS slot row per wave");
 
 // Step count and per-joint increment of one edge, then the configuration of
 // lane t at step 0 (src/batch_cc.cu:103-116):

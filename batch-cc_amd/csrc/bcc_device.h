@@ -39,39 +39,65 @@ typedef const __attribute__((address_space(4))) int* ciptr;
 //   zcubs    [nzcb][16] z-aligned cuboids                   (src/utils.cuh:186-203)
 //   aabbs    [na][8]    x y z h1 h2 h3 pad pad: cuboids whose axes are exactly
 //                       the identity and z-cuboids with axes (1,0),(0,1)
+//   boxes    [nbox][8]  lo.x lo.y lo.z 0 hi.x hi.y hi.z 0: one bounding box
+//                       per primitive above, in list order (nbox = the sum
+//                       of the six counts); a primitive's test value is
+//                       negative only for sphere centres within rp of its box
 // Cylinders are never tested by the reference (src/utils.cuh:213-245) and are
 // not uploaded.
+// What a check program holds of its environment between sweeps: the
+// obstacle buffer and the environment's meta record (4 SGPRs).  Each sweep
+// reads the record again (bcc_env_lists), so the list pointers and counts do
+// not occupy SGPRs through forward kinematics and the self tests.
 struct EnvView {
+    cfptr obs;
+    ciptr meta;
+};
+
+// Per-environment record in the scene's meta array (int32): for list k in
+// the order above, [2k] = offset in floats, [2k+1] = count; [14] = the
+// culling threshold cull_min.
+constexpr int kEnvMetaInts = 16;
+
+__device__ __forceinline__ EnvView load_env(const float* obs_, const int* meta_, int env)
+{
+    return EnvView{(cfptr)obs_, (ciptr)meta_ + env * kEnvMetaInts};
+}
+
+struct EnvLists {
     cfptr sph;
     cfptr cap;
     cfptr zcap;
     cfptr cub;
     cfptr zcub;
     cfptr aabb;
-    int ns, nc, nzc, ncb, nzcb, na;
+    const float* box;   // read per lane (one box per lane), so a plain global pointer
+    int ns, nc, nzc, ncb, nzcb, na, nbox;
+    int cull_min;       // a sweep of N spheres culls when N * nbox >= cull_min
 };
 
-// Per-environment record in the scene's meta array (int32): for list k in
-// the order above, [2k] = offset in floats, [2k+1] = count.
-constexpr int kEnvMetaInts = 16;
-
-__device__ __forceinline__ EnvView load_env(const float* obs_, const int* meta_, int env)
+__device__ __forceinline__ EnvLists bcc_env_lists(const EnvView& E)
 {
-    const cfptr obs = (cfptr)obs_;
-    const ciptr m = (ciptr)meta_ + env * kEnvMetaInts;
-    EnvView v;
-    v.sph = obs + m[0];
+    ciptr m = E.meta;
+    // an opaque copy of the pointer: the compiler cannot merge this sweep's
+    // scalar loads with another sweep's and keep the values live in between
+    asm volatile("" : "+s"(m));
+    EnvLists v;
+    v.sph = E.obs + m[0];
     v.ns = m[1];
-    v.cap = obs + m[2];
+    v.cap = E.obs + m[2];
     v.nc = m[3];
-    v.zcap = obs + m[4];
+    v.zcap = E.obs + m[4];
     v.nzc = m[5];
-    v.cub = obs + m[6];
+    v.cub = E.obs + m[6];
     v.ncb = m[7];
-    v.zcub = obs + m[8];
+    v.zcub = E.obs + m[8];
     v.nzcb = m[9];
-    v.aabb = obs + m[10];
+    v.aabb = E.obs + m[10];
     v.na = m[11];
+    v.box = (const float*)(uintptr_t)(E.obs + m[12]);
+    v.nbox = m[13];
+    v.cull_min = m[14];
     return v;
 }
 
@@ -270,67 +296,315 @@ __device__ __forceinline__ bool bcc_ref_env_hit(const RefEnvView& E, float x, fl
     return h;
 }
 
-// ------------------------------------------------------------ env sweeps
-// One pass over the environment's obstacles tests N robot spheres at once
-// (N independent tests per obstacle keep the VALU pipeline full).  `rp` are
-// the padded radii float(double(r) + 0.005) (src/utils.cuh:215).
+// ------------------------------------------------------------ exact culling
+// A sweep tests N robot spheres of every lane in a lane mask against the
+// environment.  When N times the environment's primitive count reaches the
+// scene's threshold (below it the box reduction costs more than the tests it
+// saves), the wave first computes for each half-wave the box
+// of those spheres (centre -/+ padded radius over the masked lanes), widened
+// by delta = 0.01 + 1e-5 * (largest coordinate) -- both halves' boxes are
+// wave-uniform -- and then, 64 obstacles at a time, one obstacle per lane,
+// which obstacles' boxes come within the widened box of either half
+// (a ballot, kept in LDS).  Only those obstacles are tested.
+//
+// Why the verdicts cannot change: an obstacle is skipped only if, along some
+// axis, its box is separated from a half's sphere box by more than delta.
+// Then every masked sphere centre is farther than rp + delta from the box, so
+// the exact Euclidean distance to the primitive exceeds rp + delta, and the
+// test value (a squared distance minus (rp + R)^2, or for cuboids the
+// distance formula with orthonormal axes) is at least delta^2 > 0 in exact
+// arithmetic.  Its FP32 evaluation errs by a few ulp of the magnitudes
+// involved (|centre - primitive| and the primitive's size), which the
+// 1e-5-relative widenings on both sides exceed by orders of magnitude, and
+// which the absolute 0.01 m exceeds for any scene smaller than 10 km.  NaN
+// coordinates compare false and are never culled.  Lanes outside the mask
+// do not shape the boxes; their flags are not used by the caller.
+struct HalfBoxes {
+    float lo0[3], hi0[3], lo1[3], hi1[3];   // wave-uniform
+};
 
-// U: obstacle loop unroll factor, a tuning knob of each kernel variant (more
-// unrolling overlaps scalar loads with tests but keeps more temporaries live).
-#define BCC_SWEEP(count, stride, base, TEST)                    \
-    _Pragma("unroll U") for (int k = 0; k < (count); ++k) {     \
-        const cfptr o = (base) + (stride) * k;                  \
-        _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; } \
-        SETTLED_CHECK;                                          \
+__device__ __forceinline__ float bcc_wave_min32(float v)
+{
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1)
+        v = fminf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+__device__ __forceinline__ float bcc_wave_max32(float v)
+{
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1)
+        v = fmaxf(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+__device__ __forceinline__ float bcc_read_lane(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int N>
+__device__ __forceinline__ HalfBoxes bcc_half_boxes(const float (&x)[N], const float (&y)[N], const float (&z)[N],
+                                                    const float (&rp)[N], uint64_t mask)
+{
+    const bool in = (mask >> (threadIdx.x & 63)) & 1ull;
+    float lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = INFINITY;
+        hi[a] = -INFINITY;
+    }
+    if (in) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            lo[0] = fminf(lo[0], x[i] - rp[i]);
+            lo[1] = fminf(lo[1], y[i] - rp[i]);
+            lo[2] = fminf(lo[2], z[i] - rp[i]);
+            hi[0] = fmaxf(hi[0], x[i] + rp[i]);
+            hi[1] = fmaxf(hi[1], y[i] + rp[i]);
+            hi[2] = fmaxf(hi[2], z[i] + rp[i]);
+        }
+    }
+    float big = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = bcc_wave_min32(lo[a]);
+        hi[a] = bcc_wave_max32(hi[a]);
+        big = fmaxf(big, fmaxf(fabsf(lo[a]), fabsf(hi[a])));
+    }
+    if (lo[0] <= hi[0]) {   // a half with masked lanes: widen (an empty half stays empty)
+        const float delta = fmaf(big, 1e-5f, 0.01f);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            lo[a] -= delta;
+            hi[a] += delta;
+        }
+    }
+    HalfBoxes W;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        W.lo0[a] = bcc_read_lane(lo[a], 0);
+        W.hi0[a] = bcc_read_lane(hi[a], 0);
+        W.lo1[a] = bcc_read_lane(lo[a], 32);
+        W.hi1[a] = bcc_read_lane(hi[a], 32);
+    }
+    return W;
+}
+
+// Bit j: obstacle base + j (in box order) may touch a masked sphere.
+__device__ __forceinline__ uint64_t bcc_near_mask(const EnvLists& E, int base, const HalfBoxes& W)
+{
+    const int j = base + (int)(threadIdx.x & 63);
+    bool near = false;
+    if (j < E.nbox) {
+        const float4 l = *(const float4*)(E.box + 8 * (size_t)j);
+        const float4 h = *(const float4*)(E.box + 8 * (size_t)j + 4);
+        const bool far0 = h.x < W.lo0[0] || h.y < W.lo0[1] || h.z < W.lo0[2] || l.x > W.hi0[0] ||
+                          l.y > W.hi0[1] || l.z > W.hi0[2];
+        const bool far1 = h.x < W.lo1[0] || h.y < W.lo1[1] || h.z < W.lo1[2] || l.x > W.hi1[0] ||
+                          l.y > W.hi1[1] || l.z > W.hi1[2];
+        near = !(far0 && far1);
+    }
+    return __ballot(near);
+}
+
+// Near masks of a sweep's first kCullWindows 64-primitive windows, one LDS
+// slot per window and wave (the kernels run at most kMaxBlockWaves waves per
+// workgroup; sweeps of one wave never overlap).  Keeping the masks in LDS
+// instead of the half-wave boxes in SGPRs frees those registers before the
+// tests run.  Primitives beyond the covered windows are always tested.
+constexpr int kCullWindows = 16;
+constexpr int kMaxBlockWaves = 4;
+
+__device__ __forceinline__ uint64_t* bcc_near_slots()
+{
+    __shared__ uint64_t slots[kMaxBlockWaves * kCullWindows];
+    return slots + (threadIdx.x >> 6) * kCullWindows;
+}
+
+template <int N>
+__device__ __forceinline__ void bcc_cull(const EnvLists& E, const float (&x)[N], const float (&y)[N],
+                                         const float (&z)[N], const float (&rp)[N], uint64_t mask)
+{
+    const HalfBoxes W = bcc_half_boxes<N>(x, y, z, rp, mask);
+    uint64_t* slots = bcc_near_slots();
+    const int nwin = min((E.nbox + 63) >> 6, kCullWindows);
+    for (int w = 0; w < nwin; ++w) {
+        const uint64_t m = bcc_near_mask(E, 64 * w, W);
+        if ((threadIdx.x & 63) == 0)
+            slots[w] = m;
+    }
+}
+
+__device__ __forceinline__ uint64_t bcc_window_near(int b0)
+{
+    if (b0 >= 64 * kCullWindows)
+        return ~0ull;
+    const uint64_t m = bcc_near_slots()[b0 >> 6];
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)m);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(m >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Bits of m for box indices [first, first + count) within the 64-obstacle
+// window starting at base.
+__device__ __forceinline__ uint64_t bcc_window_bits(uint64_t m, int base, int first, int count)
+{
+    const int lo = max(first - base, 0);
+    const int hi = min(first + count - base, 64);
+    if (lo >= hi)
+        return 0;
+    const uint64_t width = (hi - lo == 64) ? ~0ull : ((1ull << (hi - lo)) - 1ull);
+    return m & (width << lo);
+}
+
+// ------------------------------------------------------------ env sweeps
+// One pass over the environment's obstacles (all of them below the culling
+// threshold, else the near ones) tests N robot spheres at once (N
+// independent tests per obstacle keep the VALU pipeline full).  `rp` are the
+// padded radii float(double(r) + 0.005) (src/utils.cuh:215).
+
+// U: unroll factor of the obstacle loops, a tuning knob of each kernel
+// variant.  The culled loop visits the set bits of the window's mask for one
+// list.
+#define BCC_SWEEP(first, count, stride, base, TEST)                              \
+    {                                                                             \
+        uint64_t mk = bcc_window_bits(near, b0, (first), (count));                \
+        _Pragma("unroll U") for (; mk; mk &= mk - 1) {                            \
+            const int k = b0 + (int)__builtin_ctzll(mk) - (first);                \
+            const cfptr o = (base) + (stride) * k;                                \
+            _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; }               \
+            SETTLED_CHECK;                                                        \
+        }                                                                         \
     }
 
-// Per-lane, per-sphere hit flags; no early exit (gate spheres and lone
-// spheres: each lane needs its own answer for every sphere).
-template <int N, int U>
-__device__ __forceinline__ void bcc_env_sweep(const EnvView& E, const float (&x)[N], const float (&y)[N],
-                                              const float (&z)[N], const float (&rp)[N], bool (&h)[N])
+// Every obstacle of one list in order (sweeps below the culling threshold).
+#define BCC_SWEEP_ALL(count, stride, base, TEST)                                 \
+    _Pragma("unroll U") for (int k = 0; k < (count); ++k) {                       \
+        const cfptr o = (base) + (stride) * k;                                    \
+        _Pragma("unroll") for (int i = 0; i < N; ++i) { TEST; }                   \
+        SETTLED_CHECK;                                                            \
+    }
+
+#define BCC_SWEEP_LISTS(T_SPH, T_CAP, T_ZCAP, T_CUB, T_ZCUB, T_AABB)              \
+    {                                                                             \
+        BCC_SWEEP_ALL(E.ns, 4, E.sph, T_SPH)                                      \
+        BCC_SWEEP_ALL(E.nc, 8, E.cap, T_CAP)                                      \
+        BCC_SWEEP_ALL(E.nzc, 8, E.zcap, T_ZCAP)                                   \
+        BCC_SWEEP_ALL(E.ncb, 16, E.cub, T_CUB)                                    \
+        BCC_SWEEP_ALL(E.nzcb, 16, E.zcub, T_ZCUB)                                 \
+        BCC_SWEEP_ALL(E.na, 8, E.aabb, T_AABB)                                    \
+    }
+
+// The six lists of one 64-obstacle window.
+#define BCC_SWEEP_WINDOW(T_SPH, T_CAP, T_ZCAP, T_CUB, T_ZCUB, T_AABB)             \
+    {                                                                             \
+        const int f1 = E.ns, f2 = f1 + E.nc, f3 = f2 + E.nzc, f4 = f3 + E.ncb,    \
+                  f5 = f4 + E.nzcb;                                               \
+        BCC_SWEEP(0, E.ns, 4, E.sph, T_SPH)                                       \
+        BCC_SWEEP(f1, E.nc, 8, E.cap, T_CAP)                                      \
+        BCC_SWEEP(f2, E.nzc, 8, E.zcap, T_ZCAP)                                   \
+        BCC_SWEEP(f3, E.ncb, 16, E.cub, T_CUB)                                    \
+        BCC_SWEEP(f4, E.nzcb, 16, E.zcub, T_ZCUB)                                 \
+        BCC_SWEEP(f5, E.na, 8, E.aabb, T_AABB)                                    \
+    }
+
+// Per-lane, per-sphere hit flags for the lanes in `live` (other lanes' flags
+// are meaningless); no early exit (gate spheres and lone spheres: each lane
+// needs its own answer for every sphere).
+//
+// kDual: sweeps below the culling threshold run the plain counted loops (a
+// second copy of the code); otherwise the culled loop with every obstacle
+// marked near.
+template <int N, int U, bool kDual>
+__device__ __forceinline__ void bcc_env_sweep(const EnvView& env, const float (&x)[N], const float (&y)[N],
+                                              const float (&z)[N], const float (&rp)[N], uint64_t live, bool (&h)[N])
 {
+    const EnvLists E = bcc_env_lists(env);
     float rsq[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         h[i] = false;
         rsq[i] = rp[i] * rp[i];
     }
+    if (E.nbox == 0)
+        return;
 #define SETTLED_CHECK
-    BCC_SWEEP(E.ns, 4, E.sph, h[i] = h[i] | (bcc_sphere_val(o[0], o[1], o[2], o[3], x[i], y[i], z[i], rp[i]) < 0.0f))
-    BCC_SWEEP(E.nc, 8, E.cap, h[i] = h[i] | (bcc_capsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f))
-    BCC_SWEEP(E.nzc, 8, E.zcap, h[i] = h[i] | (bcc_zcapsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f))
-    BCC_SWEEP(E.ncb, 16, E.cub, h[i] = h[i] | (bcc_cuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f))
-    BCC_SWEEP(E.nzcb, 16, E.zcub, h[i] = h[i] | (bcc_zcuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f))
-    BCC_SWEEP(E.na, 8, E.aabb, h[i] = h[i] | (bcc_aabb_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f))
+#define T_SPH h[i] = h[i] | (bcc_sphere_val(o[0], o[1], o[2], o[3], x[i], y[i], z[i], rp[i]) < 0.0f)
+#define T_CAP h[i] = h[i] | (bcc_capsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f)
+#define T_ZCAP h[i] = h[i] | (bcc_zcapsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f)
+#define T_CUB h[i] = h[i] | (bcc_cuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f)
+#define T_ZCUB h[i] = h[i] | (bcc_zcuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f)
+#define T_AABB h[i] = h[i] | (bcc_aabb_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f)
+    const bool cull = N * E.nbox >= E.cull_min;
+    if (kDual && !cull) {
+        BCC_SWEEP_LISTS(T_SPH, T_CAP, T_ZCAP, T_CUB, T_ZCUB, T_AABB)
+        return;
+    }
+    if (cull)
+        bcc_cull<N>(E, x, y, z, rp, live);
+    for (int b0 = 0; b0 < E.nbox; b0 += 64) {
+        const uint64_t near = cull ? bcc_window_near(b0) : ~0ull;
+        BCC_SWEEP_WINDOW(T_SPH, T_CAP, T_ZCAP, T_CUB, T_ZCUB, T_AABB)
+    }
+#undef T_SPH
+#undef T_CAP
+#undef T_ZCAP
+#undef T_CUB
+#undef T_ZCUB
+#undef T_AABB
 #undef SETTLED_CHECK
 }
 
 // Does any of the N spheres touch any obstacle?  Per-lane OR.  Only lanes in
 // `need` matter; the sweep stops once every half-wave present in `need` has a
 // hit (one colliding sample decides that half's pair).
-template <int N, int U>
-__device__ __forceinline__ bool bcc_env_any(const EnvView& E, const float (&x)[N], const float (&y)[N],
+template <int N, int U, bool kDual>
+__device__ __forceinline__ bool bcc_env_any(const EnvView& env, const float (&x)[N], const float (&y)[N],
                                             const float (&z)[N], const float (&rp)[N], uint64_t need)
 {
+    const EnvLists E = bcc_env_lists(env);
     float rsq[N];
 #pragma unroll
     for (int i = 0; i < N; ++i)
         rsq[i] = rp[i] * rp[i];
     bool h = false;
+    if (E.nbox == 0)
+        return h;
 #define SETTLED_CHECK                                                         \
     if ((need & ~bcc_half_spread(__ballot(h) & need)) == 0)                   \
         return h;
-    BCC_SWEEP(E.ns, 4, E.sph, h = h | (bcc_sphere_val(o[0], o[1], o[2], o[3], x[i], y[i], z[i], rp[i]) < 0.0f))
-    BCC_SWEEP(E.nc, 8, E.cap, h = h | (bcc_capsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f))
-    BCC_SWEEP(E.nzc, 8, E.zcap, h = h | (bcc_zcapsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f))
-    BCC_SWEEP(E.ncb, 16, E.cub, h = h | (bcc_cuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f))
-    BCC_SWEEP(E.nzcb, 16, E.zcub, h = h | (bcc_zcuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f))
-    BCC_SWEEP(E.na, 8, E.aabb, h = h | (bcc_aabb_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f))
+#define T_SPH h = h | (bcc_sphere_val(o[0], o[1], o[2], o[3], x[i], y[i], z[i], rp[i]) < 0.0f)
+#define T_CAP h = h | (bcc_capsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f)
+#define T_ZCAP h = h | (bcc_zcapsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f)
+#define T_CUB h = h | (bcc_cuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f)
+#define T_ZCUB h = h | (bcc_zcuboid_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f)
+#define T_AABB h = h | (bcc_aabb_val(o, x[i], y[i], z[i], rsq[i]) < 0.0f)
+    const bool cull = N * E.nbox >= E.cull_min;
+    if (kDual && !cull) {
+        BCC_SWEEP_LISTS(T_SPH, T_CAP, T_ZCAP, T_CUB, T_ZCUB, T_AABB)
+        return h;
+    }
+    if (cull)
+        bcc_cull<N>(E, x, y, z, rp, need);
+    for (int b0 = 0; b0 < E.nbox; b0 += 64) {
+        const uint64_t near = cull ? bcc_window_near(b0) : ~0ull;
+        BCC_SWEEP_WINDOW(T_SPH, T_CAP, T_ZCAP, T_CUB, T_ZCUB, T_AABB)
+    }
+#undef T_SPH
+#undef T_CAP
+#undef T_ZCAP
+#undef T_CUB
+#undef T_ZCUB
+#undef T_AABB
 #undef SETTLED_CHECK
     return h;
 }
 
+#undef BCC_SWEEP_LISTS
+#undef BCC_SWEEP_ALL
+#undef BCC_SWEEP_WINDOW
 #undef BCC_SWEEP
 
 }  // namespace bcc

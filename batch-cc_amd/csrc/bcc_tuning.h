@@ -1,6 +1,7 @@
-// Ragged-batch launch table: per robot, the persistent grid (workgroups per
-// CU) and the target number of tiles per wave, from which a plan sizes its
-// tiles; and the kernel-variant thresholds.  Written by batch-cc_amd/tune.py.
+// Launch table: the culling threshold, per robot the kernel-variant
+// threshold and the ragged launch shape (persistent workgroups per CU and
+// the target number of tiles per wave, from which a plan sizes its tiles).
+// Written by batch-cc_amd/tune.py.
 #pragma once
 
 struct bcc_tune_entry {
@@ -9,14 +10,18 @@ struct bcc_tune_entry {
     int groups_per_cu;    // persistent workgroups per CU
 };
 
+// Exact culling of distant obstacles: a sweep of N robot spheres culls when
+// N * (primitives in the environment) is at least this.
+static const int kCullMinWork = 32;
+
 // Kernel variant per launch: the high-occupancy variant when the mean number
 // of tested obstacles per environment (weighted by pairs) is at least this,
 // per robot (panda, fetch, baxter).
-static const int kHiVariantMinObstacles[3] = {32, 16, 32};
+static const int kHiVariantMinObstacles[3] = {1073741824, 7, 1073741824};
 
 // source: tune.py on gfx950, 2026-12-04, seed 0
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 16, 6},   // panda: score 1.003
-    {1, 8, 8},   // fetch: score 1.016
-    {2, 16, 6},   // baxter: score 1.001
+    {0, 16, 4},   // panda: score 1.010
+    {1, 4, 8},   // fetch: score 1.009
+    {2, 16, 4},   // baxter: score 1.000
 };

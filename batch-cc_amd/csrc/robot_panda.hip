@@ -10,7 +10,7 @@ namespace bcc {
 // Kernel variants (waves per SIMD, obstacle-loop unroll), measured on MI355X:
 // high occupancy for scenes with many obstacles per environment, low for few.
 #ifndef BCC_PANDA_VARIANTS
-#define BCC_PANDA_VARIANTS 6, 1, 4, 1
+#define BCC_PANDA_VARIANTS 5, 1, 4, 1
 #endif
 #define BCC_ROBOT panda
 #define BCC_ROBOT_T Robot_panda

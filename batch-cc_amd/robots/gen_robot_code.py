@@ -226,6 +226,12 @@ def padded_radius(r: float) -> float:
 # one sweep.  Measured on MI355X over the benchmark workloads (gate 1..4,
 # fine 2..8, first chunk 1 or the gate chunk).
 CHUNKS = {"panda": (2, 2, 4), "fetch": (1, 2, 4), "baxter": (1, 2, 4)}
+# Sweeps below the culling threshold run plain counted loops over all
+# obstacles (a second copy of each sweep's code) or the culled loop with every
+# obstacle marked near.  The second copy pays for panda and fetch; for
+# baxter's much longer check program the extra code costs more in
+# instruction-cache misses than the plain loops save (MI355X, c2 and c4).
+DUAL_SWEEP = {"panda": True, "fetch": True, "baxter": False}
 GATE_CHUNK = 2      # defaults for robots not listed; set per robot by gen_check
 FINE_CHUNK = 4
 FIRST_CHUNK = 0     # size of the first environment chunk (0: GATE_CHUNK)
@@ -341,7 +347,7 @@ def gen_check(model: dict, order: str | None = None) -> str:
         em.line(f"const float gz[{ng}] = {{{', '.join(gz)}}};")
         em.line(f"const float grp[{ng}] = {{{', '.join(grp)}}};")
         em.line(f"bool gh[{ng}];")
-        em.line(f"bcc_env_sweep<{ng}, U>(env, gx, gy, gz, grp, gh);")
+        em.line(f"bcc_env_sweep<{ng}, U, kDualSweep>(env, gx, gy, gz, grp, live, gh);")
         for k, gi in enumerate(chunk):
             g = envp[gi]
             em.line(f"// env group {gi}" + (f" ({g['label']})" if g["label"] else ""))
@@ -365,7 +371,7 @@ def gen_check(model: dict, order: str | None = None) -> str:
                 em.line(f"const float fy[{nf}] = {{{', '.join(fy)}}};")
                 em.line(f"const float fz[{nf}] = {{{', '.join(fz)}}};")
                 em.line(f"const float frp[{nf}] = {{{', '.join(frp)}}};")
-                em.line(f"const bool h = bcc_env_any<{nf}, U>(env, fx, fy, fz, frp, need);")
+                em.line(f"const bool h = bcc_env_any<{nf}, U, kDualSweep>(env, fx, fy, fz, frp, need);")
                 em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
                 em.line("hit |= d; live &= ~d; need &= ~d;")
                 em.indent -= 1
@@ -448,6 +454,8 @@ def gen_check(model: dict, order: str | None = None) -> str:
         f"struct Robot_{robot} {{",
         f"    static constexpr int dim = {dim};",
         f"    static constexpr int num_spheres = {len(sph)};",
+        "    // sweeps below the culling threshold get their own plain loops",
+        f"    static constexpr bool kDualSweep = {'true' if DUAL_SWEEP.get(robot, True) else 'false'};",
         "    // Collision predicate for one sample per lane.  `live` holds the lanes whose",
         "    // half-wave pair is still undecided; returns the half-spread mask of lanes",
         "    // whose half found a colliding sample (each collision decides its half).",

@@ -1,17 +1,24 @@
-"""Autotune the ragged-batch launch table on the GPU and write csrc/bcc_tuning.h.
+"""Autotune the launch table on the GPU and write csrc/bcc_tuning.h.
 
-Then, per robot, dense batches with 4 .. 128 obstacles per environment (the
-benchmark scenes' mixed obstacles spread over the workspace) time
-the two compiled kernel variants (high and low occupancy) against each other;
-the obstacle count from which the high-occupancy variant wins becomes the
-robot's variant threshold.
+Three settings, in this order:
 
-For every robot, three seeded ragged batches (long pairs, few environments,
-short pairs: see batch()) are timed as replayed hipGraphs for every
-candidate (target tiles per wave, workgroups per CU); verdicts must be
-identical for every candidate.  The candidate with the lowest mean of
-time / best time over the three batches goes into the table; the library
-reads it when a plan is created with tile_pairs = 0 / groups_per_cu = 0.
+1. The culling threshold (one value for all robots): per robot, three
+   benchmark-shaped batches (see shaped_batches) are timed with scenes built
+   under each candidate threshold; the candidate with the lowest sum of
+   time / best time wins.
+2. Per robot, the kernel-variant threshold: the same batches are timed with
+   the high- and the low-occupancy variant forced; the threshold (a mean
+   obstacle count per environment from which a launch takes the high
+   variant) is the candidate with the lowest sum of time / best time when
+   each batch runs the variant the threshold picks for it.
+3. Per robot, the ragged launch shape: three seeded ragged batches (long
+   pairs, few environments, short pairs: see batch()) are timed as replayed
+   hipGraphs for every candidate (target tiles per wave, workgroups per CU);
+   the candidate with the lowest mean of time / best time goes into the table.
+
+Every timing run must reproduce the first run's verdicts.  The library reads
+the table when a plan is created with tile_pairs = 0 / groups_per_cu = 0 and
+when a scene or launch is set up.
 
     python batch-cc_amd/tune.py [--out batch-cc_amd/csrc/bcc_tuning.h] [--json timings.json] [--quick]
 
@@ -75,58 +82,135 @@ def batch(W, robot: str, kind: str, pairs: int, seed: int):
     return PackedScene.from_envs(envs), off, edges
 
 
-OBSTACLE_COUNTS = (4, 8, 16, 32, 64, 128)
+CULL_CANDIDATES = (16, 24, 32, 40, 48, 64)
+NEVER = 1 << 30
 
 
-def variant_threshold(bcc, W, robot: str, seed: int, quick: bool):
-    """Smallest mean obstacle count from which the high-occupancy kernel
-    variant is at least as fast as the low one on dense batches of the
-    benchmark scenes' kind: for each count, 256 environments of that many mixed
-    obstacles spread over the workspace, once with 4000 lognormal edges (as in
-    c1) and once with 2000 uniform edges (as in c0/c3) -- large enough that the
-    launch tail does not decide; the two times add up.
-    Returns (threshold, timings)."""
+def shaped_batches(W, robot: str, seed: int, quick: bool):
+    """Benchmark-shaped batches of one robot at reduced size, as
+    (name, PackedScene, env_off or None (dense), edges, mean obstacles per
+    environment weighted by pairs): the robot's dense or ragged benchmark
+    workload (panda: 32 obstacles of c0's kinds and uniform edges; fetch:
+    U{4..64} mixed obstacles and lognormal edges; baxter: c2's Zipf ragged
+    batch), a few-obstacle dense batch (7 per environment, like the
+    MotionBenchMaker shelves), and the robot's part of the mixed batch
+    (U{32..512} obstacles)."""
     from batch_cc_amd import PackedScene
-    rs = np.random.default_rng([seed, 12])
-    rows, thr = [], None
-    for n_obs in OBSTACLE_COUNTS:
-        envs = [W.mixed_env(rs, n_obs) for _ in range(256)]
-        sc = PackedScene.from_envs(envs)
-        ms = {0: 0.0, 1: 0.0}
-        for edges in (W.lognormal_edges(robot, 1000 if quick else 4000, rs),
-                      W.uniform_edges(robot, 500 if quick else 2000, rs)):
-            with bcc.Scene(sc) as scene:
-                de = bcc.DeviceBuffer.from_array(edges)
-                do = bcc.DeviceBuffer(edges.shape[0] * sc.num_envs)
-                ref = None
-                for v in (0, 1):
-                    bcc.debug_force_variant(v)
-                    ms[v] += bcc.time_dense(robot, scene, de, edges.shape[0], 32, do, 2, 5)
-                    got = do.to_array(np.uint8, (edges.shape[0], sc.num_envs))
+    rs = np.random.default_rng([seed, 13, ROBOTS.index(robot)])
+    f = 0.25 if quick else 1.0
+    out = []
+    if robot == "baxter":
+        _, sc, off, edges = W.make_ragged("c2", seed=seed, scale=0.25 * f)
+        out.append(("c2-shaped", sc, off, edges))
+    elif robot == "fetch":
+        _, sc, edges = W.make_config("c1", seed=seed, scale=0.3 * f)
+        out.append(("c1-shaped", sc, None, edges))
+    else:
+        _, sc, edges = W.make_config("c3", seed=seed, scale=0.1 * f)
+        out.append(("c3-shaped", sc, None, edges))
+    envs = [W.c0_env(rs, n_cub=4, n_sph=2, n_cyl=1) for _ in range(100)]
+    edges = W.uniform_edges(robot, int(4000 * f), rs)
+    out.append(("few-obstacles", PackedScene.from_envs(envs), None, edges))
+    parts, _, _ = W.make_mixed(seed=seed, scale=0.1 * f)
+    for r, sc, off, edges, _ in parts:
+        if r == robot:
+            out.append(("mixed-part", sc, off, edges))
+    res = []
+    for name, sc, off, edges in out:
+        obs = sc.counts[:, [0, 1, 2, 4, 5]].sum(1).astype(np.float64)   # cylinders are not tested
+        w = np.diff(off).astype(np.float64) if off is not None else np.ones(sc.num_envs)
+        res.append((name, sc, off, edges, float((obs * w).sum() / max(w.sum(), 1.0))))
+    return res
+
+
+def time_batch(bcc, robot: str, sc, off, edges, iters: int = 5):
+    """(ms, verdicts) of one batch on a freshly built scene (culling
+    threshold and forced variant as currently set)."""
+    with bcc.Scene(sc) as scene:
+        de = bcc.DeviceBuffer.from_array(edges)
+        if off is None:
+            M = edges.shape[0]
+            do = bcc.DeviceBuffer(M * sc.num_envs)
+            ms = bcc.time_dense(robot, scene, de, M, 32, do, 2, iters)
+            return ms, do.to_array(np.uint8, (M, sc.num_envs))
+        do = bcc.DeviceBuffer(edges.shape[0])
+        plan = bcc.RaggedPlan(robot, scene, off, de, do)
+        ms = plan.time(1, iters)
+        v = do.to_array(np.uint8, (edges.shape[0],))
+        plan.close()
+        return ms, v
+
+
+def tune_cull(bcc, W, seed: int, quick: bool):
+    """Culling threshold with the lowest sum of time / best time over every
+    robot's shaped batches.  Returns (threshold, timings)."""
+    score = {c: 0.0 for c in CULL_CANDIDATES}
+    rows = []
+    try:
+        for robot in ROBOTS:
+            for name, sc, off, edges, mean in shaped_batches(W, robot, seed, quick):
+                ms_of, ref = {}, None
+                for c in CULL_CANDIDATES:
+                    bcc.debug_cull_min(c)
+                    ms, v = time_batch(bcc, robot, sc, off, edges)
                     if ref is None:
-                        ref = got
-                    elif not np.array_equal(got, ref):
-                        bcc.debug_force_variant(-1)
-                        raise SystemExit(f"tune: kernel variants disagree for {robot} at {n_obs} obstacles")
+                        ref = v
+                    elif not np.array_equal(v, ref):
+                        raise SystemExit(f"tune: culling threshold {c} changes verdicts ({robot} {name})")
+                    ms_of[c] = ms
+                    rows.append({"robot": robot, "batch": name, "cull_min": c, "ms": ms})
+                best = min(ms_of.values())
+                for c in CULL_CANDIDATES:
+                    score[c] += ms_of[c] / best
+                print(f"[tune] cull {robot} {name}: " + " ".join(f"{c}:{ms_of[c]:.3f}" for c in CULL_CANDIDATES),
+                      flush=True)
+    finally:
+        bcc.debug_cull_min(-1)
+    return min(score, key=score.get), rows
+
+
+def variant_threshold(bcc, W, robot: str, seed: int, quick: bool, cull_min: int):
+    """Mean obstacle count from which a launch of this robot takes the
+    high-occupancy variant: the candidate (one of the batches' means, or
+    never) minimising the sum of time / best time over the shaped batches
+    when each runs the variant the threshold picks.  Returns (threshold, rows)."""
+    rows, batches = [], []
+    bcc.debug_cull_min(cull_min)
+    try:
+        for name, sc, off, edges, mean in shaped_batches(W, robot, seed, quick):
+            t = {}
+            ref = None
+            for v in (0, 1):
+                bcc.debug_force_variant(v)
+                ms, got = time_batch(bcc, robot, sc, off, edges)
+                if ref is None:
+                    ref = got
+                elif not np.array_equal(got, ref):
+                    raise SystemExit(f"tune: kernel variants disagree for {robot} {name}")
+                t[v] = ms
             bcc.debug_force_variant(-1)
-        rows.append({"robot": robot, "obstacles": n_obs, "ms_low": ms[0], "ms_high": ms[1]})
-        print(f"[tune] {robot} {n_obs} obstacles/env: low {ms[0]:.3f} ms, high {ms[1]:.3f} ms", flush=True)
-    # the first count from which high is faster, or within the 2% timing noise,
-    # at every larger count (ties go to the high-occupancy variant, which wins
-    # on the full-size benchmark batches)
-    for i, n_obs in enumerate(OBSTACLE_COUNTS):
-        if all(r["ms_high"] <= 1.02 * r["ms_low"] for r in rows[i:]):
-            thr = n_obs
-            break
-    return (thr if thr is not None else 1 << 30), rows
+            batches.append((mean, t[0], t[1]))
+            rows.append({"robot": robot, "batch": name, "mean_obstacles": mean, "ms_low": t[0], "ms_high": t[1]})
+            print(f"[tune] {robot} {name} ({mean:.1f} obstacles/env): low {t[0]:.3f} ms, high {t[1]:.3f} ms",
+                  flush=True)
+    finally:
+        bcc.debug_force_variant(-1)
+        bcc.debug_cull_min(-1)
+    cands = sorted({int(np.ceil(m)) for m, _, _ in batches}) + [NEVER]
+
+    def total(thr):
+        return sum((hi if m >= thr else lo) / min(lo, hi) for m, lo, hi in batches)
+
+    return min(cands, key=lambda c: (total(c), -c)), rows
 
 
-def header(table: dict, source: str, thresholds: dict) -> str:
+def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:
     rid = {r: i for i, r in enumerate(ROBOTS)}
     lines = [
-        "// Ragged-batch launch table: per robot, the persistent grid (workgroups per",
-        "// CU) and the target number of tiles per wave, from which a plan sizes its",
-        "// tiles; and the kernel-variant thresholds.  Written by batch-cc_amd/tune.py.",
+        "// Launch table: the culling threshold, per robot the kernel-variant",
+        "// threshold and the ragged launch shape (persistent workgroups per CU and",
+        "// the target number of tiles per wave, from which a plan sizes its tiles).",
+        "// Written by batch-cc_amd/tune.py.",
         "#pragma once",
         "",
         "struct bcc_tune_entry {",
@@ -134,6 +218,10 @@ def header(table: dict, source: str, thresholds: dict) -> str:
         "    int tiles_per_wave;   // target tiles per persistent wave",
         "    int groups_per_cu;    // persistent workgroups per CU",
         "};",
+        "",
+        "// Exact culling of distant obstacles: a sweep of N robot spheres culls when",
+        "// N * (primitives in the environment) is at least this.",
+        f"static const int kCullMinWork = {cull_min};",
         "",
         "// Kernel variant per launch: the high-occupancy variant when the mean number",
         "// of tested obstacles per environment (weighted by pairs) is at least this,",
@@ -164,6 +252,15 @@ def main():
     groups = (4, 6) if a.quick else GROUPS
     scale = 0.25 if a.quick else 1.0
     table, timings = {}, []
+    cull_min, rows = tune_cull(bcc, W, a.seed, a.quick)
+    timings += rows
+    print(f"[tune] culling threshold {cull_min}", flush=True)
+    thresholds = {}
+    for robot in ROBOTS:
+        thresholds[robot], rows = variant_threshold(bcc, W, robot, a.seed, a.quick, cull_min)
+        timings += rows
+        print(f"[tune] {robot}: high-occupancy variant from {thresholds[robot]} obstacles per environment",
+              flush=True)
     for robot in ROBOTS:
         scores = {}
         for kind in ("long", "few-envs", "short"):
@@ -171,6 +268,12 @@ def main():
             sc, off, edges = batch(W, robot, kind, int(PAIRS[robot] * scale), a.seed)
             n = edges.shape[0]
             ms_of, ref = {}, None
+            # the variant the new threshold picks for this batch, and the new culling threshold
+            obs = sc.counts[:, [0, 1, 2, 4, 5]].sum(1).astype(np.float64)
+            w = np.diff(off).astype(np.float64)
+            mean = float((obs * w).sum() / max(w.sum(), 1.0))
+            bcc.debug_force_variant(1 if mean >= thresholds[robot] else 0)
+            bcc.debug_cull_min(cull_min)
             with bcc.Scene(sc) as scene:
                 de = bcc.DeviceBuffer.from_array(edges)
                 do = bcc.DeviceBuffer(n)
@@ -188,6 +291,8 @@ def main():
                         ms_of[(T, g)] = ms
                         timings.append({"robot": robot, "batch": kind, "pairs": n, "tiles_per_wave": T,
                                         "tile": tile, "groups_per_cu": g, "ms": ms})
+            bcc.debug_force_variant(-1)
+            bcc.debug_cull_min(-1)
             best = min(ms_of.values())
             for k, v in ms_of.items():
                 scores[k] = scores.get(k, 0.0) + v / best / 3.0
@@ -198,16 +303,10 @@ def main():
         table[robot] = (k[0], k[1], scores[k])
         print(f"[tune] {robot}: tiles per wave {k[0]}, groups per CU {k[1]} (mean time / best {scores[k]:.3f})",
               flush=True)
-    thresholds = {}
-    for robot in ROBOTS:
-        thresholds[robot], rows = variant_threshold(bcc, W, robot, a.seed, a.quick)
-        timings += rows
-        print(f"[tune] {robot}: high-occupancy variant from {thresholds[robot]} obstacles per environment",
-              flush=True)
     props = bcc.device_name() if hasattr(bcc, "device_name") else "gfx950"
     src = f"tune.py on {props}, {datetime.date.today().isoformat()}, seed {a.seed}" + (" (quick)" if a.quick else "")
     with open(a.out, "w") as f:
-        f.write(header(table, src, thresholds))
+        f.write(header(table, src, thresholds, cull_min))
     if a.json:
         with open(a.json, "w") as f:
             json.dump(timings, f)

@@ -266,6 +266,12 @@ int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t 
  * the scene's mean obstacle count (the default).  Verdicts never depend on it. */
 int bcc_debug_force_variant(int variant);
 
+/* Tuning hook (no reference counterpart): scenes created afterwards cull
+ * distant obstacles in a sweep of N robot spheres only when N times the
+ * environment's primitive count is at least `work` (0 = always, a huge value
+ * = never, -1 = the built-in tuned value).  Verdicts never depend on it. */
+int bcc_debug_cull_min(int work);
+
 /* Device memory helpers for bindings that do not bring their own allocator. */
 int bcc_malloc(void **d_ptr, size_t bytes);
 int bcc_free(void *d_ptr);
