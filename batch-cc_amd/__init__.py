@@ -115,6 +115,7 @@ def lib():
     _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
     _sig(L, "bcc_debug_force_variant", I, [I])
     _sig(L, "bcc_debug_cull_min", I, [I])
+    _sig(L, "bcc_debug_half_boxes", I, [P, ctypes.c_uint64, P])
     _lib = L
     return L
 
@@ -468,6 +469,16 @@ def debug_prims(inp: np.ndarray) -> np.ndarray:
     do = DeviceBuffer(inp.shape[0] * 6 * 4)
     _check(lib().bcc_debug_prims(di.ptr, inp.shape[0], do.ptr))
     return do.to_array(np.float32, (inp.shape[0], 6))
+
+
+def debug_half_boxes(inp: np.ndarray, mask: int) -> np.ndarray:
+    """[64, 4] (x y z rp per lane) -> [2, 2, 3]: (half, lo/hi, xyz)."""
+    inp = np.ascontiguousarray(inp, np.float32)
+    assert inp.shape == (64, 4)
+    di = DeviceBuffer.from_array(inp)
+    do = DeviceBuffer(12 * 4)
+    _check(lib().bcc_debug_half_boxes(di.ptr, ctypes.c_uint64(mask), do.ptr))
+    return do.to_array(np.float32, (2, 2, 3))
 
 
 def debug_copy(d_src: DeviceBuffer, d_dst: DeviceBuffer, nbytes: int):

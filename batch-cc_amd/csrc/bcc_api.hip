@@ -456,6 +456,22 @@ __global__ void prims_kernel(const float* __restrict__ in, int n, float* __restr
     o[5] = bcc::bcc_aabb_val(ap, x, y, z, rp * rp);
 }
 
+// in[64][4]: x y z rp per lane; out[12]: lo0 xyz, hi0 xyz, lo1 xyz, hi1 xyz,
+// the widened half-wave boxes of the lanes in `mask` (one 64-thread wave).
+__global__ void half_boxes_kernel(const float* __restrict__ in, uint64_t mask, float* __restrict__ out)
+{
+    const float* p = in + 4 * threadIdx.x;
+    const float x[1] = {p[0]}, y[1] = {p[1]}, z[1] = {p[2]}, rp[1] = {p[3]};
+    const bcc::HalfBoxes W = bcc::bcc_half_boxes<1>(x, y, z, rp, mask);
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; ++a) {
+            out[a] = W.lo0[a];
+            out[3 + a] = W.hi0[a];
+            out[6 + a] = W.lo1[a];
+            out[9 + a] = W.hi1[a];
+        }
+}
+
 __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1328,6 +1344,16 @@ int bcc_debug_copy(const void* d_src, void* d_dst, size_t bytes)
     hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, 0, (const uint4*)d_src, (uint4*)d_dst, n);
     BCC_HIP(hipGetLastError(), "copy kernel");
     BCC_HIP(hipDeviceSynchronize(), "copy kernel");
+    return BCC_OK;
+}
+
+int bcc_debug_half_boxes(const float* d_in, uint64_t mask, float* d_out)
+{
+    if (!d_in || !d_out)
+        return fail(BCC_ERR_INVALID, "bad half-box arguments");
+    hipLaunchKernelGGL(half_boxes_kernel, dim3(1), dim3(64), 0, 0, d_in, mask, d_out);
+    BCC_HIP(hipGetLastError(), "half-box kernel");
+    BCC_HIP(hipDeviceSynchronize(), "half-box kernel");
     return BCC_OK;
 }
 

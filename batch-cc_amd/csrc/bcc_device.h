@@ -323,20 +323,41 @@ struct HalfBoxes {
     float lo0[3], hi0[3], lo1[3], hi1[3];   // wave-uniform
 };
 
+// Min / max over each 32-lane half-wave; every lane of a half ends with its
+// half's value.  Inside each 16-lane row the partners come from DPP (quad
+// permutations [1,0,3,2] and [2,3,0,1], then the half-row and row mirrors:
+// after the two quad steps every lane holds its quad's value, the half-row
+// mirror pairs the two quads of each 8 lanes, the row mirror the two halves
+// of each row), which are plain VALU operands; only the last step, across
+// the two rows of a half, goes through a lane shuffle.  All 64 lanes must be
+// active (the sweeps run in wave-uniform control flow).
+template <int kCtrl>
+__device__ __forceinline__ float bcc_dpp(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xf, 0xf, false));
+}
+
+constexpr int kDppQuadSwap1 = 0xb1;      // quad_perm [1,0,3,2]
+constexpr int kDppQuadSwap2 = 0x4e;      // quad_perm [2,3,0,1]
+constexpr int kDppRowHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140;
+
 __device__ __forceinline__ float bcc_wave_min32(float v)
 {
-#pragma unroll
-    for (int m = 1; m < 32; m <<= 1)
-        v = fminf(v, __shfl_xor(v, m, 64));
-    return v;
+    v = fminf(v, bcc_dpp<kDppQuadSwap1>(v));
+    v = fminf(v, bcc_dpp<kDppQuadSwap2>(v));
+    v = fminf(v, bcc_dpp<kDppRowHalfMirror>(v));
+    v = fminf(v, bcc_dpp<kDppRowMirror>(v));
+    return fminf(v, __shfl_xor(v, 16, 64));
 }
 
 __device__ __forceinline__ float bcc_wave_max32(float v)
 {
-#pragma unroll
-    for (int m = 1; m < 32; m <<= 1)
-        v = fmaxf(v, __shfl_xor(v, m, 64));
-    return v;
+    v = fmaxf(v, bcc_dpp<kDppQuadSwap1>(v));
+    v = fmaxf(v, bcc_dpp<kDppQuadSwap2>(v));
+    v = fmaxf(v, bcc_dpp<kDppRowHalfMirror>(v));
+    v = fmaxf(v, bcc_dpp<kDppRowMirror>(v));
+    return fmaxf(v, __shfl_xor(v, 16, 64));
 }
 
 __device__ __forceinline__ float bcc_read_lane(float v, int lane)

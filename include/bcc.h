@@ -272,6 +272,13 @@ int bcc_debug_force_variant(int variant);
  * = never, -1 = the built-in tuned value).  Verdicts never depend on it. */
 int bcc_debug_cull_min(int work);
 
+/* Test hook (no reference counterpart): the culling step's half-wave boxes
+ * for one wave.  d_in [64][4] = x y z padded-radius per lane; d_out [12] =
+ * lo xyz, hi xyz of lanes 0-31 then of lanes 32-63 among the lanes in `mask`,
+ * each half widened by 0.01 + 1e-5 * its largest |coordinate| (an empty half
+ * gives lo = +inf, hi = -inf). */
+int bcc_debug_half_boxes(const float *d_in, uint64_t mask, float *d_out);
+
 /* Device memory helpers for bindings that do not bring their own allocator. */
 int bcc_malloc(void **d_ptr, size_t bytes);
 int bcc_free(void *d_ptr);

@@ -209,3 +209,32 @@ def test_non_orthonormal_cuboids_are_never_culled(bcc, oracle, gpu):
         got = bcc.check_dense("panda", scene, edges)
     print(f"colliding pairs {int(exp.sum())} of {exp.size}")
     assert np.array_equal(got, exp)
+
+
+def test_half_wave_boxes_match_numpy(bcc, gpu):
+    """The culling step's cross-lane reduction: per half-wave box of the
+    masked lanes' spheres, widened by 0.01 + 1e-5 * largest |coordinate|."""
+    rng = np.random.default_rng(45)
+    full = (1 << 64) - 1
+    masks = [full, (1 << 32) - 1, full ^ ((1 << 32) - 1), 1, 1 << 63, (1 << 17) | (1 << 40), 0,
+             int(rng.integers(0, 2**63)) * 2 + 1]
+    masks += [int.from_bytes(rng.bytes(8), "little") for _ in range(6)]
+    for k, mask in enumerate(masks):
+        inp = np.zeros((64, 4), np.float32)
+        inp[:, :3] = rng.uniform(-3, 3, (64, 3)) * (10.0 ** rng.integers(-1, 3))
+        inp[:, 3] = rng.uniform(0.0, 0.3, 64)
+        got = bcc.debug_half_boxes(inp, mask)
+        for h in range(2):
+            lanes = [i for i in range(32 * h, 32 * h + 32) if (mask >> i) & 1]
+            if not lanes:
+                assert np.all(got[h, 0] == np.inf) and np.all(got[h, 1] == -np.inf), (k, h)
+                continue
+            c, r = inp[lanes, :3], inp[lanes, 3:4]
+            lo = (c - r).min(0).astype(np.float32)
+            hi = (c + r).max(0).astype(np.float32)
+            big = np.float32(max(np.abs(lo).max(), np.abs(hi).max()))
+            delta = np.float32(np.float64(big) * np.float64(np.float32(1e-5)) + np.float64(np.float32(0.01)))
+            assert np.allclose(got[h, 0], lo - delta, rtol=0, atol=1e-6 * (1 + np.abs(lo).max())), (k, h)
+            assert np.allclose(got[h, 1], hi + delta, rtol=0, atol=1e-6 * (1 + np.abs(hi).max())), (k, h)
+            # conservative: the widened box holds every masked sphere
+            assert np.all(got[h, 0] <= c - r) and np.all(got[h, 1] >= c + r), (k, h)
