@@ -169,6 +169,25 @@ __device__ __forceinline__ int setup_pair_lds(const float* __restrict__ se, int 
     return n;
 }
 
+// Sample configuration of round r, lane t of a pair under a learned order,
+// into the LDS row Q; returns the pair's step count.  The pair's start and
+// increment are recomputed from its edge every round (the same arithmetic as
+// setup_pair, so the samples are the plain ones), so the ordered kernels
+// carry no per-pair state across the check program beyond the plain
+// kernels' (edge, step, n) and the order word.
+template <int D>
+__device__ __forceinline__ int ordered_round(const float* __restrict__ se, int resolution, uint32_t top, int r, int t,
+                                             const LdsRow& Q)
+{
+    float s0[D], d[D], qq[D];
+    const int n = setup_pair_sd<D>(se, resolution, s0, d);
+    ordered_sample<D>(s0, d, n, top, r, t, qq);
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        Q.at(j) = qq[j];
+    return n;
+}
+
 // kOrdered: a learned top-4 order per pair, order[edge][env], decides which
 // samples each round visits (ordered_sample); otherwise plain order.
 // rounds (optional): number of rounds (steps) each pair ran, [edge][env].
@@ -198,12 +217,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const LdsRow Q{&qs[0][threadIdx.x]};
     const LdsRow Dl{&ds[0][threadIdx.x]};
 
-    float s0[kOrdered ? D : 1], d[kOrdered ? D : 1];
-#pragma unroll
-    for (int j = 0; j < (kOrdered ? D : 1); ++j) {
-        s0[j] = 0.0f;
-        d[j] = 0.0f;
-    }
     int cur = -1, step = 0, n = 0;
     uint32_t top = 0;
     bool exhausted = false;
@@ -219,13 +232,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 cur = g;
                 step = 0;
                 if constexpr (kOrdered) {
-                    n = setup_pair_sd<D>(edges + (size_t)g * 2 * D, resolution, s0, d);
                     top = valid_order(order[(size_t)g * num_envs + env]);
-                    float qq[D];
-                    ordered_sample<D>(s0, d, n, top, 0, t, qq);
-#pragma unroll
-                    for (int j = 0; j < D; ++j)
-                        Q.at(j) = qq[j];
+                    n = ordered_round<D>(edges + (size_t)g * 2 * D, resolution, top, 0, t, Q);
                 } else {
                     n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
                 }
@@ -245,11 +253,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 fin = (step == n);
                 if (!fin) {
                     if constexpr (kOrdered) {
-                        float qq[D];
-                        ordered_sample<D>(s0, d, n, top, step, t, qq);
-#pragma unroll
-                        for (int j = 0; j < D; ++j)
-                            Q.at(j) = qq[j];
+                        ordered_round<D>(edges + (size_t)cur * 2 * D, resolution, top, step, t, Q);
                     } else {
 #pragma unroll
                         for (int j = 0; j < D; ++j)
@@ -304,12 +308,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const int half0 = lane & 32;
     const LdsRow Q{&qs[0][threadIdx.x]};
     const LdsRow Dl{&ds[0][threadIdx.x]};
-    float s0[kOrdered ? D : 1], d[kOrdered ? D : 1];
-#pragma unroll
-    for (int j = 0; j < (kOrdered ? D : 1); ++j) {
-        s0[j] = 0.0f;
-        d[j] = 0.0f;
-    }
     uint32_t top = 0;
 
     for (;;) {
@@ -341,13 +339,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                     cur = g;
                     step = 0;
                     if constexpr (kOrdered) {
-                        n = setup_pair_sd<D>(edges + (size_t)g * 2 * D, resolution, s0, d);
                         top = valid_order(order[g]);
-                        float qq[D];
-                        ordered_sample<D>(s0, d, n, top, 0, t, qq);
-#pragma unroll
-                        for (int j = 0; j < D; ++j)
-                            Q.at(j) = qq[j];
+                        n = ordered_round<D>(edges + (size_t)g * 2 * D, resolution, top, 0, t, Q);
                     } else {
                         n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
                     }
@@ -367,11 +360,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                     fin = (step == n);
                     if (!fin) {
                         if constexpr (kOrdered) {
-                            float qq[D];
-                            ordered_sample<D>(s0, d, n, top, step, t, qq);
-#pragma unroll
-                            for (int j = 0; j < D; ++j)
-                                Q.at(j) = qq[j];
+                            ordered_round<D>(edges + (size_t)cur * 2 * D, resolution, top, step, t, Q);
                         } else {
 #pragma unroll
                             for (int j = 0; j < D; ++j)
@@ -415,6 +404,7 @@ struct RaggedArgs {
 // its high- and low-occupancy variants.
 struct VariantKnobs {
     int hi_waves, hi_unroll, lo_waves, lo_unroll;
+    int ordered_hi;   // 1: the learned-order kernels use the high-occupancy knobs
 };
 
 // One persistent launch of one kernel variant (instantiated in its own
@@ -427,14 +417,14 @@ hipError_t launch_ragged_variant(const RaggedArgs& a, hipStream_t stream)
     return hipGetLastError();
 }
 
-template <class Robot, int HW, int HU, int LW, int LU>
+template <class Robot, int HW, int HU, int LW, int LU, int OW, int OU>
 hipError_t launch_ragged(const RaggedArgs& a, hipStream_t stream)
 {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.tile_counter);
     if (a.num_tiles == 0)
         return hipGetLastError();
-    if (a.order)   // the ordered path keeps its pair state in registers: low-occupancy variant
-        return launch_ragged_variant<Robot, true, LW, LU>(a, stream);
+    if (a.order)
+        return launch_ragged_variant<Robot, true, OW, OU>(a, stream);
     if (a.variant)
         return launch_ragged_variant<Robot, false, HW, HU>(a, stream);
     return launch_ragged_variant<Robot, false, LW, LU>(a, stream);
@@ -720,7 +710,7 @@ hipError_t launch_dense_variant(const DenseArgs& a, unsigned blocks, int chunk, 
     return hipGetLastError();
 }
 
-template <class Robot, int HW, int HU, int LW, int LU>
+template <class Robot, int HW, int HU, int LW, int LU, int OW, int OU>
 hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
 {
     if (a.num_edges == 0 || a.num_envs == 0)
@@ -730,8 +720,8 @@ hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
     const long long blocks = (long long)chunks_per_env * a.num_envs;
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    if (a.order)   // the ordered path keeps its pair state in registers: low-occupancy variant
-        return launch_dense_variant<Robot, true, LW, LU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
+    if (a.order)   // one ordered instantiation per robot, with the robot's ordered-path knobs
+        return launch_dense_variant<Robot, true, OW, OU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
     if (a.variant)
         return launch_dense_variant<Robot, false, HW, HU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
     return launch_dense_variant<Robot, false, LW, LU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);

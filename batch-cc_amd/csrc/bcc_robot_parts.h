@@ -5,15 +5,16 @@
 // Included at the end of robot_<name>.hip after
 //   BCC_ROBOT       robot name (panda, fetch, baxter)
 //   BCC_ROBOT_T     its check struct (Robot_<name>)
-//   BCC_KNOBS       HW, HU, LW, LU: waves per SIMD and obstacle-loop unroll of
-//                   the high- and low-occupancy kernel variants
+//   BCC_KNOBS       HW, HU, LW, LU, OH: waves per SIMD and obstacle-loop unroll
+//                   of the high- and low-occupancy kernel variants, and
+//                   whether the learned-order kernels use the high ones
 // BCC_PART selects what this translation unit instantiates:
 //   unset  everything
 //   0      host dispatchers and the small kernels (FK, samples, work, labels)
-//   1      ordered dense kernel (low-occupancy knobs)
+//   1      ordered dense kernel (ordered-path knobs)
 //   2, 3   plain dense kernel, high / low occupancy
 //   4, 5   ragged kernel, high / low occupancy
-//   6      ordered ragged kernel (low-occupancy knobs)
+//   6      ordered ragged kernel (ordered-path knobs)
 #ifndef BCC_PART
 #define BCC_PART -1
 #endif
@@ -24,6 +25,7 @@ namespace robot_parts {
 using RobotT = BCC_ROBOT_T;
 constexpr VariantKnobs kKnobs = {BCC_KNOBS};
 constexpr int HW = kKnobs.hi_waves, HU = kKnobs.hi_unroll, LW = kKnobs.lo_waves, LU = kKnobs.lo_unroll;
+constexpr int OW = kKnobs.ordered_hi ? HW : LW, OU = kKnobs.ordered_hi ? HU : LU;
 
 }  // namespace robot_parts
 
@@ -32,7 +34,7 @@ constexpr int HW = kKnobs.hi_waves, HU = kKnobs.hi_unroll, LW = kKnobs.lo_waves,
 
 #if BCC_PART == 0
 // instantiated in parts 1-6
-extern template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+extern template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
 extern template hipError_t launch_dense_variant<robot_parts::RobotT, false, robot_parts::HW, robot_parts::HU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
@@ -42,7 +44,7 @@ extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, rob
     const RaggedArgs&, hipStream_t);
 extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
     const RaggedArgs&, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
     const RaggedArgs&, hipStream_t);
 #endif
 
@@ -52,13 +54,13 @@ const VariantKnobs BCC_CAT2(knobs_, BCC_ROBOT) = robot_parts::kKnobs;
 hipError_t BCC_CAT2(launch_dense_, BCC_ROBOT)(const DenseArgs& a, hipStream_t stream)
 {
     using namespace robot_parts;
-    return launch_dense<RobotT, HW, HU, LW, LU>(a, stream);
+    return launch_dense<RobotT, HW, HU, LW, LU, OW, OU>(a, stream);
 }
 
 hipError_t BCC_CAT2(launch_ragged_, BCC_ROBOT)(const RaggedArgs& a, hipStream_t stream)
 {
     using namespace robot_parts;
-    return launch_ragged<RobotT, HW, HU, LW, LU>(a, stream);
+    return launch_ragged<RobotT, HW, HU, LW, LU, OW, OU>(a, stream);
 }
 
 hipError_t BCC_CAT2(launch_fk_, BCC_ROBOT)(const float* qs, int n, float* out, hipStream_t stream)
@@ -84,7 +86,7 @@ hipError_t BCC_CAT2(launch_label_, BCC_ROBOT)(const LabelArgs& a, hipStream_t st
 #endif
 
 #if BCC_PART == 1
-template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
 #endif
 #if BCC_PART == 2
@@ -104,7 +106,7 @@ template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_part
     const RaggedArgs&, hipStream_t);
 #endif
 #if BCC_PART == 6
-template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
     const RaggedArgs&, hipStream_t);
 #endif
 

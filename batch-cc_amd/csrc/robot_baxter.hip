@@ -7,10 +7,11 @@ namespace bcc {
 #include "robots/baxter_check.inc"
 }  // namespace bcc
 
-// Kernel variants (waves per SIMD, obstacle-loop unroll), measured on MI355X:
-// high occupancy for scenes with many obstacles per environment, low for few.
+// Kernel variants (waves per SIMD, obstacle-loop unroll of the high- and
+// low-occupancy variants; 1 if the learned-order kernels use the high ones),
+// measured on MI355X.
 #ifndef BCC_BAXTER_VARIANTS
-#define BCC_BAXTER_VARIANTS 5, 1, 4, 1
+#define BCC_BAXTER_VARIANTS 5, 1, 4, 1, 0
 #endif
 #define BCC_ROBOT baxter
 #define BCC_ROBOT_T Robot_baxter
