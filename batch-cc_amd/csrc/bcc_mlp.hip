@@ -215,7 +215,7 @@ __global__ void to_bf16_kernel(const float* __restrict__ a, long long n, uint16_
 // column l % 16) is already the next layer's B operand (lane l: k rows
 // 4 (l/16) + v, column l % 16) and activations never leave registers.
 // Columns are pairs, 16 per MFMA tile, 64 per wave.  Weights are read as
-// prepacked A fragments: fragment (rt, kt) of a layer with W [K][N] holds for
+// prepacked A fragments (staged through LDS, see below): fragment (rt, kt) of a layer with W [K][N] holds for
 // lane l the four values W[16 kt + 4 (l/16) + i][16 rt + l % 16], i = 0..3.
 // Every product, its k order and the bias/ReLU/rounding steps are those of
 // the layer-by-layer path (gemm_bf16_kernel + bias_act_kernel), so logits
@@ -229,6 +229,36 @@ constexpr int kFragL2 = kFragL1 + kRT12 * kKT1;
 constexpr int kFragL3 = kFragL2 + kRT12 * kKT2;
 constexpr int kFrags = kFragL3 + kRT3 * kKT2;
 constexpr int kFuseCols = 4;                     // 16-pair column tiles per wave
+
+// Weight staging: the workgroup's four waves share each weight fragment,
+// copied once per workgroup into LDS instead of being read from L2 by every
+// wave.  Stage rt holds layer 2's row tile rt (16 contiguous fragments) and
+// layer 3's two fragments at k block rt (layers 2 and 3 hold three quarters
+// of the weights).
+constexpr int kStageFrags = kKT2 + kRT3;                      // 18 fragments
+constexpr int kStageLoads = (kStageFrags * 64 + 255) / 256;   // per thread
+
+__device__ __forceinline__ void stage_load(const bf16x4* __restrict__ frag, int rt, bf16x4 (&pre)[kStageLoads])
+{
+#pragma unroll
+    for (int i = 0; i < kStageLoads; ++i) {
+        const int e = threadIdx.x + 256 * i;
+        if (e < kKT2 * 64)
+            pre[i] = frag[(kFragL2 + rt * kKT2) * 64 + e];
+        else if (e < kStageFrags * 64)
+            pre[i] = frag[(kFragL3 + ((e >> 6) - kKT2) * kKT2 + rt) * 64 + (e & 63)];
+    }
+}
+
+__device__ __forceinline__ void stage_store(bf16x4* buf, const bf16x4 (&pre)[kStageLoads])
+{
+#pragma unroll
+    for (int i = 0; i < kStageLoads; ++i) {
+        const int e = threadIdx.x + 256 * i;
+        if (e < kStageFrags * 64)
+            buf[e] = pre[i];
+    }
+}
 
 __global__ void pack_fragments_kernel(const uint16_t* __restrict__ W1, const uint16_t* __restrict__ W2,
                                       const uint16_t* __restrict__ W3, uint16_t* __restrict__ frag)
@@ -301,7 +331,9 @@ __global__ __launch_bounds__(256) void fused_order_kernel(
             }
     }
 
-    // layer 1: Y1^T (256 x 64) kept as BF16 B operands of layer 2
+    // layer 1: Y1^T (256 x 64) kept as BF16 B operands of layer 2 (its
+    // weights are read directly: staging them too, inside this fully unrolled
+    // loop, costs the kernel its second resident workgroup)
     bf16x4 y1[kKT2][kFuseCols];
 #pragma unroll
     for (int rt = 0; rt < kRT12; ++rt) {
@@ -327,7 +359,10 @@ __global__ __launch_bounds__(256) void fused_order_kernel(
         }
     }
 
-    // layers 2 and 3: each layer-2 row tile is layer 3's next k block
+    // layers 2 and 3: each layer-2 row tile is layer 3's next k block; its
+    // weights come through LDS, copied by the whole workgroup (the loads'
+    // latency is covered by the other resident workgroup's MFMAs)
+    __shared__ bf16x4 wbuf[kStageFrags * 64];
     f32x4 acc3[kRT3][kFuseCols];
 #pragma unroll
     for (int r3 = 0; r3 < kRT3; ++r3)
@@ -335,13 +370,20 @@ __global__ __launch_bounds__(256) void fused_order_kernel(
         for (int ct = 0; ct < kFuseCols; ++ct)
             acc3[r3][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int rt = 0; rt < kRT12; ++rt) {
+        {
+            bf16x4 pre[kStageLoads];
+            stage_load(frag, rt, pre);
+            __syncthreads();   // every wave is done with the previous stage
+            stage_store(wbuf, pre);
+        }
+        __syncthreads();
         f32x4 acc[kFuseCols];
 #pragma unroll
         for (int ct = 0; ct < kFuseCols; ++ct)
             acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < kKT2; ++kt) {
-            const bf16x4 a = frag[(kFragL2 + rt * kKT2 + kt) * 64 + lane];
+            const bf16x4 a = wbuf[kt * 64 + lane];
 #pragma unroll
             for (int ct = 0; ct < kFuseCols; ++ct)
                 acc[ct] = mfma16(a, y1[kt][ct], acc[ct]);
@@ -358,7 +400,7 @@ __global__ __launch_bounds__(256) void fused_order_kernel(
         }
 #pragma unroll
         for (int r3 = 0; r3 < kRT3; ++r3) {
-            const bf16x4 a = frag[(kFragL3 + r3 * kKT2 + rt) * 64 + lane];
+            const bf16x4 a = wbuf[(kKT2 + r3) * 64 + lane];
 #pragma unroll
             for (int ct = 0; ct < kFuseCols; ++ct)
                 acc3[r3][ct] = mfma16(a, y2[ct], acc3[r3][ct]);
