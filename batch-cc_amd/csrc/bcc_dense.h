@@ -203,9 +203,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const int e_begin = ck * chunk;
     const int e_end = min(num_edges, e_begin + chunk);
 
+    // rounds from the edge (ordered_round) or stepped by an LDS increment row
+    constexpr bool kFromEdge = kOrdered || Robot::kRoundsFromEdge;
     __shared__ int next_edge;
     __shared__ float qs[D][kBlock];
-    __shared__ float ds[kOrdered ? 1 : D][kBlock];
+    __shared__ float ds[kFromEdge ? 1 : D][kBlock];
     if (threadIdx.x == 0)
         next_edge = e_begin;
     __syncthreads();
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const LdsRow Dl{&ds[0][threadIdx.x]};
 
     int cur = -1, step = 0, n = 0;
-    uint32_t top = 0;
+    uint32_t top = 0xffffffffu;   // plain order unless a learned order is given
     bool exhausted = false;
 
     for (;;) {
@@ -231,12 +233,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
             if (g < e_end) {
                 cur = g;
                 step = 0;
-                if constexpr (kOrdered) {
+                if constexpr (kOrdered)
                     top = valid_order(order[(size_t)g * num_envs + env]);
+                if constexpr (kFromEdge)
                     n = ordered_round<D>(edges + (size_t)g * 2 * D, resolution, top, 0, t, Q);
-                } else {
+                else
                     n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
-                }
             } else {
                 exhausted = true;
             }
@@ -252,7 +254,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 ++step;
                 fin = (step == n);
                 if (!fin) {
-                    if constexpr (kOrdered) {
+                    if constexpr (kFromEdge) {
                         ordered_round<D>(edges + (size_t)cur * 2 * D, resolution, top, step, t, Q);
                     } else {
 #pragma unroll
@@ -294,7 +296,8 @@ struct RaggedTile {
 // Each round's samples are computed from the pair's edge (ordered_round; in
 // plain order: start + delta * (t n), then `step` additions of delta, the
 // plain samples), not stepped from an increment row in LDS: on MI355X this
-// is 4-6% faster for fetch and baxter batches and equal for panda.
+// is 4-6% faster for fetch and baxter batches and equal for panda.  Dense
+// kernels do the same where the robot's kRoundsFromEdge says so.
 template <class Robot, bool kOrdered, int kWaves, int kUnroll>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void ragged_kernel(
     const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges,

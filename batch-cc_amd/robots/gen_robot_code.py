@@ -232,6 +232,11 @@ CHUNKS = {"panda": (2, 2, 4), "fetch": (1, 2, 4), "baxter": (1, 2, 4)}
 # baxter's much longer check program the extra code costs more in
 # instruction-cache misses than the plain loops save (MI355X, c2 and c4).
 DUAL_SWEEP = {"panda": True, "fetch": True, "baxter": False}
+# Dense kernels compute each round's samples from the edge (as the ragged and
+# learned-order kernels do) or step them by an increment row in LDS: the
+# first is faster for fetch (c1 -1.7%), no faster for panda and baxter
+# (within the 2% run-to-run spread).  MI355X measurements.
+ROUNDS_FROM_EDGE = {"panda": False, "fetch": True, "baxter": False}
 GATE_CHUNK = 2      # defaults for robots not listed; set per robot by gen_check
 FINE_CHUNK = 4
 FIRST_CHUNK = 0     # size of the first environment chunk (0: GATE_CHUNK)
@@ -456,6 +461,8 @@ def gen_check(model: dict, order: str | None = None) -> str:
         f"    static constexpr int num_spheres = {len(sph)};",
         "    // sweeps below the culling threshold get their own plain loops",
         f"    static constexpr bool kDualSweep = {'true' if DUAL_SWEEP.get(robot, True) else 'false'};",
+        "    // dense kernels compute each round's samples from the edge",
+        f"    static constexpr bool kRoundsFromEdge = {'true' if ROUNDS_FROM_EDGE.get(robot, True) else 'false'};",
         "    // Collision predicate for one sample per lane.  `live` holds the lanes whose",
         "    // half-wave pair is still undecided; returns the half-spread mask of lanes",
         "    // whose half found a colliding sample (each collision decides its half).",
