@@ -219,13 +219,14 @@ def padded_radius(r: float) -> float:
 
 
 # Spheres per obstacle sweep, per robot: (first environment chunk, later
-# environment chunks, fine spheres of a gated group).  Fewer spheres per sweep
-# keep fewer temporaries live (less register spilling) and let a decided
-# half-wave leave sooner; more spheres amortise each obstacle's scalar loads.
-# A first chunk of one gate lets samples that hit the base links leave after
-# one sweep.  Measured on MI355X over the benchmark workloads (gate 1..4,
-# fine 2..8, first chunk 1 or the gate chunk).
-CHUNKS = {"panda": (2, 2, 4), "fetch": (1, 2, 4), "baxter": (1, 2, 4)}
+# environment chunks, fine spheres of a gated group).  With exact culling,
+# larger sweeps pay: the half-wave box reduction and the near-obstacle
+# ballot are paid once per sweep and cover more spheres.  Measured on MI355X
+# (c1, c3, c2, c4's three parts, the README run, a dense baxter batch) over
+# gate chunks 1-8 and fine chunks 2-32; fine 16 takes all but the two largest
+# groups in one sweep.  Panda's (2, 4, 16) trades 5% on c3 for 16% on c4's
+# panda part; fetch's gate chunk 6 is best on c1 (c4's fetch part prefers 8).
+CHUNKS = {"panda": (2, 4, 16), "fetch": (1, 6, 16), "baxter": (1, 4, 16)}
 # Sweeps below the culling threshold run plain counted loops over all
 # obstacles (a second copy of each sweep's code) or the culled loop with every
 # obstacle marked near.  The second copy pays for panda and fetch; for
