@@ -12,16 +12,16 @@ struct bcc_tune_entry {
 
 // Exact culling of distant obstacles: a sweep of N robot spheres culls when
 // N * (primitives in the environment) is at least this.
-static const int kCullMinWork = 40;
+static const int kCullMinWork = 32;
 
 // Kernel variant per launch: the high-occupancy variant when the mean number
 // of tested obstacles per environment (weighted by pairs) is at least this,
 // per robot (panda, fetch, baxter).
-static const int kHiVariantMinObstacles[3] = {7, 7, 1073741824};
+static const int kHiVariantMinObstacles[3] = {32, 7, 1073741824};
 
 // source: tune.py on gfx950, 2026-12-04, seed 0
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 8, 8},   // panda: score 1.008
-    {1, 4, 8},   // fetch: score 1.002
-    {2, 16, 5},   // baxter: score 1.001
+    {0, 8, 6},   // panda: score 1.023
+    {1, 4, 8},   // fetch: score 1.000
+    {2, 16, 4},   // baxter: score 1.000
 };
