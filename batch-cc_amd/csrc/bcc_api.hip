@@ -930,7 +930,7 @@ int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env
 namespace {
 
 // Plan over one CSR layout; `ordered` selects the ordered kernel (d_order:
-// one word per pair, on the device), which runs the robot's ordered-path knobs.
+// one word per pair, on the device); the kernel variant is chosen as for plain plans.
 int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* d_edges, int resolution,
                 bool ordered, const uint32_t* d_order, uint8_t* d_results, int tile_pairs, int groups_per_cu,
                 bcc_ragged_plan** out)
@@ -960,8 +960,7 @@ int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, c
     double weighted = 0.0;   // mean obstacles per environment, weighted by pairs
     for (int e = 0; e < E; ++e)
         weighted += (double)scene->obstacles[e] * (double)(env_offsets[e + 1] - env_offsets[e]);
-    const int variant = ordered ? knobs(robot).ordered_hi
-                                : choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
+    const int variant = choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
     const bcc_tune_entry& tune = tuned(robot);
     if (groups_per_cu == 0)
         groups_per_cu = tune.groups_per_cu;

@@ -399,7 +399,6 @@ struct RaggedArgs {
 // its high- and low-occupancy variants.
 struct VariantKnobs {
     int hi_waves, hi_unroll, lo_waves, lo_unroll;
-    int ordered_hi;   // 1: the learned-order kernels use the high-occupancy knobs
 };
 
 // One persistent launch of one kernel variant (instantiated in its own
@@ -412,14 +411,16 @@ hipError_t launch_ragged_variant(const RaggedArgs& a, hipStream_t stream)
     return hipGetLastError();
 }
 
-template <class Robot, int HW, int HU, int LW, int LU, int OW, int OU>
+template <class Robot, int HW, int HU, int LW, int LU>
 hipError_t launch_ragged(const RaggedArgs& a, hipStream_t stream)
 {
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.tile_counter);
     if (a.num_tiles == 0)
         return hipGetLastError();
+    if (a.order && a.variant)
+        return launch_ragged_variant<Robot, true, HW, HU>(a, stream);
     if (a.order)
-        return launch_ragged_variant<Robot, true, OW, OU>(a, stream);
+        return launch_ragged_variant<Robot, true, LW, LU>(a, stream);
     if (a.variant)
         return launch_ragged_variant<Robot, false, HW, HU>(a, stream);
     return launch_ragged_variant<Robot, false, LW, LU>(a, stream);
@@ -705,7 +706,7 @@ hipError_t launch_dense_variant(const DenseArgs& a, unsigned blocks, int chunk, 
     return hipGetLastError();
 }
 
-template <class Robot, int HW, int HU, int LW, int LU, int OW, int OU>
+template <class Robot, int HW, int HU, int LW, int LU>
 hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
 {
     if (a.num_edges == 0 || a.num_envs == 0)
@@ -715,8 +716,10 @@ hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
     const long long blocks = (long long)chunks_per_env * a.num_envs;
     if (blocks > 0x7fffffffLL)
         return hipErrorInvalidConfiguration;
-    if (a.order)   // one ordered instantiation per robot, with the robot's ordered-path knobs
-        return launch_dense_variant<Robot, true, OW, OU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
+    if (a.order && a.variant)
+        return launch_dense_variant<Robot, true, HW, HU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
+    if (a.order)
+        return launch_dense_variant<Robot, true, LW, LU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
     if (a.variant)
         return launch_dense_variant<Robot, false, HW, HU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);
     return launch_dense_variant<Robot, false, LW, LU>(a, (unsigned)blocks, chunk, chunks_per_env, stream);

@@ -5,16 +5,15 @@
 // Included at the end of robot_<name>.hip after
 //   BCC_ROBOT       robot name (panda, fetch, baxter)
 //   BCC_ROBOT_T     its check struct (Robot_<name>)
-//   BCC_KNOBS       HW, HU, LW, LU, OH: waves per SIMD and obstacle-loop unroll
-//                   of the high- and low-occupancy kernel variants, and
-//                   whether the learned-order kernels use the high ones
+//   BCC_KNOBS       HW, HU, LW, LU: waves per SIMD and obstacle-loop unroll of
+//                   the high- and low-occupancy kernel variants
 // BCC_PART selects what this translation unit instantiates:
 //   unset  everything
 //   0      host dispatchers and the small kernels (FK, samples, work, labels)
-//   1      ordered dense kernel (ordered-path knobs)
+//   1, 7   ordered dense kernel, low / high occupancy
 //   2, 3   plain dense kernel, high / low occupancy
 //   4, 5   ragged kernel, high / low occupancy
-//   6      ordered ragged kernel (ordered-path knobs)
+//   6, 8   ordered ragged kernel, low / high occupancy
 #ifndef BCC_PART
 #define BCC_PART -1
 #endif
@@ -25,7 +24,6 @@ namespace robot_parts {
 using RobotT = BCC_ROBOT_T;
 constexpr VariantKnobs kKnobs = {BCC_KNOBS};
 constexpr int HW = kKnobs.hi_waves, HU = kKnobs.hi_unroll, LW = kKnobs.lo_waves, LU = kKnobs.lo_unroll;
-constexpr int OW = kKnobs.ordered_hi ? HW : LW, OU = kKnobs.ordered_hi ? HU : LU;
 
 }  // namespace robot_parts
 
@@ -33,8 +31,10 @@ constexpr int OW = kKnobs.ordered_hi ? HW : LW, OU = kKnobs.ordered_hi ? HU : LU
 #define BCC_CAT2(a, b) BCC_CAT2_(a, b)
 
 #if BCC_PART == 0
-// instantiated in parts 1-6
-extern template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
+// instantiated in parts 1-8
+extern template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+    const DenseArgs&, unsigned, int, int, hipStream_t);
+extern template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::HW, robot_parts::HU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
 extern template hipError_t launch_dense_variant<robot_parts::RobotT, false, robot_parts::HW, robot_parts::HU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
@@ -44,7 +44,9 @@ extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, rob
     const RaggedArgs&, hipStream_t);
 extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
     const RaggedArgs&, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+    const RaggedArgs&, hipStream_t);
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::HW, robot_parts::HU>(
     const RaggedArgs&, hipStream_t);
 #endif
 
@@ -54,13 +56,13 @@ const VariantKnobs BCC_CAT2(knobs_, BCC_ROBOT) = robot_parts::kKnobs;
 hipError_t BCC_CAT2(launch_dense_, BCC_ROBOT)(const DenseArgs& a, hipStream_t stream)
 {
     using namespace robot_parts;
-    return launch_dense<RobotT, HW, HU, LW, LU, OW, OU>(a, stream);
+    return launch_dense<RobotT, HW, HU, LW, LU>(a, stream);
 }
 
 hipError_t BCC_CAT2(launch_ragged_, BCC_ROBOT)(const RaggedArgs& a, hipStream_t stream)
 {
     using namespace robot_parts;
-    return launch_ragged<RobotT, HW, HU, LW, LU, OW, OU>(a, stream);
+    return launch_ragged<RobotT, HW, HU, LW, LU>(a, stream);
 }
 
 hipError_t BCC_CAT2(launch_fk_, BCC_ROBOT)(const float* qs, int n, float* out, hipStream_t stream)
@@ -86,7 +88,11 @@ hipError_t BCC_CAT2(launch_label_, BCC_ROBOT)(const LabelArgs& a, hipStream_t st
 #endif
 
 #if BCC_PART == 1
-template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
+template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+    const DenseArgs&, unsigned, int, int, hipStream_t);
+#endif
+#if BCC_PART == 7
+template hipError_t launch_dense_variant<robot_parts::RobotT, true, robot_parts::HW, robot_parts::HU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
 #endif
 #if BCC_PART == 2
@@ -106,7 +112,11 @@ template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_part
     const RaggedArgs&, hipStream_t);
 #endif
 #if BCC_PART == 6
-template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::OW, robot_parts::OU>(
+template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
+    const RaggedArgs&, hipStream_t);
+#endif
+#if BCC_PART == 8
+template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::HW, robot_parts::HU>(
     const RaggedArgs&, hipStream_t);
 #endif
 

@@ -8,10 +8,9 @@ namespace bcc {
 }  // namespace bcc
 
 // Kernel variants (waves per SIMD, obstacle-loop unroll of the high- and
-// low-occupancy variants; 1 if the learned-order kernels use the high ones),
-// measured on MI355X.
+// low-occupancy variants), measured on MI355X.
 #ifndef BCC_FETCH_VARIANTS
-#define BCC_FETCH_VARIANTS 8, 1, 4, 1, 1
+#define BCC_FETCH_VARIANTS 8, 1, 4, 1
 #endif
 #define BCC_ROBOT fetch
 #define BCC_ROBOT_T Robot_fetch

@@ -8,10 +8,9 @@ namespace bcc {
 }  // namespace bcc
 
 // Kernel variants (waves per SIMD, obstacle-loop unroll of the high- and
-// low-occupancy variants; 1 if the learned-order kernels use the high ones),
-// measured on MI355X.
+// low-occupancy variants), measured on MI355X.
 #ifndef BCC_PANDA_VARIANTS
-#define BCC_PANDA_VARIANTS 6, 1, 4, 1, 0
+#define BCC_PANDA_VARIANTS 6, 1, 4, 1
 #endif
 #define BCC_ROBOT panda
 #define BCC_ROBOT_T Robot_panda

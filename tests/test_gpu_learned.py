@@ -169,12 +169,17 @@ def test_ordered_checking_is_exact_for_any_order(bcc, oracle, gpu):
             orders.append((o[:, 0] | o[:, 1] << 8 | o[:, 2] << 16 | o[:, 3] << 24).astype(np.uint32))
         orders.append(rng.integers(0, 2**32, M * E, dtype=np.uint64).astype(np.uint32))   # mostly invalid
         orders.append(np.full(M * E, 0x03020100, np.uint32))                              # first bins
-        for o in orders:
-            do.zero()
-            bcc.check_dense_ex("panda", scene, de, M, 32, do, bcc.DeviceBuffer.from_array(o), dr)
-            assert np.array_equal(do.to_array(np.uint8, (M, E)), exp)
-            r = dr.to_array(np.uint8, (M, E))
-            assert np.all(r >= 1)
+        try:
+            for v in (0, 1):   # both compiled variants of the ordered kernel
+                bcc.debug_force_variant(v)
+                for o in orders:
+                    do.zero()
+                    bcc.check_dense_ex("panda", scene, de, M, 32, do, bcc.DeviceBuffer.from_array(o), dr)
+                    assert np.array_equal(do.to_array(np.uint8, (M, E)), exp), v
+                    r = dr.to_array(np.uint8, (M, E))
+                    assert np.all(r >= 1)
+        finally:
+            bcc.debug_force_variant(-1)
     n = np.array([oracle.steps(edges[m, 0], edges[m, 1], 32)[0] for m in range(M)])
     assert np.all(plain_rounds <= n[:, None]) and np.all(plain_rounds[plain == 0] == np.broadcast_to(
         n[:, None], plain.shape)[plain == 0])

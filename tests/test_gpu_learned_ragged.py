@@ -91,6 +91,7 @@ def test_ordered_ragged_plan_is_exact_for_any_order(bcc, oracle, gpu, robot):
         orders.append(model.order_pairs(scene, de, _pair_env(sc, off)).to_array(np.uint32, (P,)))
         for k, o in enumerate(orders):
             d_order = bcc.DeviceBuffer.from_array(o)
+            bcc.debug_force_variant(k % 2)   # both compiled variants of the ordered kernel
             for tile in (0, 1):
                 plan = bcc.RaggedPlan(robot, scene, off, de, do, tile_pairs=tile, d_order=d_order)
                 for _ in range(2):                    # graph replays
@@ -103,6 +104,7 @@ def test_ordered_ragged_plan_is_exact_for_any_order(bcc, oracle, gpu, robot):
                 bcc.synchronize()
                 assert np.array_equal(do.to_array(np.uint8, (P,)), exp), (k, tile)
                 plan.close()
+            bcc.debug_force_variant(-1)
 
 
 def test_mixed_group_of_ordered_plans_matches_plain_group(bcc, gpu):
