@@ -115,6 +115,7 @@ def lib():
     _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
     _sig(L, "bcc_debug_force_variant", I, [I])
     _sig(L, "bcc_debug_cull_min", I, [I])
+    _sig(L, "bcc_debug_dense_chunk", I, [I])
     _sig(L, "bcc_debug_half_boxes", I, [P, ctypes.c_uint64, P])
     _lib = L
     return L
@@ -424,6 +425,11 @@ class RaggedGroup:
 def debug_force_variant(variant: int):
     """-1: automatic kernel variant per launch; 0 / 1: force low / high occupancy."""
     _check(lib().bcc_debug_force_variant(int(variant)))
+
+
+def debug_dense_chunk(edges: int):
+    """Edges per workgroup of dense launches; 0 restores the automatic choice."""
+    _check(lib().bcc_debug_dense_chunk(int(edges)))
 
 
 def debug_cull_min(work: int):

@@ -380,8 +380,12 @@ int device_cus();
 // Edges per dense workgroup: 64, halved (down to 8) while the launch would
 // have fewer than 8 workgroups per CU, so small batches still fill the GPU
 // and end with a short tail.
+std::atomic<int> g_dense_chunk{0};   // > 0: forced edges per workgroup (tuning hook)
+
 int dense_chunk(int num_envs, int num_edges)
 {
+    if (const int forced = g_dense_chunk.load(); forced > 0)
+        return forced;
     const long long target = 8LL * device_cus();
     int chunk = 64;
     while (chunk > 8 && (long long)num_envs * ((num_edges + chunk - 1) / chunk) < target)
@@ -1233,6 +1237,14 @@ int bcc_check_ragged(int robot, const bcc_scene* scene, const int64_t* env_offse
     (void)hipFree(d_edges);
     (void)hipFree(d_out);
     return rc;
+}
+
+int bcc_debug_dense_chunk(int edges)
+{
+    if (edges < 0 || edges > 4096)
+        return fail(BCC_ERR_INVALID, "edges per workgroup must be in 0..4096");
+    g_dense_chunk.store(edges);
+    return BCC_OK;
 }
 
 int bcc_debug_cull_min(int work)

@@ -272,6 +272,11 @@ int bcc_debug_force_variant(int variant);
  * = never, -1 = the built-in tuned value).  Verdicts never depend on it. */
 int bcc_debug_cull_min(int work);
 
+/* Tuning hook (no reference counterpart): edges per workgroup of dense
+ * launches (0 = chosen per launch: 64, halved down to 8 while the launch has
+ * fewer than 8 workgroups per CU).  Verdicts never depend on it. */
+int bcc_debug_dense_chunk(int edges);
+
 /* Test hook (no reference counterpart): the culling step's half-wave boxes
  * for one wave.  d_in [64][4] = x y z padded-radius per lane; d_out [12] =
  * lo xyz, hi xyz of lanes 0-31 then of lanes 32-63 among the lanes in `mask`,

@@ -154,3 +154,12 @@ def test_force_variant_rejects_out_of_range(bcc):
     assert L.bcc_debug_force_variant(2) == 1
     assert b"variant" in L.bcc_last_error()
     assert L.bcc_debug_force_variant(-1) == 0
+
+
+def test_tuning_hooks_reject_out_of_range(bcc):
+    L = bcc.lib()
+    assert L.bcc_debug_dense_chunk(-1) == 1
+    assert b"edges per workgroup" in L.bcc_last_error()
+    assert L.bcc_debug_dense_chunk(5000) == 1
+    assert L.bcc_debug_dense_chunk(0) == 0
+    assert L.bcc_debug_cull_min(-1) == 0
