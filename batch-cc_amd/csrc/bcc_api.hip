@@ -353,7 +353,10 @@ int choose_variant(int robot, double mean_obstacles)
     const int forced = g_force_variant.load();
     if (forced >= 0)
         return forced;
-    return mean_obstacles >= (double)kHiVariantMinObstacles[robot] ? 1 : 0;
+    return (mean_obstacles >= (double)kHiVariantMinObstacles[robot] &&
+            mean_obstacles < (double)kHiVariantMaxObstacles[robot])
+               ? 1
+               : 0;
 }
 
 const bcc::VariantKnobs& knobs(int robot)

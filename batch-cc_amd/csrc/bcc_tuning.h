@@ -15,13 +15,14 @@ struct bcc_tune_entry {
 static const int kCullMinWork = 32;
 
 // Kernel variant per launch: the high-occupancy variant when the mean number
-// of tested obstacles per environment (weighted by pairs) is at least this,
-// per robot (panda, fetch, baxter).
-static const int kHiVariantMinObstacles[3] = {32, 7, 1073741824};
+// of tested obstacles per environment (weighted by pairs) is at least the
+// first value and below the second, per robot (panda, fetch, baxter).
+static const int kHiVariantMinObstacles[3] = {20, 0, 1073741824};
+static const int kHiVariantMaxObstacles[3] = {163, 1073741824, 1073741824};
 
 // source: tune.py on gfx950, 2026-12-04, seed 0
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 8, 6},   // panda: score 1.023
-    {1, 4, 8},   // fetch: score 1.000
+    {0, 8, 6},   // panda: score 1.015
+    {1, 4, 8},   // fetch: score 1.003
     {2, 16, 4},   // baxter: score 1.000
 };

@@ -6,11 +6,11 @@ Three settings, in this order:
    benchmark-shaped batches (see shaped_batches) are timed with scenes built
    under each candidate threshold; the candidate with the lowest sum of
    time / best time wins.
-2. Per robot, the kernel-variant threshold: the same batches are timed with
-   the high- and the low-occupancy variant forced; the threshold (a mean
-   obstacle count per environment from which a launch takes the high
-   variant) is the candidate with the lowest sum of time / best time when
-   each batch runs the variant the threshold picks for it.
+2. Per robot, the kernel-variant range: the same batches are timed with
+   the high- and the low-occupancy variant forced; the range of mean
+   obstacle counts per environment in which a launch takes the high variant
+   is the candidate with the lowest sum of time / best time when each batch
+   runs the variant the range picks for it.
 3. Per robot, the ragged launch shape: three seeded ragged batches (long
    pairs, few environments, short pairs: see batch()) are timed as replayed
    hipGraphs for every candidate (target tiles per wave, workgroups per CU);
@@ -170,10 +170,12 @@ def tune_cull(bcc, W, seed: int, quick: bool):
 
 
 def variant_threshold(bcc, W, robot: str, seed: int, quick: bool, cull_min: int):
-    """Mean obstacle count from which a launch of this robot takes the
-    high-occupancy variant: the candidate (one of the batches' means, or
-    never) minimising the sum of time / best time over the shaped batches
-    when each runs the variant the threshold picks.  Returns (threshold, rows)."""
+    """Range [lo, hi) of mean obstacle counts in which a launch of this robot
+    takes the high-occupancy variant: the pair of candidates (0, midpoints
+    between the batches' means, never) minimising the sum of time / best time over the shaped
+    batches when each runs the variant the range picks (with culling, very
+    many obstacles are cheap again and can favour the low-occupancy variant).
+    Returns ((lo, hi), rows)."""
     rows, batches = [], []
     bcc.debug_cull_min(cull_min)
     try:
@@ -196,12 +198,17 @@ def variant_threshold(bcc, W, robot: str, seed: int, quick: bool, cull_min: int)
     finally:
         bcc.debug_force_variant(-1)
         bcc.debug_cull_min(-1)
-    cands = sorted({int(np.ceil(m)) for m, _, _ in batches}) + [NEVER]
+    # bounds halfway between consecutive batch means (a bound at a batch's own
+    # mean would decide that batch by rounding), plus 0 and never
+    means = sorted(m for m, _, _ in batches)
+    cands = sorted({0, NEVER} | {int(round((a + b) / 2)) for a, b in zip(means, means[1:])})
 
-    def total(thr):
-        return sum((hi if m >= thr else lo) / min(lo, hi) for m, lo, hi in batches)
+    def total(lo_thr, hi_thr):
+        return sum((hi if lo_thr <= m < hi_thr else lo) / min(lo, hi) for m, lo, hi in batches)
 
-    return min(cands, key=lambda c: (total(c), -c)), rows
+    pairs = [(a, b) for a in cands for b in cands if a < b] + [(NEVER, NEVER)]
+    best = min(pairs, key=lambda p: (total(*p), -p[0], -p[1]))
+    return best, rows
 
 
 def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:
@@ -224,10 +231,12 @@ def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:
         f"static const int kCullMinWork = {cull_min};",
         "",
         "// Kernel variant per launch: the high-occupancy variant when the mean number",
-        "// of tested obstacles per environment (weighted by pairs) is at least this,",
-        "// per robot (panda, fetch, baxter).",
+        "// of tested obstacles per environment (weighted by pairs) is at least the",
+        "// first value and below the second, per robot (panda, fetch, baxter).",
         "static const int kHiVariantMinObstacles[3] = {"
-        + ", ".join(str(thresholds[r]) for r in ROBOTS) + "};",
+        + ", ".join(str(thresholds[r][0]) for r in ROBOTS) + "};",
+        "static const int kHiVariantMaxObstacles[3] = {"
+        + ", ".join(str(thresholds[r][1]) for r in ROBOTS) + "};",
         "",
         f"// source: {source}",
         "static const bcc_tune_entry kTuneTable[] = {",
@@ -259,8 +268,8 @@ def main():
     for robot in ROBOTS:
         thresholds[robot], rows = variant_threshold(bcc, W, robot, a.seed, a.quick, cull_min)
         timings += rows
-        print(f"[tune] {robot}: high-occupancy variant from {thresholds[robot]} obstacles per environment",
-              flush=True)
+        print(f"[tune] {robot}: high-occupancy variant for {thresholds[robot][0]} <= obstacles per environment "
+              f"< {thresholds[robot][1]}", flush=True)
     for robot in ROBOTS:
         scores = {}
         for kind in ("long", "few-envs", "short"):
@@ -272,7 +281,7 @@ def main():
             obs = sc.counts[:, [0, 1, 2, 4, 5]].sum(1).astype(np.float64)
             w = np.diff(off).astype(np.float64)
             mean = float((obs * w).sum() / max(w.sum(), 1.0))
-            bcc.debug_force_variant(1 if mean >= thresholds[robot] else 0)
+            bcc.debug_force_variant(1 if thresholds[robot][0] <= mean < thresholds[robot][1] else 0)
             bcc.debug_cull_min(cull_min)
             with bcc.Scene(sc) as scene:
                 de = bcc.DeviceBuffer.from_array(edges)

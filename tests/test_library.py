@@ -128,6 +128,8 @@ def test_tuning_table_covers_every_robot():
     assert all(1 <= int(t) <= 1024 and 1 <= int(g) <= 32 for _, t, g in rows)
     thr = re.search(r"kHiVariantMinObstacles\[3\] = \{(\d+), (\d+), (\d+)\}", text)
     assert thr and all(int(x) >= 0 for x in thr.groups())
+    hi = re.search(r"kHiVariantMaxObstacles\[3\] = \{(\d+), (\d+), (\d+)\}", text)
+    assert hi and all(int(b) > int(a) or int(a) == int(b) == 1 << 30 for a, b in zip(thr.groups(), hi.groups()))
     cull = re.search(r"kCullMinWork = (\d+);", text)
     assert cull and int(cull.group(1)) >= 0
 
