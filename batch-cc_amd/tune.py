@@ -198,8 +198,15 @@ def variant_threshold(bcc, W, robot: str, seed: int, quick: bool, cull_min: int)
     finally:
         bcc.debug_force_variant(-1)
         bcc.debug_cull_min(-1)
-    # bounds halfway between consecutive batch means (a bound at a batch's own
-    # mean would decide that batch by rounding), plus 0 and never
+    return pick_range(batches), rows
+
+
+def pick_range(batches) -> tuple:
+    """batches: (mean obstacles, ms low variant, ms high variant).  The range
+    [lo, hi) of mean obstacle counts that runs the high variant, minimising
+    the summed time / best time; bounds lie halfway between consecutive
+    batch means (a bound at a batch's own mean would decide that batch by
+    rounding), or are 0 / never; ties go to the larger bounds."""
     means = sorted(m for m, _, _ in batches)
     cands = sorted({0, NEVER} | {int(round((a + b) / 2)) for a, b in zip(means, means[1:])})
 
@@ -207,8 +214,7 @@ def variant_threshold(bcc, W, robot: str, seed: int, quick: bool, cull_min: int)
         return sum((hi if lo_thr <= m < hi_thr else lo) / min(lo, hi) for m, lo, hi in batches)
 
     pairs = [(a, b) for a in cands for b in cands if a < b] + [(NEVER, NEVER)]
-    best = min(pairs, key=lambda p: (total(*p), -p[0], -p[1]))
-    return best, rows
+    return min(pairs, key=lambda p: (total(*p), -p[0], -p[1]))
 
 
 def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:

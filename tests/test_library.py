@@ -165,3 +165,21 @@ def test_tuning_hooks_reject_out_of_range(bcc):
     assert L.bcc_debug_dense_chunk(5000) == 1
     assert L.bcc_debug_dense_chunk(0) == 0
     assert L.bcc_debug_cull_min(-1) == 0
+
+
+def test_tuner_variant_range_selection():
+    """tune.pick_range on the MI355X timings behind the committed table: panda
+    runs the high variant only between its few-obstacle and its c4-shaped
+    batch, fetch always, baxter never; a batch is never decided by a bound
+    placed at its own mean."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bcc_tune", os.path.join(REPO, "batch-cc_amd", "tune.py"))
+    tune = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(tune)
+    never = tune.NEVER
+    panda = [(32.0, 1.967, 1.686), (7.0, 2.992, 3.013), (293.5, 2.406, 2.596)]
+    assert tune.pick_range(panda) == (20, 163)
+    assert tune.pick_range([(34.4, 1.216, 0.991), (7.0, 2.290, 2.043), (246.8, 3.167, 2.778)]) == (0, never)
+    assert tune.pick_range([(145.6, 1.597, 1.647), (7.0, 3.087, 3.510), (279.8, 6.592, 8.432)]) == (never, never)
+    lo, hi = tune.pick_range(panda)
+    assert all(not (m == lo or m == hi) for m, _, _ in panda)
