@@ -225,8 +225,9 @@ def padded_radius(r: float) -> float:
 # (c1, c3, c2, c4's three parts, the README run, a dense baxter batch) over
 # gate chunks 1-8 and fine chunks 2-32; fine 16 takes all but the two largest
 # groups in one sweep.  Panda's (2, 4, 16) trades 5% on c3 for 16% on c4's
-# panda part; fetch's gate chunk 6 is best on c1 (c4's fetch part prefers 8).
-CHUNKS = {"panda": (2, 4, 16), "fetch": (1, 6, 16), "baxter": (1, 4, 16)}
+# panda part; fetch's gate chunk 5 is best on c1 and c4's fetch part (4 and 6
+# are 7-8% slower: a register-allocation optimum rather than a trend).
+CHUNKS = {"panda": (2, 4, 16), "fetch": (1, 5, 16), "baxter": (1, 4, 16)}
 # Sweeps below the culling threshold run plain counted loops over all
 # obstacles (a second copy of each sweep's code) or the culled loop with every
 # obstacle marked near.  The second copy pays for panda and fetch; for

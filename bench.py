@@ -40,7 +40,7 @@ METRIC = "edge-env checks/sec at bit-exact verdicts, learned-ordered vs plain, 1
 # (x2.000) + WRITE_SIZE (x1.000), the scales read off a plain copy kernel;
 # commands and raw counts in profiles/dense_kernel_fetch_c1.md.  Only the
 # default workload is measured; re-measure after kernel changes.
-MEASURED_TRAFFIC = {"c1": (2 * 5.100421e6 * 1024 + 1.2637156e7 * 1024,
+MEASURED_TRAFFIC = {"c1": (2 * 3.156683e6 * 1024 + 8.665456e6 * 1024,
                            "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/dense_kernel_fetch_c1.md")}
 
 
