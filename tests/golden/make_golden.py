@@ -13,8 +13,16 @@ fk_<robot>.json
     sphere to the wrong frame or local offset fails the comparison).
 
 structure_<robot>.json
-    Counts of the reference's check program read from the source text:
-    environment and self-collision calls at each nesting level.
+    The reference's check program read from the source text: counts of the
+    environment and self-collision calls at each nesting level, and every
+    group's membership (env_groups: [gate, [fine spheres]]; self_groups:
+    [[gate a, gate b], [[fine a, fine b], ...]]) in the reference's order.
+    The membership comes from scan_program() below, a comment-stripped
+    brace-nesting scan of the call sites that shares nothing with
+    extract_model.py's parser: a call is a group's gate when an `if` on it
+    opens a block holding further calls, and spheres are numbered by first
+    appearance of their argument text (gate before fine, a before b), the
+    numbering the committed models use.
 
 limits.json
     Joint limits (s_a, s_m) per robot as written in src/Robots.hh:17-166.
@@ -40,6 +48,76 @@ def load_extractor():
     sys.modules["extract_model"] = mod
     spec.loader.exec_module(mod)
     return mod
+
+
+# fkcc<Robot> bodies in src/utils.cuh (Panda :342-5442, Fetch :10382-24992,
+# Baxter :24994-38241)
+FKCC_LINES = {"panda": (342, 5442), "fetch": (10382, 24992), "baxter": (24994, 38241)}
+_TOKEN = re.compile(r"\b(sphere_environment_in_collision|sphere_sphere_self_collision)\s*\(|[{}]")
+
+
+def _strip_comments(text: str) -> str:
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    return re.sub(r"//[^\n]*", "", text)
+
+
+def _close_paren(text: str, i: int) -> int:
+    depth = 0
+    for k in range(i, len(text)):
+        if text[k] == "(":
+            depth += 1
+        elif text[k] == ")":
+            depth -= 1
+            if depth == 0:
+                return k
+    raise ValueError("unbalanced parentheses")
+
+
+def scan_program(text: str) -> dict:
+    """Group membership of one fkcc body, from brace nesting alone."""
+    t = _strip_comments(text)
+    stack = []            # per open block: index of the call whose `if` opened it, or None
+    calls = []            # (kind, argument texts, parent call index or None)
+    opener = None
+    pos = 0
+    while (m := _TOKEN.search(t, pos)):
+        tok = m.group(0)
+        pos = m.end()
+        if tok == "{":
+            stack.append(opener)
+            opener = None
+            continue
+        if tok == "}":
+            stack.pop()
+            continue
+        kind = "env" if m.group(1) == "sphere_environment_in_collision" else "self"
+        op = m.end() - 1
+        cp = _close_paren(t, op)
+        args = [re.sub(r"\s+", "", a) for a in t[op + 1:cp].split(",")]
+        parent = next((o for o in reversed(stack) if o is not None), None)
+        calls.append((kind, args, parent))
+        opener = len(calls) - 1 if re.match(r"\s*\)\s*\{", t[cp + 1:]) else None
+        pos = cp + 1
+    if stack:
+        raise ValueError("unbalanced braces")
+    index: dict = {}
+
+    def sid(key):
+        return index.setdefault(tuple(key), len(index))
+
+    env, self_ = [], []
+    for ci, (kind, args, parent) in enumerate(calls):
+        if parent is not None:
+            continue
+        kids = [k for k, c in enumerate(calls) if c[2] == ci]
+        if any(c[2] in kids for c in calls):
+            raise ValueError("check program nested deeper than gate -> fine")
+        if kind == "env":
+            env.append([sid(args[1:5]), [sid(calls[k][1][1:5]) for k in kids]])
+        else:
+            self_.append([[sid(args[0:4]), sid(args[4:8])],
+                          [[sid(calls[k][1][0:4]), sid(calls[k][1][4:8])] for k in kids]])
+    return {"env_groups": env, "self_groups": self_, "num_spheres": len(index)}
 
 
 def main(ref_root="/root/reference"):
@@ -87,9 +165,12 @@ def main(ref_root="/root/reference"):
         raw_env = len(re.findall(r"sphere_environment_in_collision\(",
                                  em.function_body(src, robot)))
         raw_self = len(re.findall(r"sphere_sphere_self_collision\(", em.function_body(src, robot)))
+        a, b = FKCC_LINES[robot]
+        groups = scan_program("\n".join(src.split("\n")[a - 1:b]))
         with open(os.path.join(HERE, f"structure_{robot}.json"), "w") as f:
             json.dump({"env_top": env_top, "env_fine": env_fine, "self_top": self_top, "self_fine": self_fine,
-                       "env_calls_in_text": raw_env, "self_calls_in_text": raw_self}, f, indent=1)
+                       "env_calls_in_text": raw_env, "self_calls_in_text": raw_self,
+                       "source": f"src/utils.cuh:{a}-{b}, brace-nesting scan (scan_program)", **groups}, f)
         print(robot, len(spheres), "spheres;", env_top, env_fine, self_top, self_fine)
 
     # joint limits straight from the text of src/Robots.hh

@@ -100,15 +100,76 @@ def test_panda_joint_axes_match_published_dh(oracle):
             assert np.linalg.norm(v - np.dot(v, z) * z) < 1e-9, j
 
 
+def _model(robot):
+    return json.load(open(os.path.join(os.path.dirname(HERE), "batch-cc_amd", "robots", f"{robot}.json")))
+
+
+def _structure_mismatch(m, s):
+    """First difference between a robot model's check program and the
+    membership scanned from the reference text (structure_<robot>.json), or
+    None.  Compares every gate, every fine sphere and self pair in order, and
+    the lone flags (a gate with no nested calls decides by itself)."""
+    if len(m["spheres"]) != s["num_spheres"]:
+        return f"{len(m['spheres'])} spheres vs {s['num_spheres']}"
+    env = [[g["gate"], g["fine"]] for g in m["env_program"]]
+    if len(env) != len(s["env_groups"]):
+        return f"{len(env)} env groups vs {len(s['env_groups'])}"
+    for i, (got, exp) in enumerate(zip(env, s["env_groups"])):
+        if got != exp:
+            return f"env group {i}: {got} vs {exp}"
+        if m["env_program"][i]["lone"] != (not exp[1]):
+            return f"env group {i}: lone flag"
+    self_ = [[list(g["gate"]), [list(p) for p in g["fine"]]] for g in m["self_program"]]
+    if len(self_) != len(s["self_groups"]):
+        return f"{len(self_)} self groups vs {len(s['self_groups'])}"
+    for i, (got, exp) in enumerate(zip(self_, s["self_groups"])):
+        if got != exp:
+            return f"self group {i}: {got} vs {exp}"
+        if m["self_program"][i]["lone"] != (not exp[1]):
+            return f"self group {i}: lone flag"
+    return None
+
+
 @pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
 def test_check_program_structure_matches_reference_text(bcc, robot):
     s = load(f"structure_{robot}.json")
-    m = json.load(open(os.path.join(os.path.dirname(HERE), "batch-cc_amd", "robots", f"{robot}.json")))
+    m = _model(robot)
     assert len(m["env_program"]) == s["env_top"]
     assert sum(len(g["fine"]) for g in m["env_program"]) == s["env_fine"]
     assert len(m["self_program"]) == s["self_top"]
     assert sum(len(g["fine"]) for g in m["self_program"]) == s["self_fine"]
     assert len(m["program"]) == s["env_top"] + s["self_top"]
+    # membership of every group, in order (scanned independently of the
+    # extractor that wrote the model)
+    assert _structure_mismatch(m, s) is None, _structure_mismatch(m, s)
+
+
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+def test_structure_check_catches_a_moved_fine_sphere(robot):
+    """A plausible extractor slip — a fine sphere or self pair attached to the
+    neighbouring gate — must fail the membership comparison."""
+    import copy
+    s = load(f"structure_{robot}.json")
+    m = _model(robot)
+    gated = [i for i, g in enumerate(m["env_program"]) if g["fine"]]
+    for a, b in zip(gated, gated[1:]):
+        bad = copy.deepcopy(m)
+        bad["env_program"][b]["fine"].insert(0, bad["env_program"][a]["fine"].pop())
+        assert _structure_mismatch(bad, s) is not None
+    sgated = [i for i, g in enumerate(m["self_program"]) if g["fine"]]
+    for a, b in list(zip(sgated, sgated[1:]))[:50]:
+        bad = copy.deepcopy(m)
+        bad["self_program"][b]["fine"].append(bad["self_program"][a]["fine"].pop())
+        assert _structure_mismatch(bad, s) is not None
+    # swapping a gate's two spheres, or two fine spheres within a group, too
+    bad = copy.deepcopy(m)
+    g = bad["env_program"][gated[0]]
+    if len(g["fine"]) > 1:
+        g["fine"][0], g["fine"][1] = g["fine"][1], g["fine"][0]
+        assert _structure_mismatch(bad, s) is not None
+    bad = copy.deepcopy(m)
+    bad["self_program"][0]["gate"] = bad["self_program"][0]["gate"][::-1]
+    assert _structure_mismatch(bad, s) is not None
 
 
 def test_joint_limits_match_reference_text(bcc):
