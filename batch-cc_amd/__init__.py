@@ -115,6 +115,7 @@ def lib():
     _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
     _sig(L, "bcc_debug_force_variant", I, [I])
     _sig(L, "bcc_debug_cull_min", I, [I])
+    _sig(L, "bcc_debug_scene_cull_min", I, [P, ctypes.POINTER(I)])
     _sig(L, "bcc_debug_dense_chunk", I, [I])
     _sig(L, "bcc_debug_half_boxes", I, [P, ctypes.c_uint64, P])
     _lib = L
@@ -196,6 +197,13 @@ class Scene:
     @property
     def num_envs(self) -> int:
         return lib().bcc_scene_num_envs(self.handle)
+
+    @property
+    def cull_min(self) -> int:
+        """Culling threshold stamped at creation (0: every sweep culls)."""
+        v = ctypes.c_int()
+        _check(lib().bcc_debug_scene_cull_min(self.handle, ctypes.byref(v)))
+        return v.value
 
     def close(self):
         if self.handle:
@@ -347,6 +355,20 @@ class RaggedPlan:
         self.robot = robot
         self.scene, self.d_edges, self.d_out, self.d_order = scene, d_edges, d_out, d_order
         self.handle = ctypes.c_void_p()
+        # the library reads env_off[num_envs] on the host and the kernels
+        # index the device buffers by pair: check both before creating it
+        if self.off.ndim != 1 or self.off.size != scene.num_envs + 1:
+            raise BCCError("env_off must have num_envs + 1 entries")
+        if self.off[0] != 0 or np.any(np.diff(self.off) < 0):
+            raise BCCError("env_off must start at 0 and be non-decreasing")
+        total = int(self.off[-1])
+        dim = lib().bcc_robot_dimension(robot_id(robot))
+        if d_edges.nbytes < total * 2 * dim * 4:
+            raise BCCError(f"d_edges holds {d_edges.nbytes} bytes, {total} pairs need {total * 2 * dim * 4}")
+        if d_out.nbytes < total:
+            raise BCCError(f"d_out holds {d_out.nbytes} bytes, {total} pairs need {total}")
+        if d_order is not None and d_order.nbytes < 4 * total:
+            raise BCCError(f"d_order holds {d_order.nbytes} bytes, {total} pairs need {4 * total}")
         if d_order is None:
             _check(lib().bcc_ragged_plan_create_tuned(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
                                                       int(resolution), d_out.ptr, int(tile_pairs),

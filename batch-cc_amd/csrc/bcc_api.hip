@@ -250,6 +250,8 @@ int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene**
                 record_box(k, &L[k][i], L[6]);
     }
     std::vector<int> meta((size_t)num_envs * bcc::kEnvMetaInts, 0);
+    const int cull_set = g_cull_min.load();
+    const int cull_min = cull_set >= 0 ? cull_set : kCullMinWork;
     size_t total = 0;
     auto align16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
     for (int e = 0; e < num_envs; ++e) {
@@ -262,8 +264,7 @@ int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene**
             meta[(size_t)e * bcc::kEnvMetaInts + 2 * k + 1] = (int)(l.size() / stride[k]);
             total += l.size();
         }
-        const int cull_min = g_cull_min.load();
-        meta[(size_t)e * bcc::kEnvMetaInts + 14] = cull_min >= 0 ? cull_min : kCullMinWork;
+        meta[(size_t)e * bcc::kEnvMetaInts + 14] = cull_min;
     }
     total = align16(total) + 16;
     std::vector<float> host(total, 0.0f);
@@ -299,6 +300,7 @@ int build_scene(int num_envs, const std::vector<bcc_env_desc>& envs, bcc_scene**
     bcc_scene* s = new bcc_scene;
     s->num_envs = num_envs;
     s->obs_floats = total;
+    s->cull_min = cull_min;
     s->obstacles.resize(num_envs);
     for (int e = 0; e < num_envs; ++e) {
         const bcc_env_desc& d = envs[e];
@@ -396,10 +398,24 @@ int dense_chunk(int num_envs, int num_edges)
     return chunk;
 }
 
+// Edges per workgroup of the learned-order kernel: its probe pass checks
+// 64 / kProbe1 edges per wave at a time, so it wants larger chunks; 512,
+// halved down to 32 while the launch has fewer than 8 workgroups per CU.
+int ordered_chunk(int num_envs, int num_edges)
+{
+    if (const int forced = g_dense_chunk.load(); forced > 0)
+        return std::min(forced, bcc::kOrderedChunkMax);
+    const long long target = 8LL * device_cus();
+    int chunk = 512;
+    while (chunk > 32 && (long long)num_envs * ((num_edges + chunk - 1) / chunk) < target)
+        chunk /= 2;
+    return chunk;
+}
+
 int dispatch_dense(int robot, const bcc_scene* scene, bcc::DenseArgs a, hipStream_t s)
 {
     a.variant = choose_variant(robot, mean_obstacles(scene));
-    a.chunk = dense_chunk(a.num_envs, a.num_edges);
+    a.chunk = a.order ? ordered_chunk(a.num_envs, a.num_edges) : dense_chunk(a.num_envs, a.num_edges);
     hipError_t e;
     switch (robot) {
     case BCC_PANDA: e = bcc::launch_dense_panda(a, s); break;
@@ -841,11 +857,19 @@ static int work_common(int robot, const bcc_scene* scene, const float* d_edges, 
 
 }  // extern "C"
 
+// A plan holds one batch's tile list, buffers and captured graph.  Two
+// kinds:
+//   plain    the ragged kernel over every tile;
+//   ordered  a learned order given for every pair: learn_finish_kernel over
+//            segments that are the tiles themselves (identity queue).
+enum PlanKind { kPlanPlain = 0, kPlanOrdered = 1 };
+
 struct bcc_ragged_plan {
     int robot = 0;
+    int kind = kPlanPlain;
     const bcc_scene* scene = nullptr;
     bcc::RaggedTile* d_tiles = nullptr;
-    int* d_counter = nullptr;
+    int* d_counter = nullptr;    // plain: the tile counter; ordered: kCtrInts counters
     int num_tiles = 0;
     int64_t num_pairs = 0;
     int tile_pairs = 0;
@@ -853,6 +877,11 @@ struct bcc_ragged_plan {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bcc::RaggedArgs args{};
+    // ordered plans
+    bcc::LearnArgs learn{};
+    int preset_segs = 0;              // segments in the queue: every tile
+    int* d_queue_idx = nullptr;       // [num_pairs] identity
+    bcc::QueueSeg* d_segs = nullptr;  // [num_tiles]
 };
 
 struct bcc_ragged_group {
@@ -873,6 +902,20 @@ int dispatch_ragged(int robot, const bcc::RaggedArgs& a, hipStream_t s)
     }
     if (e != hipSuccess)
         return hip_fail(e, "ragged kernel launch");
+    return BCC_OK;
+}
+
+int dispatch_learn(int robot, const bcc::LearnArgs& a, hipStream_t s)
+{
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_learn_panda(a, s); break;
+    case BCC_FETCH: e = bcc::launch_learn_fetch(a, s); break;
+    case BCC_BAXTER: e = bcc::launch_learn_baxter(a, s); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "learned-order kernel launch");
     return BCC_OK;
 }
 
@@ -908,6 +951,20 @@ int device_cus()
     return cus;
 }
 
+// Every launch of a plan (captured once into its graph, or run eagerly).
+int run_plan(const bcc_ragged_plan* p, hipStream_t s)
+{
+    if (p->kind == kPlanPlain)
+        return dispatch_ragged(p->robot, p->args, s);
+    hipLaunchKernelGGL(bcc::reset_pipeline_kernel, dim3(1), dim3(64), 0, s, p->d_counter, p->preset_segs);
+    if (hipError_t e = hipGetLastError(); e != hipSuccess)
+        return hip_fail(e, "plan counter reset");
+    if (p->num_tiles == 0)
+        return BCC_OK;
+    return dispatch_learn(p->robot, p->learn, s);
+}
+
+
 }  // namespace
 
 extern "C" {
@@ -922,6 +979,8 @@ void bcc_ragged_plan_destroy_impl(bcc_ragged_plan* p)
         (void)hipGraphDestroy(p->graph);
     (void)hipFree(p->d_tiles);
     (void)hipFree(p->d_counter);
+    (void)hipFree(p->d_queue_idx);
+    (void)hipFree(p->d_segs);
     delete p;
 }
 
@@ -936,85 +995,26 @@ int bcc_ragged_plan_create(int robot, const bcc_scene* scene, const int64_t* env
 
 namespace {
 
-// Plan over one CSR layout; `ordered` selects the ordered kernel (d_order:
-// one word per pair, on the device); the kernel variant is chosen as for plain plans.
-int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* d_edges, int resolution,
-                bool ordered, const uint32_t* d_order, uint8_t* d_results, int tile_pairs, int groups_per_cu,
-                bcc_ragged_plan** out)
-{
-    if (!out)
-        return fail(BCC_ERR_INVALID, "out is NULL");
-    *out = nullptr;
-    if (int rc = check_dense_args(robot, scene, d_edges, 0, resolution, d_results))
-        return rc;
-    if (!env_offsets)
-        return fail(BCC_ERR_INVALID, "env_offsets is NULL");
-    const int E = scene->num_envs;
-    if (env_offsets[0] != 0)
-        return fail(BCC_ERR_INVALID, "env_offsets[0] must be 0");
-    for (int e = 0; e < E; ++e)
-        if (env_offsets[e + 1] < env_offsets[e])
-            return fail(BCC_ERR_INVALID, "env_offsets must be non-decreasing");
-    const int64_t total = env_offsets[E];
-    if (total > 0x7fffffffLL)
-        return fail(BCC_ERR_INVALID, "more than 2^31 pairs in one ragged batch");
-    if (total > 0 && (!d_edges || !d_results))
-        return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
-    if (ordered && total > 0 && !d_order)
-        return fail(BCC_ERR_INVALID, "d_order is NULL");
-    if (tile_pairs < -1024 || groups_per_cu < 0 || groups_per_cu > 32)
-        return fail(BCC_ERR_INVALID, "tile_pairs must be >= -1024 and groups_per_cu in 0..32");
-    double weighted = 0.0;   // mean obstacles per environment, weighted by pairs
-    for (int e = 0; e < E; ++e)
-        weighted += (double)scene->obstacles[e] * (double)(env_offsets[e + 1] - env_offsets[e]);
-    const int variant = choose_variant(robot, total > 0 ? weighted / (double)total : 0.0);
-    const bcc_tune_entry& tune = tuned(robot);
-    if (groups_per_cu == 0)
-        groups_per_cu = tune.groups_per_cu;
-    if (tile_pairs <= 0) {
-        // waves that can be resident at once: the grid, capped by the variant's occupancy
-        const bcc::VariantKnobs& k = knobs(robot);
-        const int resident = std::min(groups_per_cu, variant ? k.hi_waves : k.lo_waves);
-        const int per_wave = tile_pairs < 0 ? -tile_pairs : tune.tiles_per_wave;
-        tile_pairs = tile_for(total, device_cus() * resident * (bcc::kBlock / 64), per_wave);
-    }
+// Batch description shared by the plan kinds: tiles (env, first, end) over
+// an index range, the verdict slot of index i in environment env, and the
+// pair-weighted mean obstacle count that picks the kernel variant.
+struct PlanBatch {
     std::vector<bcc::RaggedTile> tiles;
-    for (int e = 0; e < E; ++e)
-        for (int64_t b = env_offsets[e]; b < env_offsets[e + 1]; b += tile_pairs)
-            tiles.push_back(bcc::RaggedTile{e, (int)b, (int)std::min<int64_t>(b + tile_pairs, env_offsets[e + 1]), 0});
-    // heavy environments first: more obstacles, then larger tiles
-    std::stable_sort(tiles.begin(), tiles.end(), [&](const bcc::RaggedTile& x, const bcc::RaggedTile& y) {
-        const int64_t wx = (int64_t)scene->obstacles[x.env] * (x.end - x.begin);
-        const int64_t wy = (int64_t)scene->obstacles[y.env] * (y.end - y.begin);
-        return wx > wy;
-    });
-    bcc_ragged_plan* p = new bcc_ragged_plan;
-    p->robot = robot;
-    p->scene = scene;
-    p->num_tiles = (int)tiles.size();
-    p->num_pairs = total;
-    p->tile_pairs = tile_pairs;
-    p->groups_per_cu = groups_per_cu;
-    hipError_t err = hipMalloc((void**)&p->d_tiles, sizeof(bcc::RaggedTile) * std::max<size_t>(tiles.size(), 1));
-    if (err == hipSuccess)
-        err = hipMalloc((void**)&p->d_counter, sizeof(int));
-    if (err == hipSuccess && !tiles.empty())
-        err = hipMemcpy(p->d_tiles, tiles.data(), sizeof(bcc::RaggedTile) * tiles.size(), hipMemcpyHostToDevice);
-    if (err != hipSuccess) {
-        bcc_ragged_plan_destroy_impl(p);
-        return hip_fail(err, "ragged plan upload");
-    }
-    p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
-                              resolution, device_cus() * groups_per_cu, d_results};
-    p->args.variant = variant;
-    p->args.order = d_order;
-    // capture reset + persistent kernel once; launches replay the graph
+    bcc::PairSlots slots{1, 0};
+    int64_t total = 0;
+    double mean_obstacles = 0.0;
+};
+
+// Captures the plan's launch sequence into its graph (the plan's buffers
+// must be set); on failure destroys the plan.
+int capture_plan(bcc_ragged_plan* p, bcc_ragged_plan** out)
+{
     hipStream_t cap;
-    err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+    hipError_t err = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
     if (err == hipSuccess) {
         err = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
         if (err == hipSuccess) {
-            const int rc = dispatch_ragged(robot, p->args, cap);
+            const int rc = run_plan(p, cap);
             hipGraph_t g = nullptr;
             const hipError_t e2 = hipStreamEndCapture(cap, &g);
             p->graph = g;
@@ -1031,10 +1031,153 @@ int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, c
     }
     if (err != hipSuccess) {
         bcc_ragged_plan_destroy_impl(p);
-        return hip_fail(err, "ragged graph capture");
+        return hip_fail(err, "plan graph capture");
     }
     *out = p;
     return BCC_OK;
+}
+
+// Plan of either kind over a batch.  d_order: ordered plans, one word per
+// pair (index i).
+int create_plan_kind(int robot, const bcc_scene* scene, PlanBatch& b, const float* d_edges, int resolution,
+                     int kind, const uint32_t* d_order, uint8_t* d_results, int tile_pairs, int groups_per_cu,
+                     bcc_ragged_plan** out)
+{
+    const int variant = choose_variant(robot, b.mean_obstacles);
+    const bcc_tune_entry& tune = tuned(robot);
+    if (groups_per_cu == 0)
+        groups_per_cu = tune.groups_per_cu;
+    bcc_ragged_plan* p = new bcc_ragged_plan;
+    p->robot = robot;
+    p->kind = kind;
+    p->scene = scene;
+    p->num_tiles = (int)b.tiles.size();
+    p->num_pairs = b.total;
+    p->tile_pairs = tile_pairs;
+    p->groups_per_cu = groups_per_cu;
+    const size_t n = (size_t)std::max<int64_t>(b.total, 1);
+    const size_t nt = std::max<size_t>(b.tiles.size(), 1);
+    hipError_t err = hipMalloc((void**)&p->d_tiles, sizeof(bcc::RaggedTile) * nt);
+    if (err == hipSuccess)
+        err = hipMalloc((void**)&p->d_counter, sizeof(int) * bcc::kCtrInts);
+    if (err == hipSuccess && !b.tiles.empty())
+        err = hipMemcpy(p->d_tiles, b.tiles.data(), sizeof(bcc::RaggedTile) * b.tiles.size(), hipMemcpyHostToDevice);
+    if (err == hipSuccess && kind != kPlanPlain) {
+        err = hipMalloc((void**)&p->d_queue_idx, sizeof(int) * n);
+        if (err == hipSuccess)
+            err = hipMalloc((void**)&p->d_segs, sizeof(bcc::QueueSeg) * nt);
+    }
+    if (err == hipSuccess && kind == kPlanOrdered) {
+        // every tile is a segment of an identity queue
+        std::vector<int> idx(n, 0);
+        std::vector<bcc::QueueSeg> segs(b.tiles.size());
+        size_t q = 0;
+        for (size_t t = 0; t < b.tiles.size(); ++t) {
+            segs[t] = bcc::QueueSeg{b.tiles[t].env, (int)q, b.tiles[t].end - b.tiles[t].begin, 0};
+            for (int i = b.tiles[t].begin; i < b.tiles[t].end; ++i)
+                idx[q++] = i;
+        }
+        p->preset_segs = (int)segs.size();
+        err = hipMemcpy(p->d_queue_idx, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice);
+        if (err == hipSuccess && !segs.empty())
+            err = hipMemcpy(p->d_segs, segs.data(), sizeof(bcc::QueueSeg) * segs.size(), hipMemcpyHostToDevice);
+    }
+    if (err != hipSuccess) {
+        bcc_ragged_plan_destroy_impl(p);
+        return hip_fail(err, "plan upload");
+    }
+    const int grid = device_cus() * groups_per_cu;
+    if (kind == kPlanPlain) {
+        p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
+                                  resolution, grid, d_results};
+        p->args.variant = variant;
+    } else {
+        bcc::LearnArgs& L = p->learn;
+        L.obs = scene->d_obs;
+        L.meta = scene->d_meta;
+        L.edges = d_edges;
+        L.ctr = p->d_counter;
+        L.resolution = resolution;
+        L.grid = grid;
+        L.variant = variant;
+        L.slots = b.slots;
+        L.out = d_results;
+        L.queue_idx = p->d_queue_idx;
+        L.segs = p->d_segs;
+        L.order = d_order;
+    }
+    return capture_plan(p, out);
+}
+
+// Tiles of a CSR batch: env e owns pairs [env_offsets[e], env_offsets[e+1]).
+int ragged_batch(const bcc_scene* scene, const int64_t* env_offsets, int robot, int tile_pairs, int groups_per_cu,
+                 int max_tile, PlanBatch& b, int* tile_out)
+{
+    if (!env_offsets)
+        return fail(BCC_ERR_INVALID, "env_offsets is NULL");
+    const int E = scene->num_envs;
+    if (env_offsets[0] != 0)
+        return fail(BCC_ERR_INVALID, "env_offsets[0] must be 0");
+    for (int e = 0; e < E; ++e)
+        if (env_offsets[e + 1] < env_offsets[e])
+            return fail(BCC_ERR_INVALID, "env_offsets must be non-decreasing");
+    const int64_t total = env_offsets[E];
+    if (total > 0x7fffffffLL)
+        return fail(BCC_ERR_INVALID, "more than 2^31 pairs in one ragged batch");
+    if (tile_pairs < -1024 || groups_per_cu < 0 || groups_per_cu > 32)
+        return fail(BCC_ERR_INVALID, "tile_pairs must be >= -1024 and groups_per_cu in 0..32");
+    double weighted = 0.0;   // mean obstacles per environment, weighted by pairs
+    for (int e = 0; e < E; ++e)
+        weighted += (double)scene->obstacles[e] * (double)(env_offsets[e + 1] - env_offsets[e]);
+    b.total = total;
+    b.mean_obstacles = total > 0 ? weighted / (double)total : 0.0;
+    b.slots = bcc::PairSlots{1, 0};
+    const int variant = choose_variant(robot, b.mean_obstacles);
+    const bcc_tune_entry& tune = tuned(robot);
+    if (groups_per_cu == 0)
+        groups_per_cu = tune.groups_per_cu;
+    if (tile_pairs <= 0) {
+        // waves that can be resident at once: the grid, capped by the variant's occupancy
+        const bcc::VariantKnobs& k = knobs(robot);
+        const int resident = std::min(groups_per_cu, variant ? k.hi_waves : k.lo_waves);
+        const int per_wave = tile_pairs < 0 ? -tile_pairs : tune.tiles_per_wave;
+        tile_pairs = tile_for(total, device_cus() * resident * (bcc::kBlock / 64), per_wave);
+    }
+    tile_pairs = std::min(tile_pairs, max_tile);
+    for (int e = 0; e < E; ++e)
+        for (int64_t i = env_offsets[e]; i < env_offsets[e + 1]; i += tile_pairs)
+            b.tiles.push_back(bcc::RaggedTile{e, (int)i, (int)std::min<int64_t>(i + tile_pairs, env_offsets[e + 1]), 0});
+    // heavy environments first: more obstacles, then larger tiles
+    std::stable_sort(b.tiles.begin(), b.tiles.end(), [&](const bcc::RaggedTile& x, const bcc::RaggedTile& y) {
+        const int64_t wx = (int64_t)scene->obstacles[x.env] * (x.end - x.begin);
+        const int64_t wy = (int64_t)scene->obstacles[y.env] * (y.end - y.begin);
+        return wx > wy;
+    });
+    *tile_out = tile_pairs;
+    return BCC_OK;
+}
+
+// Plain or ordered plan over one CSR layout.
+int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* d_edges, int resolution,
+                bool ordered, const uint32_t* d_order, uint8_t* d_results, int tile_pairs, int groups_per_cu,
+                bcc_ragged_plan** out)
+{
+    if (!out)
+        return fail(BCC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (int rc = check_dense_args(robot, scene, d_edges, 0, resolution, d_results))
+        return rc;
+    PlanBatch b;
+    // the pipeline kernels queue a tile's undecided pairs in LDS
+    const int max_tile = ordered ? bcc::kOrderedTileMax : 1 << 30;
+    if (int rc = ragged_batch(scene, env_offsets, robot, tile_pairs, groups_per_cu, max_tile, b, &tile_pairs))
+        return rc;
+    if (b.total > 0 && (!d_edges || !d_results))
+        return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
+    if (ordered && b.total > 0 && !d_order)
+        return fail(BCC_ERR_INVALID, "d_order is NULL");
+    return create_plan_kind(robot, scene, b, d_edges, resolution, ordered ? kPlanOrdered : kPlanPlain, d_order,
+                            d_results, tile_pairs, groups_per_cu, out);
 }
 
 }  // namespace
@@ -1069,7 +1212,7 @@ int bcc_ragged_plan_launch_eager(bcc_ragged_plan* plan, void* stream)
 {
     if (!plan)
         return fail(BCC_ERR_INVALID, "plan is NULL");
-    return dispatch_ragged(plan->robot, plan->args, (hipStream_t)stream);
+    return run_plan(plan, (hipStream_t)stream);
 }
 
 int bcc_ragged_plan_num_tiles(const bcc_ragged_plan* plan)
@@ -1253,6 +1396,14 @@ int bcc_debug_dense_chunk(int edges)
 int bcc_debug_cull_min(int work)
 {
     g_cull_min.store(work < 0 ? -1 : work);
+    return BCC_OK;
+}
+
+int bcc_debug_scene_cull_min(const bcc_scene* scene, int* out)
+{
+    if (!scene || !out)
+        return fail(BCC_ERR_INVALID, "null argument");
+    *out = scene->cull_min;
     return BCC_OK;
 }
 

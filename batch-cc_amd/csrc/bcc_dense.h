@@ -59,58 +59,6 @@ __device__ __forceinline__ int setup_pair(const float* __restrict__ se, int reso
 // kinematics and the self program dominate and want registers); the host
 // picks one per launch from the scene's mean obstacle count.
 
-// Sample of round r, lane t under a learned order.  Round 0 is plain step 0
-// (the first sample of every bin, i.e. one sample every 1/31 of the edge);
-// then come the remaining samples of the four predicted bins (top, one byte
-// per rank), bin by bin, then the remaining samples of the other bins in
-// plain step order.  Every one of the 32 n samples appears exactly once over
-// the n rounds, each computed exactly as in plain order: start + delta *
-// (bin * n) (fmaf), then i additions of delta.  An order that does not name
-// four distinct bins below 32 (top = 0xffffffff after validation) means
-// plain order.
-template <int D>
-__device__ __forceinline__ void ordered_sample(const float (&s)[D], const float (&d)[D], int n, uint32_t top,
-                                               int r, int t, float (&q)[D])
-{
-    int bin, i;
-    if (top == 0xffffffffu || r == 0) {
-        bin = t;
-        i = r;
-    } else {
-        const int j = 32 * (r - 1) + t;
-        const int a = 4 * (n - 1);
-        if (j < a) {
-            const int k = j / (n - 1);
-            bin = (top >> (8 * k)) & 31;
-            i = 1 + j - k * (n - 1);
-        } else {
-            const int j2 = j - a;
-            i = 1 + j2 / 28;
-            int m = j2 - (i - 1) * 28;   // m-th bin outside the top four, ascending
-            const uint32_t taken = (1u << (top & 31)) | (1u << ((top >> 8) & 31)) | (1u << ((top >> 16) & 31)) |
-                                   (1u << ((top >> 24) & 31));
-            bin = 0;
-            for (int b = 0; b < 32; ++b) {
-                if ((taken >> b) & 1u)
-                    continue;
-                if (m == 0) {
-                    bin = b;
-                    break;
-                }
-                --m;
-            }
-        }
-    }
-    const float bn = (float)(bin * n);
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-        float c = fmaf(d[j], bn, s[j]);
-        for (int k = 0; k < i; ++k)
-            c = c + d[j];
-        q[j] = c;
-    }
-}
-
 template <int D>
 __device__ __forceinline__ int setup_pair_sd(const float* __restrict__ se, int resolution, float (&s)[D],
                                              float (&d)[D])
@@ -134,8 +82,9 @@ __device__ __forceinline__ int setup_pair_sd(const float* __restrict__ se, int r
     return n;
 }
 
-// A learned order word names four distinct bins below 32, one byte each;
-// anything else means plain order (0xffffffff).
+// A learned order word names four distinct bins below 32, one byte each, the
+// most likely to hold a colliding sample first; anything else means no
+// prediction (0xffffffff).
 __device__ __forceinline__ uint32_t valid_order(uint32_t top)
 {
     const uint32_t taken = (1u << (top & 31)) | (1u << ((top >> 8) & 31)) | (1u << ((top >> 16) & 31)) |
@@ -143,18 +92,46 @@ __device__ __forceinline__ uint32_t valid_order(uint32_t top)
     return (__popc(taken) != 4 || (top & 0xe0e0e0e0u)) ? 0xffffffffu : top;
 }
 
-// Per-thread sample configuration (and, in plain order, the per-step
-// increment) live in LDS, [joint][thread], not in registers: the check
-// program reads q[j] once per joint (for its sin/cos) and the step loop adds
-// d once per step, so neither needs to hold VGPRs across the whole program,
-// which leaves the registers to the frames and sweep temporaries.
+// Per-thread sample configuration lives in LDS, [joint][thread], not in
+// registers: the check program reads q[j] once per joint (for its sin/cos)
+// and the step loop adds delta once per step, so the configuration does not
+// hold VGPRs across the whole program, which leaves the registers to the
+// frames and sweep temporaries.
 struct LdsRow {
     float* p;   // element j at p[j * kBlock]
     __device__ __forceinline__ float operator[](int j) const { return p[j * kBlock]; }
     __device__ __forceinline__ float& at(int j) const { return p[j * kBlock]; }
 };
 
-// Plain-order pair setup into LDS rows (same arithmetic as setup_pair).
+// Plain order, step 0 of lane t (src/batch_cc.cu:103-116) into the LDS row;
+// returns the pair's step count.
+template <int D>
+__device__ __forceinline__ int plain_round0(const float* __restrict__ se, int resolution, int t, const LdsRow& Q)
+{
+    float qq[D], dd[D];
+    const int n = setup_pair<D>(se, resolution, t, qq, dd);
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        Q.at(j) = qq[j];
+    return n;
+}
+
+// Plain order, the next step: config += delta (src/batch_cc.cu:126-128), with
+// delta recomputed from the edge (the same arithmetic, so the same value) so
+// that no increment has to stay resident between steps.  Linear in the step
+// count: each step adds delta once to the carried configuration.
+template <int D>
+__device__ __forceinline__ void plain_next(const float* __restrict__ se, int resolution, const LdsRow& Q)
+{
+    float s[D], d[D];
+    setup_pair_sd<D>(se, resolution, s, d);
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        Q.at(j) = Q[j] + d[j];
+}
+
+// Plain-order pair setup into LDS rows, configuration and increment (same
+// arithmetic as setup_pair).
 template <int D>
 __device__ __forceinline__ int setup_pair_lds(const float* __restrict__ se, int resolution, int t, const LdsRow& q,
                                               const LdsRow& d)
@@ -169,33 +146,12 @@ __device__ __forceinline__ int setup_pair_lds(const float* __restrict__ se, int 
     return n;
 }
 
-// Sample configuration of round r, lane t of a pair under a learned order,
-// into the LDS row Q; returns the pair's step count.  The pair's start and
-// increment are recomputed from its edge every round (the same arithmetic as
-// setup_pair, so the samples are the plain ones), so the ordered kernels
-// carry no per-pair state across the check program beyond the plain
-// kernels' (edge, step, n) and the order word.
-template <int D>
-__device__ __forceinline__ int ordered_round(const float* __restrict__ se, int resolution, uint32_t top, int r, int t,
-                                             const LdsRow& Q)
-{
-    float s0[D], d[D], qq[D];
-    const int n = setup_pair_sd<D>(se, resolution, s0, d);
-    ordered_sample<D>(s0, d, n, top, r, t, qq);
-#pragma unroll
-    for (int j = 0; j < D; ++j)
-        Q.at(j) = qq[j];
-    return n;
-}
-
-// kOrdered: a learned top-4 order per pair, order[edge][env], decides which
-// samples each round visits (ordered_sample); otherwise plain order.
 // rounds (optional): number of rounds (steps) each pair ran, [edge][env].
-template <class Robot, bool kOrdered, int kWaves, int kUnroll>
+template <class Robot, int kWaves, int kUnroll>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void dense_kernel(
     const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges, int num_edges,
-    int num_envs, int resolution, int chunk, int chunks_per_env, const uint32_t* __restrict__ order,
-    uint8_t* __restrict__ rounds, uint8_t* __restrict__ out)
+    int num_envs, int resolution, int chunk, int chunks_per_env, uint8_t* __restrict__ rounds,
+    uint8_t* __restrict__ out)
 {
     constexpr int D = Robot::dim;
     const int env = blockIdx.x / chunks_per_env;
@@ -203,8 +159,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const int e_begin = ck * chunk;
     const int e_end = min(num_edges, e_begin + chunk);
 
-    // rounds from the edge (ordered_round) or stepped by an LDS increment row
-    constexpr bool kFromEdge = kOrdered || Robot::kRoundsFromEdge;
+    // steps taken from the edge (plain_next) or from an LDS increment row
+    constexpr bool kFromEdge = Robot::kRoundsFromEdge;
     __shared__ int next_edge;
     __shared__ float qs[D][kBlock];
     __shared__ float ds[kFromEdge ? 1 : D][kBlock];
@@ -220,7 +176,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const LdsRow Dl{&ds[0][threadIdx.x]};
 
     int cur = -1, step = 0, n = 0;
-    uint32_t top = 0xffffffffu;   // plain order unless a learned order is given
     bool exhausted = false;
 
     for (;;) {
@@ -233,10 +188,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
             if (g < e_end) {
                 cur = g;
                 step = 0;
-                if constexpr (kOrdered)
-                    top = valid_order(order[(size_t)g * num_envs + env]);
                 if constexpr (kFromEdge)
-                    n = ordered_round<D>(edges + (size_t)g * 2 * D, resolution, top, 0, t, Q);
+                    n = plain_round0<D>(edges + (size_t)g * 2 * D, resolution, t, Q);
                 else
                     n = setup_pair_lds<D>(edges + (size_t)g * 2 * D, resolution, t, Q, Dl);
             } else {
@@ -255,7 +208,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 fin = (step == n);
                 if (!fin) {
                     if constexpr (kFromEdge) {
-                        ordered_round<D>(edges + (size_t)cur * 2 * D, resolution, top, step, t, Q);
+                        plain_next<D>(edges + (size_t)cur * 2 * D, resolution, Q);
                     } else {
 #pragma unroll
                         for (int j = 0; j < D; ++j)
@@ -275,6 +228,180 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     }
 }
 
+// ------------------------------------------------------------ learned order
+// A learned order changes which samples a pair visits first, never which
+// samples it has: a probe pass checks K samples of each pair, one per lane,
+// so 64 / K pairs share a wave (the check program decides lane groups of K
+// instead of half-waves).  The probed samples are step 0 of the pair's first
+// K predicted bins (bin b = lane b's samples {b n + i}; its step-0 sample is
+// start + delta * (b n), exactly the plain one).  Pairs a probe leaves
+// undecided run the plain step loop over all 32 n samples.  The verdict is an
+// OR over the same sample set either way.  Without a prediction (an invalid
+// order word) a probe takes K evenly spaced bins.
+#ifndef BCC_PROBE1
+#define BCC_PROBE1 2
+#endif
+constexpr int kProbe1 = BCC_PROBE1;   // predicted-probe samples per pair
+static_assert(kProbe1 == 1 || kProbe1 == 2 || kProbe1 == 4, "the probe visits up to the four predicted bins");
+
+constexpr int kOrderedChunkMax = 1024;   // edges per workgroup of the ordered dense kernel
+constexpr int kOrderedTileMax = 256;     // pairs per tile / segment of the pipeline kernels
+
+template <int K>
+__device__ __forceinline__ int probe_bin(uint32_t top, int k)
+{
+    return top == 0xffffffffu ? (32 * k + 16) / K : (int)((top >> (8 * k)) & 31u);
+}
+
+// Lane k of a probe group: step 0 of the pair's k-th probed bin into Q.
+template <int D, int K>
+__device__ __forceinline__ void probe_sample(const float* __restrict__ se, int resolution, uint32_t top, int k,
+                                             const LdsRow& Q)
+{
+    float s[D], d[D];
+    const int n = setup_pair_sd<D>(se, resolution, s, d);
+    const float bn = (float)(probe_bin<K>(top, k) * n);
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        Q.at(j) = fmaf(d[j], bn, s[j]);
+}
+
+// Plain step loop of one wave over a queue of pairs (the pairs a probe left
+// undecided): each half-wave takes the next queued pair from the LDS counter
+// *next.  pair_edge / pair_env / pair_out map a queue entry to its edge,
+// environment-independent verdict slot and rounds slot.
+template <class Robot, int kUnroll, class Pair>
+__device__ __forceinline__ void plain_queue(const EnvView& E, const float* __restrict__ edges, int resolution,
+                                            const int* queue, int nq, int* next, const LdsRow& Q, Pair pair)
+{
+    constexpr int D = Robot::dim;
+    const int lane = threadIdx.x & 63;
+    const int t = lane & 31;
+    const int half0 = lane & 32;
+    int cur = -1, step = 0, n = 0;
+    const float* se = nullptr;
+    bool exhausted = false;
+    for (;;) {
+        const bool want = (cur < 0) && !exhausted;
+        int qi = 0;
+        if (want && t == 0)
+            qi = atomicAdd(next, 1);
+        qi = __shfl(qi, half0);
+        if (want) {
+            if (qi < nq) {
+                cur = queue[qi];
+                step = 0;
+                se = edges + pair.edge(cur) * 2 * D;
+                n = plain_round0<D>(se, resolution, t, Q);
+            } else {
+                exhausted = true;
+            }
+        }
+        const uint64_t live = __ballot(cur >= 0);
+        if (live == 0)
+            break;
+        const uint64_t hit = Robot::template check<kUnroll>(Q, E, live);
+        if (cur >= 0) {
+            const bool h = (hit >> lane) & 1ull;
+            bool fin = h;
+            if (!h) {
+                ++step;
+                fin = (step == n);
+                if (!fin)
+                    plain_next<D>(se, resolution, Q);
+            }
+            if (fin) {
+                if (t == 0)
+                    pair.finish(cur, h, h ? step + 1 : step);
+                cur = -1;
+            }
+        }
+    }
+}
+
+struct DensePairs {
+    const uint32_t* order;
+    uint8_t* out;
+    uint8_t* rounds;
+    int num_envs, env;
+    __device__ __forceinline__ size_t edge(int g) const { return (size_t)g; }
+    __device__ __forceinline__ size_t slot(int g) const { return (size_t)g * num_envs + env; }
+    __device__ __forceinline__ uint32_t top(int g) const { return valid_order(order[slot(g)]); }
+    // rounds: 0 = decided by the probe, else 1 + plain steps run
+    __device__ __forceinline__ void finish(int g, bool h, int steps) const
+    {
+        out[slot(g)] = h ? 1 : 0;
+        if (rounds)
+            rounds[slot(g)] = (uint8_t)min(steps + 1, 255);
+    }
+    __device__ __forceinline__ void probe_hit(int g) const
+    {
+        out[slot(g)] = 1;
+        if (rounds)
+            rounds[slot(g)] = 0;
+    }
+};
+
+// Learned-order dense kernel: a workgroup owns one environment and a chunk of
+// up to kOrderedChunkMax edges.  Probe pass: each wave takes 64 / K edges at
+// a time and lane group e (K lanes) checks edge e's K probed samples; edges
+// without a hit go to an LDS queue.  After a barrier the half-waves run the
+// plain step loop over the queue.  order[edge][env] holds the learned top-4
+// bins; rounds (optional) as DensePairs::finish.
+template <class Robot, int K, int kWaves, int kUnroll>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void ordered_dense_kernel(
+    const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges, int num_edges,
+    int num_envs, int resolution, int chunk, int chunks_per_env, const uint32_t* __restrict__ order,
+    uint8_t* __restrict__ rounds, uint8_t* __restrict__ out)
+{
+    constexpr int D = Robot::dim;
+    constexpr int kPairs = 64 / K;
+    const int env = blockIdx.x / chunks_per_env;
+    const int ck = blockIdx.x - env * chunks_per_env;
+    const int e_begin = ck * chunk;
+    const int e_end = min(num_edges, e_begin + chunk);   // chunk <= kOrderedChunkMax (host-checked)
+
+    __shared__ int next_edge, queued, next_queued;
+    __shared__ int queue[kOrderedChunkMax];
+    __shared__ float qs[D][kBlock];
+    if (threadIdx.x == 0) {
+        next_edge = e_begin;
+        queued = 0;
+        next_queued = 0;
+    }
+    __syncthreads();
+
+    const EnvView E = load_env(obs, meta, env);
+    const int lane = threadIdx.x & 63;
+    const int grp = lane / K;
+    const int k = lane - grp * K;
+    const LdsRow Q{&qs[0][threadIdx.x]};
+    const DensePairs P{order, out, rounds, num_envs, env};
+
+    for (;;) {
+        int g0 = 0;
+        if (lane == 0)
+            g0 = atomicAdd(&next_edge, kPairs);
+        g0 = __builtin_amdgcn_readfirstlane(__shfl(g0, 0));
+        if (g0 >= e_end)
+            break;
+        const int g = g0 + grp;
+        const bool valid = g < e_end;
+        if (valid)
+            probe_sample<D, K>(edges + (size_t)g * 2 * D, resolution, P.top(g), k, Q);
+        const uint64_t live = __ballot(valid);
+        const uint64_t hit = Robot::template check<kUnroll, K>(Q, E, live);
+        if (valid && k == 0) {
+            if ((hit >> lane) & 1ull)
+                P.probe_hit(g);
+            else
+                queue[atomicAdd(&queued, 1)] = g;
+        }
+    }
+    __syncthreads();
+    plain_queue<Robot, kUnroll>(E, edges, resolution, queue, queued, &next_queued, Q, P);
+}
+
 // ------------------------------------------------------------ ragged batches
 // Ragged batch: env e owns the contiguous pairs [env_off[e], env_off[e+1])
 // (CSR); pair p checks edges[p] against environment e and writes out[p].
@@ -291,18 +418,13 @@ struct RaggedTile {
     int pad;
 };
 
-//
-// kOrdered: order[p] holds pair p's learned top-4 bins (as in dense_kernel).
-// Each round's samples are computed from the pair's edge (ordered_round; in
-// plain order: start + delta * (t n), then `step` additions of delta, the
-// plain samples), not stepped from an increment row in LDS: on MI355X this
-// is 4-6% faster for fetch and baxter batches and equal for panda.  Dense
-// kernels do the same where the robot's kRoundsFromEdge says so.
-template <class Robot, bool kOrdered, int kWaves, int kUnroll>
+// Steps are taken from the edge (plain_next): on MI355X this is 4-6% faster
+// for fetch and baxter batches than an LDS increment row and equal for panda.
+template <class Robot, int kWaves, int kUnroll>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void ragged_kernel(
     const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges,
     const RaggedTile* __restrict__ tiles, int num_tiles, int* __restrict__ tile_counter, int resolution,
-    const uint32_t* __restrict__ order, uint8_t* __restrict__ out)
+    uint8_t* __restrict__ out)
 {
     constexpr int D = Robot::dim;
     constexpr int kBlockWaves = kBlock / 64;
@@ -313,7 +435,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const int t = lane & 31;
     const int half0 = lane & 32;
     const LdsRow Q{&qs[0][threadIdx.x]};
-    uint32_t top = 0xffffffffu;   // plain order unless a learned order is given
 
     for (;;) {
         // -- take a tile (wave-uniform)
@@ -343,9 +464,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 if (g < p_end) {
                     cur = g;
                     step = 0;
-                    if constexpr (kOrdered)
-                        top = valid_order(order[g]);
-                    n = ordered_round<D>(edges + (size_t)g * 2 * D, resolution, top, 0, t, Q);
+                    n = plain_round0<D>(edges + (size_t)g * 2 * D, resolution, t, Q);
                 } else {
                     exhausted = true;
                 }
@@ -361,7 +480,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                     ++step;
                     fin = (step == n);
                     if (!fin)
-                        ordered_round<D>(edges + (size_t)cur * 2 * D, resolution, top, step, t, Q);
+                        plain_next<D>(edges + (size_t)cur * 2 * D, resolution, Q);
                 }
                 if (fin) {
                     if (t == 0)
@@ -391,8 +510,7 @@ struct RaggedArgs {
     int resolution;
     int grid;        // persistent workgroups
     uint8_t* out;
-    int variant = 0;                   // 1: high-occupancy kernel variant, 0: low
-    const uint32_t* order = nullptr;   // learned top-4 bins per pair, or null (plain order)
+    int variant = 0;   // 1: high-occupancy kernel variant, 0: low
 };
 
 // Kernel variant knobs of a robot: (waves per SIMD, obstacle-loop unroll) of
@@ -403,11 +521,11 @@ struct VariantKnobs {
 
 // One persistent launch of one kernel variant (instantiated in its own
 // translation unit, see bcc_robot_parts.h).
-template <class Robot, bool kOrdered, int W, int U>
+template <class Robot, int W, int U>
 hipError_t launch_ragged_variant(const RaggedArgs& a, hipStream_t stream)
 {
-    hipLaunchKernelGGL((ragged_kernel<Robot, kOrdered, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta,
-                       a.edges, a.tiles, a.num_tiles, a.tile_counter, a.resolution, a.order, a.out);
+    hipLaunchKernelGGL((ragged_kernel<Robot, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
+                       a.tiles, a.num_tiles, a.tile_counter, a.resolution, a.out);
     return hipGetLastError();
 }
 
@@ -417,13 +535,169 @@ hipError_t launch_ragged(const RaggedArgs& a, hipStream_t stream)
     hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, stream, a.tile_counter);
     if (a.num_tiles == 0)
         return hipGetLastError();
-    if (a.order && a.variant)
-        return launch_ragged_variant<Robot, true, HW, HU>(a, stream);
-    if (a.order)
-        return launch_ragged_variant<Robot, true, LW, LU>(a, stream);
     if (a.variant)
-        return launch_ragged_variant<Robot, false, HW, HU>(a, stream);
-    return launch_ragged_variant<Robot, false, LW, LU>(a, stream);
+        return launch_ragged_variant<Robot, HW, HU>(a, stream);
+    return launch_ragged_variant<Robot, LW, LU>(a, stream);
+}
+
+// ------------------------------------------------------------ learned-order plans
+// A ragged plan with a learned order runs learn_finish_kernel over segments
+// (one environment each) of a queue of pairs: each pair checks the step-0
+// samples of its top kProbe1 predicted bins, and the pairs still undecided
+// run the plain step loop over all 32 n samples.  Every verdict is an OR over
+// the reference's sample set; the order only changes which samples are
+// visited first.  For an ordered plan the segments are its tiles and the
+// queue is the identity (set up on the host).
+//
+// Pairs are addressed by tiles (env, first, end) of an index range: index i
+// is the row of the edge array and its verdict lives at out[i * stride + env
+// * env_mult] (ragged: i).
+struct PairSlots {
+    int stride;
+    int env_mult;
+    __device__ __forceinline__ size_t slot(int i, int env) const
+    {
+        return (size_t)i * stride + (size_t)env * env_mult;
+    }
+};
+
+// Queued pairs of one environment: queue entries [base, base + count).
+struct QueueSeg {
+    int env;
+    int base;
+    int count;   // at most kOrderedTileMax
+    int pad;
+};
+
+// Device counters of one plan launch, zeroed (segment count: preset) at the
+// start of every launch by reset_pipeline_kernel.
+enum : int {
+    kCtrSegs = 2,          // segments in the queue (preset)
+    kCtrSegWork = 3,       // segment work counter
+    kCtrLearnedHits = 4,   // pairs decided by the learned probe
+    kCtrInts = 8
+};
+
+static __global__ void reset_pipeline_kernel(int* ctr, int preset_segs)
+{
+    if (blockIdx.x == 0 && threadIdx.x < kCtrInts)
+        ctr[threadIdx.x] = threadIdx.x == kCtrSegs ? preset_segs : 0;
+}
+
+struct SlotPairs {
+    PairSlots s;
+    int env;
+    uint8_t* out;
+    __device__ __forceinline__ size_t edge(int i) const { return (size_t)i; }
+    __device__ __forceinline__ void finish(int i, bool h, int) const { out[s.slot(i, env)] = h ? 1 : 0; }
+};
+
+// Persistent: each wave takes the next segment (the count of segments is
+// read from ctr, preset by the plan's reset node), probes its pairs
+// at the step-0 samples of their top K predicted bins (order[entry], as in
+// valid_order; no order: no probe), then runs the plain step loop over the
+// pairs still undecided with its two half-waves.
+template <class Robot, int K, int kWaves, int kUnroll>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void learn_finish_kernel(
+    const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges,
+    const QueueSeg* __restrict__ segs, int* __restrict__ ctr, int resolution, PairSlots slots,
+    const uint32_t* __restrict__ order, const int* __restrict__ queue_idx, uint8_t* __restrict__ out)
+{
+    constexpr int D = Robot::dim;
+    constexpr int kBlockWaves = kBlock / 64;
+    constexpr int kPairs = 64 / K;
+    __shared__ int queued[kBlockWaves], next_queued[kBlockWaves];
+    __shared__ int queue[kBlockWaves][kOrderedTileMax];
+    __shared__ float qs[D][kBlock];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int grp = lane / K;
+    const int k = lane - grp * K;
+    const LdsRow Q{&qs[0][threadIdx.x]};
+    const int num_segs = __builtin_amdgcn_readfirstlane(__atomic_load_n(ctr + kCtrSegs, __ATOMIC_RELAXED));
+
+    for (;;) {
+        int sg = 0;
+        if (lane == 0)
+            sg = atomicAdd(ctr + kCtrSegWork, 1);
+        sg = __builtin_amdgcn_readfirstlane(__shfl(sg, 0));
+        if (sg >= num_segs)
+            return;
+        const QueueSeg S = segs[sg];
+        const int env = __builtin_amdgcn_readfirstlane(S.env);
+        const int base = __builtin_amdgcn_readfirstlane(S.base);
+        const int count = __builtin_amdgcn_readfirstlane(S.count);
+        if (lane == 0) {
+            queued[wave] = 0;
+            next_queued[wave] = 0;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        const EnvView E = load_env(obs, meta, env);
+        if (order) {
+            int hits = 0;
+            for (int j0 = 0; j0 < count; j0 += kPairs) {
+                const int j = j0 + grp;
+                const bool valid = j < count;
+                const int i = valid ? queue_idx[base + j] : 0;
+                if (valid)
+                    probe_sample<D, K>(edges + (size_t)i * 2 * D, resolution, valid_order(order[base + j]), k, Q);
+                const uint64_t live = __ballot(valid);
+                const uint64_t hit = Robot::template check<kUnroll, K>(Q, E, live);
+                const bool h = (hit >> lane) & 1ull;
+                if (valid && k == 0) {
+                    if (h)
+                        out[slots.slot(i, env)] = 1;
+                    else
+                        queue[wave][atomicAdd(&queued[wave], 1)] = i;
+                }
+                hits += __popcll(__ballot(valid && k == 0 && h));
+            }
+            if (lane == 0 && hits)
+                atomicAdd(ctr + kCtrLearnedHits, hits);
+        } else {
+            for (int j = lane; j < count; j += 64)
+                queue[wave][j] = queue_idx[base + j];
+            if (lane == 0)
+                queued[wave] = count;
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const int nq = __builtin_amdgcn_readfirstlane(queued[wave]);
+        plain_queue<Robot, kUnroll>(E, edges, resolution, queue[wave], nq, &next_queued[wave], Q,
+                                    SlotPairs{slots, env, out});
+    }
+}
+
+struct LearnArgs {
+    const float* obs;
+    const int* meta;
+    const float* edges;
+    int* ctr;                  // [kCtrInts]
+    int resolution;
+    int grid;                  // persistent workgroups
+    int variant;               // 1: high-occupancy kernel variant, 0: low
+    PairSlots slots;
+    uint8_t* out;
+    int* queue_idx;
+    QueueSeg* segs;
+    const uint32_t* order;     // per queue entry, or null
+};
+
+template <class Robot, int W, int U>
+hipError_t launch_learn_finish_variant(const LearnArgs& a, hipStream_t stream)
+{
+    hipLaunchKernelGGL((learn_finish_kernel<Robot, kProbe1, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs,
+                       a.meta, a.edges, a.segs, a.ctr, a.resolution, a.slots, a.order, a.queue_idx, a.out);
+    return hipGetLastError();
+}
+
+// pipeline stage 3 (learned probe + plain finish) of one plan launch
+template <class Robot, int HW, int HU, int LW, int LU>
+hipError_t launch_learn(const LearnArgs& a, hipStream_t stream)
+{
+    return a.variant ? launch_learn_finish_variant<Robot, HW, HU>(a, stream)
+                     : launch_learn_finish_variant<Robot, LW, LU>(a, stream);
 }
 
 // ------------------------------------------------------------ test hooks
@@ -701,8 +975,13 @@ template <class Robot, bool kOrdered, int W, int U>
 hipError_t launch_dense_variant(const DenseArgs& a, unsigned blocks, int chunk, int chunks_per_env,
                                 hipStream_t stream)
 {
-    hipLaunchKernelGGL((dense_kernel<Robot, kOrdered, W, U>), dim3(blocks), dim3(kBlock), 0, stream, a.obs, a.meta,
-                       a.edges, a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order, a.rounds, a.out);
+    if constexpr (kOrdered)
+        hipLaunchKernelGGL((ordered_dense_kernel<Robot, kProbe1, W, U>), dim3(blocks), dim3(kBlock), 0, stream, a.obs,
+                           a.meta, a.edges, a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.order,
+                           a.rounds, a.out);
+    else
+        hipLaunchKernelGGL((dense_kernel<Robot, W, U>), dim3(blocks), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
+                           a.num_edges, a.num_envs, a.resolution, chunk, chunks_per_env, a.rounds, a.out);
     return hipGetLastError();
 }
 
@@ -712,6 +991,8 @@ hipError_t launch_dense(const DenseArgs& a, hipStream_t stream)
     if (a.num_edges == 0 || a.num_envs == 0)
         return hipSuccess;
     const int chunk = a.chunk > 0 ? a.chunk : 64;
+    if (a.order && chunk > kOrderedChunkMax)
+        return hipErrorInvalidValue;
     const int chunks_per_env = (a.num_edges + chunk - 1) / chunk;
     const long long blocks = (long long)chunks_per_env * a.num_envs;
     if (blocks > 0x7fffffffLL)
@@ -754,6 +1035,7 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
                                      float* cfg_last, hipStream_t stream);                                     \
     hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream);                                     \
     hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream);                                 \
+    hipError_t launch_learn_##name(const LearnArgs& a, hipStream_t stream);                                   \
     hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream);                                   \
     extern const VariantKnobs knobs_##name;
 

@@ -139,6 +139,29 @@ __device__ __forceinline__ uint64_t bcc_half_spread(uint64_t m)
     return (lo ? 0xffffffffull : 0ull) | (hi ? 0xffffffff00000000ull : 0ull);
 }
 
+// Spread a lane mask to aligned groups of G lanes (G a power of two, at most
+// 32): a group is marked if any of its lanes is marked.  G = 32 is the
+// half-wave spread; G = 1 leaves the mask as it is.
+template <int G>
+__device__ __forceinline__ uint64_t bcc_group_spread(uint64_t m)
+{
+    static_assert(G >= 1 && G <= 32 && (G & (G - 1)) == 0, "group width must be a power of two up to 32");
+    if constexpr (G == 32) {
+        return bcc_half_spread(m);
+    } else if constexpr (G == 1) {
+        return m;
+    } else {
+        // OR each group's bits down into its lowest bit, keep those bits,
+        // then copy each kept bit across its group (no carries: groups are
+        // disjoint)
+#pragma unroll
+        for (int s = 1; s < G; s <<= 1)
+            m |= m >> s;
+        constexpr uint64_t kLow = ~0ull / ((1ull << G) - 1ull);   // bit 0 of every group
+        return (m & kLow) * ((1ull << G) - 1ull);
+    }
+}
+
 // ------------------------------------------------------------ primitives
 // Each returns the reference's clearance value; < 0 means collision.
 
@@ -579,9 +602,9 @@ __device__ __forceinline__ void bcc_env_sweep(const EnvView& env, const float (&
 }
 
 // Does any of the N spheres touch any obstacle?  Per-lane OR.  Only lanes in
-// `need` matter; the sweep stops once every half-wave present in `need` has a
-// hit (one colliding sample decides that half's pair).
-template <int N, int U, bool kDual>
+// `need` matter; the sweep stops once every lane group of G lanes present in
+// `need` has a hit (one colliding sample decides that group's pair).
+template <int N, int U, bool kDual, int G = 32>
 __device__ __forceinline__ bool bcc_env_any(const EnvView& env, const float (&x)[N], const float (&y)[N],
                                             const float (&z)[N], const float (&rp)[N], uint64_t need)
 {
@@ -594,7 +617,7 @@ __device__ __forceinline__ bool bcc_env_any(const EnvView& env, const float (&x)
     if (E.nbox == 0)
         return h;
 #define SETTLED_CHECK                                                         \
-    if ((need & ~bcc_half_spread(__ballot(h) & need)) == 0)                   \
+    if ((need & ~bcc_group_spread<G>(__ballot(h) & need)) == 0)               \
         return h;
 #define T_SPH h = h | (bcc_sphere_val(o[0], o[1], o[2], o[3], x[i], y[i], z[i], rp[i]) < 0.0f)
 #define T_CAP h = h | (bcc_capsule_val(o, x[i], y[i], z[i], rp[i]) < 0.0f)

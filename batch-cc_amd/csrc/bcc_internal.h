@@ -19,6 +19,7 @@ struct bcc_scene {
     int* d_ref_meta = nullptr;
     size_t obs_floats = 0;
     std::vector<int> obstacles;  // tested primitives per environment (host copy)
+    int cull_min = 0;            // culling threshold stamped into every environment's record
 };
 
 namespace bcc_detail {

@@ -13,7 +13,7 @@
 //   1, 7   ordered dense kernel, low / high occupancy
 //   2, 3   plain dense kernel, high / low occupancy
 //   4, 5   ragged kernel, high / low occupancy
-//   6, 8   ordered ragged kernel, low / high occupancy
+//   6, 8   learned-order ragged kernel (predicted probe + plain finish), low / high occupancy
 #ifndef BCC_PART
 #define BCC_PART -1
 #endif
@@ -40,14 +40,14 @@ extern template hipError_t launch_dense_variant<robot_parts::RobotT, false, robo
     const DenseArgs&, unsigned, int, int, hipStream_t);
 extern template hipError_t launch_dense_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
     const DenseArgs&, unsigned, int, int, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::HW, robot_parts::HU>(
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(
     const RaggedArgs&, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
+extern template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(
     const RaggedArgs&, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
-    const RaggedArgs&, hipStream_t);
-extern template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::HW, robot_parts::HU>(
-    const RaggedArgs&, hipStream_t);
+extern template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(
+    const LearnArgs&, hipStream_t);
+extern template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(
+    const LearnArgs&, hipStream_t);
 #endif
 
 #if BCC_PART == -1 || BCC_PART == 0
@@ -63,6 +63,12 @@ hipError_t BCC_CAT2(launch_ragged_, BCC_ROBOT)(const RaggedArgs& a, hipStream_t 
 {
     using namespace robot_parts;
     return launch_ragged<RobotT, HW, HU, LW, LU>(a, stream);
+}
+
+hipError_t BCC_CAT2(launch_learn_, BCC_ROBOT)(const LearnArgs& a, hipStream_t stream)
+{
+    using namespace robot_parts;
+    return launch_learn<RobotT, HW, HU, LW, LU>(a, stream);
 }
 
 hipError_t BCC_CAT2(launch_fk_, BCC_ROBOT)(const float* qs, int n, float* out, hipStream_t stream)
@@ -104,20 +110,20 @@ template hipError_t launch_dense_variant<robot_parts::RobotT, false, robot_parts
     const DenseArgs&, unsigned, int, int, hipStream_t);
 #endif
 #if BCC_PART == 4
-template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::HW, robot_parts::HU>(
-    const RaggedArgs&, hipStream_t);
+template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(const RaggedArgs&,
+                                                                                                 hipStream_t);
 #endif
 #if BCC_PART == 5
-template hipError_t launch_ragged_variant<robot_parts::RobotT, false, robot_parts::LW, robot_parts::LU>(
-    const RaggedArgs&, hipStream_t);
+template hipError_t launch_ragged_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(const RaggedArgs&,
+                                                                                                 hipStream_t);
 #endif
 #if BCC_PART == 6
-template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::LW, robot_parts::LU>(
-    const RaggedArgs&, hipStream_t);
+template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(
+    const LearnArgs&, hipStream_t);
 #endif
 #if BCC_PART == 8
-template hipError_t launch_ragged_variant<robot_parts::RobotT, true, robot_parts::HW, robot_parts::HU>(
-    const RaggedArgs&, hipStream_t);
+template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(
+    const LearnArgs&, hipStream_t);
 #endif
 
 }  // namespace bcc

@@ -359,7 +359,7 @@ def gen_check(model: dict, order: str | None = None) -> str:
             g = envp[gi]
             em.line(f"// env group {gi}" + (f" ({g['label']})" if g["label"] else ""))
             if g["lone"]:
-                em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot(gh[{k}]) & live); hit |= d; live &= ~d; }}")
+                em.line(f"{{ const uint64_t d = bcc_group_spread<G>(__ballot(gh[{k}]) & live); hit |= d; live &= ~d; }}")
                 continue
             em.line(f"if (uint64_t need = __ballot(gh[{k}]) & live) {{")
             em.indent += 1
@@ -378,8 +378,8 @@ def gen_check(model: dict, order: str | None = None) -> str:
                 em.line(f"const float fy[{nf}] = {{{', '.join(fy)}}};")
                 em.line(f"const float fz[{nf}] = {{{', '.join(fz)}}};")
                 em.line(f"const float frp[{nf}] = {{{', '.join(frp)}}};")
-                em.line(f"const bool h = bcc_env_any<{nf}, U, kDualSweep>(env, fx, fy, fz, frp, need);")
-                em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
+                em.line(f"const bool h = bcc_env_any<{nf}, U, kDualSweep, G>(env, fx, fy, fz, frp, need);")
+                em.line("const uint64_t d = bcc_group_spread<G>(__ballot(h) & need);")
                 em.line("hit |= d; live &= ~d; need &= ~d;")
                 em.indent -= 1
                 em.line("}")
@@ -401,7 +401,7 @@ def gen_check(model: dict, order: str | None = None) -> str:
         gate = (f"bcc_self_hit({ca[0]}, {ca[1]}, {ca[2]}, {ra}, "
                 f"{cb[0]}, {cb[1]}, {cb[2]}, {rb})")
         if g["lone"]:
-            em.line(f"{{ const uint64_t d = bcc_half_spread(__ballot({gate}) & live); hit |= d; live &= ~d; }}")
+            em.line(f"{{ const uint64_t d = bcc_group_spread<G>(__ballot({gate}) & live); hit |= d; live &= ~d; }}")
             return
         em.line(f"if (uint64_t need = __ballot({gate}) & live) {{")
         em.indent += 1
@@ -428,7 +428,7 @@ def gen_check(model: dict, order: str | None = None) -> str:
             terms = [f"bcc_self_hit({xc[0]}, {xc[1]}, {xc[2]}, {rx}, {yc[0]}, {yc[1]}, {yc[2]}, {ry})"
                      for xc, rx in acs]
             em.line("const bool h = " + "\n                | ".join(terms) + ";")
-            em.line("const uint64_t d = bcc_half_spread(__ballot(h) & need);")
+            em.line("const uint64_t d = bcc_group_spread<G>(__ballot(h) & need);")
             em.line("hit |= d; live &= ~d; need &= ~d;")
             em.indent -= 1
             em.line("}")
@@ -465,12 +465,14 @@ def gen_check(model: dict, order: str | None = None) -> str:
         f"    static constexpr bool kDualSweep = {'true' if DUAL_SWEEP.get(robot, True) else 'false'};",
         "    // dense kernels compute each round's samples from the edge",
         f"    static constexpr bool kRoundsFromEdge = {'true' if ROUNDS_FROM_EDGE.get(robot, True) else 'false'};",
-        "    // Collision predicate for one sample per lane.  `live` holds the lanes whose",
-        "    // half-wave pair is still undecided; returns the half-spread mask of lanes",
-        "    // whose half found a colliding sample (each collision decides its half).",
+        "    // Collision predicate for one sample per lane.  Lanes form aligned groups of",
+        "    // G (32: a half-wave, the reference's 32 samples of one pair per step; fewer:",
+        "    // several pairs per half-wave, G samples each).  `live` holds the lanes whose",
+        "    // group's pair is still undecided; returns the group-spread mask of lanes",
+        "    // whose group found a colliding sample (each collision decides its group).",
         "    // q[j] is joint j of this lane's sample (an array or an LDS row view);",
         "    // U is the obstacle-loop unroll factor of the calling kernel variant.",
-        "    template <int U, class Q>",
+        "    template <int U, int G = 32, class Q>",
         "    static __device__ __forceinline__ uint64_t check(const Q& q, const EnvView& env, uint64_t live)",
         "    {",
     ]

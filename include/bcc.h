@@ -166,9 +166,11 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene *scene, const int64_
                                  const float *d_edges, int resolution, uint8_t *d_results, int tile_pairs,
                                  int groups_per_cu, bcc_ragged_plan **out);
 /* Same, with a learned visiting order per pair: d_order [total pairs] on the
- * device, uint32 as in bcc_check_dense_ex (d_order[p] for pair p; any value
- * that does not name four distinct bins below 32 means plain order).  The
- * plan reads d_order at every launch.  Verdicts equal the plain plan's.  No
+ * device, uint32 as in bcc_check_dense_ex (d_order[p] for pair p).  Each pair
+ * first checks the step-0 samples of its first predicted bins, then runs the
+ * plain step loop; an order that does not name four distinct bins below 32
+ * probes evenly spaced bins instead.  Tiles hold at most 256 pairs.  The plan
+ * reads d_order at every launch.  Verdicts equal the plain plan's.  No
  * reference counterpart (learned ordering). */
 int bcc_ragged_plan_create_ordered(int robot, const bcc_scene *scene, const int64_t *env_offsets,
                                    const float *d_edges, int resolution, const uint32_t *d_order,
@@ -198,11 +200,15 @@ int bcc_ragged_group_destroy(bcc_ragged_group *group);
 /*
  * Dense check with options (device buffers):
  *   d_order  (optional) learned visiting order per pair, [num_edges][num_envs]
- *            uint32: four distinct sample bins (0..31), one byte per rank.
- *            Step 0 is checked as in plain order, then the remaining samples
- *            of those bins, then the rest in plain order; any other value
- *            means plain order.  Verdicts do not depend on the order.
- *   d_rounds (optional) rounds (32-sample steps) each pair ran, uint8.
+ *            uint32: four distinct sample bins (0..31), one byte per rank,
+ *            most likely to collide first (bin b = lane b's samples b n + i).
+ *            The pair first checks the step-0 samples of its first predicted
+ *            bins (a probe, several pairs per wave), then, if none collided,
+ *            runs the plain step loop over all 32 n samples.  Any other value
+ *            probes evenly spaced bins.  Verdicts do not depend on the order.
+ *   d_rounds (optional) uint8 per pair: without d_order the rounds (32-sample
+ *            steps) the pair ran; with d_order 0 if the probe decided the
+ *            pair, else 1 + the plain steps it ran.
  */
 int bcc_check_dense_ex(int robot, const bcc_scene *scene, const float *d_edges, int num_edges, int resolution,
                        const uint32_t *d_order, uint8_t *d_rounds, uint8_t *d_results, void *stream);
@@ -257,6 +263,7 @@ int bcc_model_logits_dense(bcc_model *model, const bcc_scene *scene, const float
 int bcc_model_order_pairs(bcc_model *model, const bcc_scene *scene, const float *d_edges, int64_t num_pairs,
                           const int32_t *d_pair_env, const int32_t *d_pair_edge, uint32_t *d_order,
                           float *d_logits, void *stream);
+
 /* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
 int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,
                    int m, int n, int k, float alpha, float beta);
@@ -271,6 +278,10 @@ int bcc_debug_force_variant(int variant);
  * environment's primitive count is at least `work` (0 = always, a huge value
  * = never, -1 = the built-in tuned value).  Verdicts never depend on it. */
 int bcc_debug_cull_min(int work);
+
+/* Test hook (no reference counterpart): the culling threshold stamped into
+ * `scene` when it was created (see bcc_debug_cull_min). */
+int bcc_debug_scene_cull_min(const bcc_scene *scene, int *out);
 
 /* Tuning hook (no reference counterpart): edges per workgroup of dense
  * launches (0 = chosen per launch: 64, halved down to 8 while the launch has

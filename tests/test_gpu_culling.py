@@ -2,7 +2,14 @@
 every list kind planted at signed distances around the cull margin (and
 beyond it) from a robot sphere, obstacles at 64-obstacle window boundaries
 among many distant ones, huge and non-orthonormal primitives.  The kernels
-must give the oracle's verdicts, which never cull."""
+must give the oracle's verdicts, which never cull.
+
+The kernels cull a sweep only when (spheres in the sweep) x (primitives in the
+environment) reaches the threshold stamped into the scene at creation, so the
+one-primitive environments below would never cull at the tuned threshold:
+every test runs with the threshold forced to 0 (every sweep culls, asserted
+through the scene) and with culling off (a threshold no sweep reaches)."""
+import contextlib
 import json
 import os
 
@@ -18,6 +25,24 @@ ROBOTS_DIR = os.path.join(os.path.dirname(HERE), "batch-cc_amd", "robots")
 # the kernels cull only beyond 0.01 m (plus a relative term)
 OFFSETS = (-2e-3, -1e-4, -1e-6, 0.0, 1e-6, 1e-4, 2e-3, 9.9e-3, 1.01e-2, 2e-2, 5e-2)
 KIND_NAMES = ("sphere", "capsule", "z_capsule", "cuboid", "z_cuboid", "aabb", "big_aabb")
+
+
+CULL = {"always": 0, "never": 1 << 30}
+
+
+@contextlib.contextmanager
+def _scene(bcc, packed, cull):
+    """Scene created under a forced culling threshold (restored afterwards)."""
+    bcc.debug_cull_min(CULL[cull])
+    try:
+        scene = bcc.Scene(packed)
+    finally:
+        bcc.debug_cull_min(-1)
+    try:
+        assert scene.cull_min == CULL[cull]
+        yield scene
+    finally:
+        scene.close()
 
 
 def _pad(r):
@@ -98,8 +123,9 @@ def _tested_spheres(robot):
     return out
 
 
+@pytest.mark.parametrize("cull", ["always", "never"])
 @pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
-def test_obstacles_at_the_cull_margin_match_oracle(bcc, oracle, gpu, robot):
+def test_obstacles_at_the_cull_margin_match_oracle(bcc, oracle, gpu, robot, cull):
     from batch_cc_amd import workloads as W
     rng = np.random.default_rng([41, RID[robot]])
     qs = _free_configs(oracle, bcc, W, robot, 8, rng)
@@ -116,8 +142,13 @@ def test_obstacles_at_the_cull_margin_match_oracle(bcc, oracle, gpu, robot):
     sc = bcc.PackedScene.from_envs(envs)
     edges = np.stack([qs, qs], axis=1).astype(np.float32)     # one configuration per edge
     exp = oracle.check_dense(RID[robot], sc, edges, threads=16)
-    with bcc.Scene(sc) as scene:
+    with _scene(bcc, sc, cull) as scene:
         got = bcc.check_dense(robot, scene, edges)
+        # the same pairs through a ragged plan (one configuration per pair)
+        E = sc.num_envs
+        off = np.arange(E + 1, dtype=np.int64) * len(qs)
+        got_r = bcc.check_ragged(robot, scene, off, np.tile(edges, (E, 1, 1)))
+    assert np.array_equal(got_r, exp.T.reshape(-1))
     own = np.array(own)
     mine = exp[own, np.arange(len(envs))]
     print(f"{robot}: {len(envs)} planted environments, own-configuration collisions {int(mine.sum())}, "
@@ -130,7 +161,8 @@ def test_obstacles_at_the_cull_margin_match_oracle(bcc, oracle, gpu, robot):
     assert np.all(mine[offs < -1e-5] == 1)
 
 
-def test_window_boundaries_with_many_distant_obstacles(bcc, oracle, gpu):
+@pytest.mark.parametrize("cull", ["always", "never"])
+def test_window_boundaries_with_many_distant_obstacles(bcc, oracle, gpu, cull):
     """Environments of 64, 128 and 130 primitives, all far away except one near
     primitive at list positions around the 64-obstacle window edges (the
     kernels' box order: spheres, capsules, z-capsules, cuboids, z-cuboids,
@@ -173,12 +205,12 @@ def test_window_boundaries_with_many_distant_obstacles(bcc, oracle, gpu):
     sc = bcc.PackedScene.from_envs(envs)
     edges = np.stack([qs, qs], axis=1).astype(np.float32)
     exp = oracle.check_dense(0, sc, edges, threads=16)
-    with bcc.Scene(sc) as scene:
+    with _scene(bcc, sc, cull) as scene:
         got = bcc.check_dense(robot, scene, edges)
     print(f"{len(envs)} environments, colliding pairs {int(exp.sum())}")
     assert exp.sum() > 0
     assert np.array_equal(got, exp)
-    with bcc.Scene(sc) as scene:   # the same pairs through a ragged plan
+    with _scene(bcc, sc, cull) as scene:   # the same pairs through a ragged plan
         E = sc.num_envs
         off = np.arange(E + 1, dtype=np.int64) * len(qs)
         ragged_edges = np.tile(edges, (E, 1, 1))
@@ -186,16 +218,17 @@ def test_window_boundaries_with_many_distant_obstacles(bcc, oracle, gpu):
     assert np.array_equal(got_r, exp.T.reshape(-1))
 
 
-def test_non_orthonormal_cuboids_are_never_culled(bcc, oracle, gpu):
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+def test_non_orthonormal_cuboids_are_never_culled(bcc, oracle, gpu, robot):
     """A cuboid whose axes are scaled (not a Euclidean distance) keeps the
     reference formula's verdicts: its box is unbounded, so it is always tested."""
     from batch_cc_amd import workloads as W
-    rng = np.random.default_rng(44)
-    qs = _free_configs(oracle, bcc, W, "panda", 4, rng)
-    spheres = _tested_spheres("panda")
+    rng = np.random.default_rng([44, RID[robot]])
+    qs = _free_configs(oracle, bcc, W, robot, 4, rng)
+    spheres = _tested_spheres(robot)
     envs = []
     for q in qs:
-        fk = oracle.fk(0, q)
+        fk = oracle.fk(RID[robot], q)
         for scale in (0.5, 0.9, 1.5, 3.0):
             for off in (-1e-3, 5e-3, 0.05, 0.2, 0.5):
                 s = spheres[int(rng.integers(0, len(spheres)))]
@@ -204,9 +237,9 @@ def test_non_orthonormal_cuboids_are_never_culled(bcc, oracle, gpu):
                 envs.append({name: rec[None, :]})
     sc = bcc.PackedScene.from_envs(envs)
     edges = np.stack([qs, qs], axis=1).astype(np.float32)
-    exp = oracle.check_dense(0, sc, edges, threads=16)
-    with bcc.Scene(sc) as scene:
-        got = bcc.check_dense("panda", scene, edges)
+    exp = oracle.check_dense(RID[robot], sc, edges, threads=16)
+    with _scene(bcc, sc, "always") as scene:
+        got = bcc.check_dense(robot, scene, edges)
     print(f"colliding pairs {int(exp.sum())} of {exp.size}")
     assert np.array_equal(got, exp)
 

@@ -258,3 +258,44 @@ def test_both_kernel_variants_match_oracle(bcc, oracle, gpu, robot):
                 assert np.array_equal(bcc.check_ragged(robot, scene, off, edges[pick]), exp[pick, env_of]), v
     finally:
         bcc.debug_force_variant(-1)
+
+
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+def test_high_resolution_matches_oracle(bcc, oracle, gpu, robot):
+    """Resolution 1024: uniform edges take ~100-250 steps per pair, and in
+    these sparse scenes many pairs are free and run every step.  Each step adds
+    delta once to the carried configuration (linear in the step count), in the
+    plain dense kernel, the learned-order dense kernel (random orders), and the
+    plain and learned-order ragged kernels."""
+    from batch_cc_amd import workloads as W
+    rng = np.random.default_rng(21)
+    sc = _sparse_scene(rng, 6, 3)
+    edges = W.uniform_edges(robot, 40, rng)
+    E, M = sc.num_envs, edges.shape[0]
+    res = 1024
+    exp = oracle.check_dense(ROBOTS[robot], sc, edges, res, threads=16)
+    n = np.array([oracle.steps(edges[m, 0], edges[m, 1], res)[0] for m in range(M)])
+    print(f"{robot}: steps per pair {n.min()}..{n.max()}, collision fraction {exp.mean():.3f}")
+    assert n.max() > 64 and 0.02 < exp.mean() < 0.98
+    o = np.array([rng.permutation(32)[:4] for _ in range(M * E)], np.uint32)
+    order = (o[:, 0] | o[:, 1] << 8 | o[:, 2] << 16 | o[:, 3] << 24).astype(np.uint32)
+    off = np.arange(E + 1, dtype=np.int64) * M
+    ragged_edges = np.ascontiguousarray(np.tile(edges, (E, 1, 1)))
+    with bcc.Scene(sc) as scene:
+        got = bcc.check_dense(robot, scene, edges, res)
+        de = bcc.DeviceBuffer.from_array(edges)
+        do = bcc.DeviceBuffer(E * M)
+        bcc.check_dense_ex(robot, scene, de, M, res, do, bcc.DeviceBuffer.from_array(order))
+        got_o = do.to_array(np.uint8, (M, E))
+        got_r = bcc.check_ragged(robot, scene, off, ragged_edges, res)
+        dre = bcc.DeviceBuffer.from_array(ragged_edges)
+        dro = bcc.DeviceBuffer(E * M)
+        rorder = bcc.DeviceBuffer.from_array(np.ascontiguousarray(order.reshape(M, E).T.reshape(-1)))
+        plan = bcc.RaggedPlan(robot, scene, off, dre, dro, resolution=res, d_order=rorder)
+        plan.launch()
+        got_ro = dro.to_array(np.uint8, (E * M,))
+        plan.close()
+    assert np.array_equal(got, exp)
+    assert np.array_equal(got_o, exp)
+    assert np.array_equal(got_r, exp.T.reshape(-1))
+    assert np.array_equal(got_ro, exp.T.reshape(-1))
