@@ -96,10 +96,13 @@ def lib():
     _sig(L, "bcc_model_features", I, [P, P, P, ctypes.c_int64, P, P, P, P])
     _sig(L, "bcc_model_order_pairs", I, [P, P, P, ctypes.c_int64, P, P, P, P, P])
     _sig(L, "bcc_label_pairs", I, [I, P, P, I, P, P, I, P, P])
+    _sig(L, "bcc_label_pairs_steps", I, [I, P, P, I, P, P, I, I, P, P])
     _sig(L, "bcc_model_grads", I, [P, P, P, I, P, ctypes.POINTER(ctypes.c_float)])
     _sig(L, "bcc_model_train", I, [P, P, P, I, I, I, ctypes.c_float, P])
     _sig(L, "bcc_model_logits", I, [P, P, ctypes.c_int64, P])
     _sig(L, "bcc_model_order_dense", I, [P, P, P, I, P, P])
+    _sig(L, "bcc_time_learned_dense", I, [P, P, P, I, I, P, P, I, I, P, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_model_logits_dense", I, [P, P, P, I, P, P, P])
     _sig(L, "bcc_debug_gemm", I, [I, I, P, I, P, I, P, I, I, I, I, ctypes.c_float, ctypes.c_float])
     _sig(L, "bcc_malloc", I, [ctypes.POINTER(P), ctypes.c_size_t])
@@ -532,15 +535,17 @@ def time_dense_ex(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, re
     return ms.value
 
 
-def label_pairs(robot, scene: Scene, d_edges: DeviceBuffer, pair_env, pair_edge, resolution: int = 32):
-    """Bin masks (uint32 per pair) from the exact checker."""
+def label_pairs(robot, scene: Scene, d_edges: DeviceBuffer, pair_env, pair_edge, resolution: int = 32,
+                steps: int = 0):
+    """Bin masks (uint32 per pair) from the exact checker: bit t = one of the
+    first `steps` samples of bin t collides (0: any of its samples)."""
     pe = np.ascontiguousarray(pair_env, np.int32)
     pg = np.ascontiguousarray(pair_edge, np.int32)
     n = pe.size
     dpe, dpg = DeviceBuffer.from_array(pe), DeviceBuffer.from_array(pg)
     dl = DeviceBuffer(n * 4)
-    _check(lib().bcc_label_pairs(robot_id(robot), scene.handle, d_edges.ptr, n, dpe.ptr, dpg.ptr, int(resolution),
-                                 dl.ptr, None))
+    _check(lib().bcc_label_pairs_steps(robot_id(robot), scene.handle, d_edges.ptr, n, dpe.ptr, dpg.ptr,
+                                       int(resolution), int(steps), dl.ptr, None))
     synchronize()
     return dl
 
@@ -616,6 +621,17 @@ class Model:
         _check(lib().bcc_model_order_dense(self.handle, scene.handle, d_edges.ptr, int(num_edges), d.ptr, None))
         synchronize()
         return d
+
+    def time_learned_dense(self, scene: Scene, d_edges: DeviceBuffer, num_edges: int, d_order: DeviceBuffer,
+                           d_out: DeviceBuffer, warmup: int, iters: int, resolution: int = 32) -> tuple:
+        """(step ms, inference ms): hipEvent-timed learned steps over a dense
+        batch (inference for every pair into d_order, then the ordered check
+        into d_out), and the inference alone."""
+        step, order = ctypes.c_double(0.0), ctypes.c_double(0.0)
+        _check(lib().bcc_time_learned_dense(self.handle, scene.handle, d_edges.ptr, int(num_edges), int(resolution),
+                                            d_order.ptr, d_out.ptr, int(warmup), int(iters), None,
+                                            ctypes.byref(step), ctypes.byref(order)))
+        return step.value, order.value
 
     def logits_dense(self, scene: Scene, d_edges: DeviceBuffer, num_edges: int):
         """(order uint32 [M, E], logits float32 [M, E, 32]) from the fused kernel."""

@@ -865,15 +865,15 @@ __global__ __launch_bounds__(kBlock) void work_kernel(const float* __restrict__ 
     }
 }
 
-// Training labels: for each pair, bit t of labels[p] is set when bin t (lane
-// t's n samples) holds a colliding sample.  Every sample is evaluated (no
-// early exit) with the reference-order replay.  Pair p: env = pair_env[p],
-// edge = pair_edge[p].
+// Training labels: for each pair, bit t of labels[p] is set when one of the
+// first `steps` samples of bin t (lane t's samples t n + i, i < steps; steps
+// = 0: all n) collides.  Every such sample is evaluated (no early exit) with
+// the reference-order replay.  Pair p: env = pair_env[p], edge = pair_edge[p].
 template <class Robot>
 __global__ __launch_bounds__(kBlock) void label_kernel(const float* __restrict__ ref_obs,
                                                        const int* __restrict__ ref_meta,
                                                        const float* __restrict__ edges, int num_pairs,
-                                                       int resolution, ModelDev model,
+                                                       int resolution, int steps, ModelDev model,
                                                        const int* __restrict__ pair_env,
                                                        const int* __restrict__ pair_edge,
                                                        uint32_t* __restrict__ labels)
@@ -894,7 +894,8 @@ __global__ __launch_bounds__(kBlock) void label_kernel(const float* __restrict__
             q[j] = fmaf(d[j], tn, s[j]);
         bool any = false;
         uint64_t w = 0, tr = 0;
-        for (int i = 0; i < n; ++i) {
+        const int last = steps > 0 ? min(n, steps) : n;
+        for (int i = 0; i < last; ++i) {
             any = ref_sample_work<Robot>(q, R, model, w, tr) || any;
 #pragma unroll
             for (int j = 0; j < D; ++j)
@@ -912,6 +913,7 @@ struct LabelArgs {
     const float* edges;
     int num_pairs;
     int resolution;
+    int steps;   // samples per bin evaluated (0: all)
     ModelDev model;
     const int* pair_env;
     const int* pair_edge;
@@ -925,7 +927,7 @@ hipError_t launch_label(const LabelArgs& a, hipStream_t stream)
         return hipSuccess;
     const long long blocks = std::min<long long>(((long long)a.num_pairs + 7) / 8, 65536);
     hipLaunchKernelGGL(label_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.ref_obs, a.ref_meta,
-                       a.edges, a.num_pairs, a.resolution, a.model, a.pair_env, a.pair_edge, a.labels);
+                       a.edges, a.num_pairs, a.resolution, a.steps, a.model, a.pair_env, a.pair_edge, a.labels);
     return hipGetLastError();
 }
 

@@ -559,6 +559,8 @@ struct bcc_model {
     float* lo = nullptr;      // joint lower limits [dim]
     float* range = nullptr;   // joint ranges [dim]
     uint16_t* frag = nullptr; // MFMA A fragments of the BF16 weights (fused inference)
+    float* enc = nullptr;     // environment codes of the last scene ordered [enc_envs][64]
+    int enc_envs = 0;
     float* ws = nullptr;      // split-K partial tiles (weight gradients)
     int cap = 0;              // scratch rows
     uint16_t *h1 = nullptr, *h2 = nullptr, *g1 = nullptr, *g2 = nullptr, *g3 = nullptr;
@@ -674,7 +676,7 @@ void destroy_model(bcc_model* M)
         return;
     free_scratch(M);
     for (void* p : {(void*)M->w, (void*)M->wb, (void*)M->g, (void*)M->m, (void*)M->v, (void*)M->lo,
-                    (void*)M->range, (void*)M->loss, (void*)M->frag, (void*)M->ws})
+                    (void*)M->range, (void*)M->loss, (void*)M->frag, (void*)M->ws, (void*)M->enc})
         (void)hipFree(p);
     delete M;
 }
@@ -865,19 +867,27 @@ int bcc_model_features(const bcc_model* M, const bcc_scene* scene, const float* 
     return BCC_OK;
 }
 
-int bcc_label_pairs(int robot, const bcc_scene* scene, const float* d_edges, int num_pairs,
-                    const int32_t* d_pair_env, const int32_t* d_pair_edge, int resolution, uint32_t* d_labels,
-                    void* stream)
+int bcc_label_pairs_steps(int robot, const bcc_scene* scene, const float* d_edges, int num_pairs,
+                          const int32_t* d_pair_env, const int32_t* d_pair_edge, int resolution, int steps,
+                          uint32_t* d_labels, void* stream)
 {
-    if (!scene || num_pairs < 0 || resolution < 1 || (num_pairs > 0 && (!d_edges || !d_pair_env || !d_pair_edge ||
-                                                                         !d_labels)))
+    if (!scene || num_pairs < 0 || resolution < 1 || steps < 0 ||
+        (num_pairs > 0 && (!d_edges || !d_pair_env || !d_pair_edge || !d_labels)))
         return fail(BCC_ERR_INVALID, "bad arguments");
     ModelDev md;
     if (int rc = bcc_detail::device_model(robot, &md))
         return rc;
-    LabelArgs a{scene->d_ref_obs, scene->d_ref_meta, d_edges, num_pairs, resolution, md, d_pair_env, d_pair_edge,
-                d_labels};
+    LabelArgs a{scene->d_ref_obs, scene->d_ref_meta, d_edges, num_pairs, resolution, steps, md, d_pair_env,
+                d_pair_edge, d_labels};
     return dispatch_label(robot, a, (hipStream_t)stream);
+}
+
+int bcc_label_pairs(int robot, const bcc_scene* scene, const float* d_edges, int num_pairs,
+                    const int32_t* d_pair_env, const int32_t* d_pair_edge, int resolution, uint32_t* d_labels,
+                    void* stream)
+{
+    return bcc_label_pairs_steps(robot, scene, d_edges, num_pairs, d_pair_env, d_pair_edge, resolution, 0, d_labels,
+                                 stream);
 }
 
 int bcc_model_grads(bcc_model* M, const uint16_t* d_x, const uint32_t* d_labels, int rows, float* d_grad,
@@ -959,36 +969,99 @@ int bcc_model_logits(bcc_model* M, const uint16_t* d_x, int64_t rows, float* d_l
 namespace {
 
 // Fused inference: pack the weight fragments, encode the environments, one
-// kernel for all layers and the top-4 selection.  Pairs: every (edge, env)
-// pair of a dense batch, edge-major (pair_env null), or the list
-// (pair_env[p], pair_edge ? pair_edge[p] : p).  d_logits (optional) receives
-// [pairs][32].
+// kernel for all layers and the top-4 selection, all on stream s (no
+// synchronization).  Pairs: every (edge, env) pair of a dense batch,
+// edge-major (pair_env null), or the list (pair_env[p], pair_edge ?
+// pair_edge[p] : p).  d_logits (optional) receives [pairs][32].  The
+// environment codes live in a model-owned buffer, allocated only when a scene
+// with more environments than before is ordered.
 int order_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges, long long pairs, const int* pair_env,
                 const int* pair_edge, uint32_t* d_order, float* d_logits, hipStream_t s)
 {
     if (pairs == 0 || scene->num_envs == 0)
         return BCC_OK;
-    float* enc = nullptr;
-    BCC_HIP(hipMalloc((void**)&enc, sizeof(float) * kEnc * scene->num_envs), "order scratch");
+    if (M->enc_envs < scene->num_envs) {
+        BCC_HIP(hipStreamSynchronize(s), "order scratch");
+        (void)hipFree(M->enc);
+        M->enc = nullptr;
+        M->enc_envs = 0;
+        BCC_HIP(hipMalloc((void**)&M->enc, sizeof(float) * kEnc * scene->num_envs), "order scratch");
+        M->enc_envs = scene->num_envs;
+    }
     hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks_for((long long)kFrags * 64 * 4)), dim3(256), 0, s,
                        M->wb + P_W1, M->wb + P_W2, M->wb + P_W3, M->frag);
     hipLaunchKernelGGL(encode_env_kernel, dim3(scene->num_envs), dim3(64), 0, s, scene->d_ref_obs, scene->d_ref_meta,
-                       scene->num_envs, enc);
+                       scene->num_envs, M->enc);
     hipLaunchKernelGGL(fused_order_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s,
-                       (const bf16x4*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, enc, d_edges, M->dim, M->lo,
+                       (const bf16x4*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, M->enc, d_edges, M->dim, M->lo,
                        M->range, scene->num_envs, pair_env, pair_edge, pairs, d_order, d_logits);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess)
-        e = hipStreamSynchronize(s);
-    (void)hipFree(enc);
-    if (e != hipSuccess)
+    if (hipError_t e = hipGetLastError(); e != hipSuccess)
         return hip_fail(e, "fused order");
+    return BCC_OK;
+}
+
+// Average over `iters` runs (after `warmup`) of what `step` launches on s,
+// hipEvent-timed on s.
+template <class F>
+int time_on_stream(int warmup, int iters, hipStream_t s, double* avg_ms, F step)
+{
+    for (int i = 0; i < warmup; ++i)
+        if (int rc = step())
+            return rc;
+    hipEvent_t t0, t1;
+    BCC_HIP(hipEventCreate(&t0), "hipEventCreate");
+    hipError_t e = hipEventCreate(&t1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    int rc = BCC_OK;
+    e = hipEventRecord(t0, s);
+    for (int i = 0; i < iters && e == hipSuccess && rc == BCC_OK; ++i)
+        rc = step();
+    if (e == hipSuccess)
+        e = hipEventRecord(t1, s);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(t1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc != BCC_OK)
+        return rc;
+    if (e != hipSuccess)
+        return hip_fail(e, "learned step timing");
+    *avg_ms = (double)ms / iters;
     return BCC_OK;
 }
 
 }  // namespace
 
 extern "C" {
+
+int bcc_time_learned_dense(bcc_model* M, const bcc_scene* scene, const float* d_edges, int num_edges,
+                           int resolution, uint32_t* d_order, uint8_t* d_results, int warmup, int iters,
+                           void* stream, double* step_ms, double* order_ms)
+{
+    if (!M || !scene || num_edges < 0 || (num_edges > 0 && (!d_edges || !d_order || !d_results)) || iters < 1 ||
+        warmup < 0 || !step_ms)
+        return fail(BCC_ERR_INVALID, "bad arguments");
+    hipStream_t s = (hipStream_t)stream;
+    const long long pairs = (long long)num_edges * scene->num_envs;
+    auto order = [&]() { return order_fused(M, scene, d_edges, pairs, nullptr, nullptr, d_order, nullptr, s); };
+    auto step = [&]() {
+        if (int rc = order())
+            return rc;
+        return bcc_check_dense_ex(M->robot, scene, d_edges, num_edges, resolution, d_order, nullptr, d_results,
+                                  stream);
+    };
+    if (int rc = time_on_stream(warmup, iters, s, step_ms, step))
+        return rc;
+    if (order_ms)
+        return time_on_stream(warmup, iters, s, order_ms, order);
+    return BCC_OK;
+}
 
 int bcc_model_order_dense(bcc_model* M, const bcc_scene* scene, const float* d_edges, int num_edges,
                           uint32_t* d_order, void* stream)

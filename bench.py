@@ -1,17 +1,32 @@
-"""Benchmark: edge-env checks/sec of the plain batch edge collision checker on
-one MI355X, with its roofline and the CPU baseline beside it.
+"""Benchmark: edge-env checks/sec on one MI355X, learned-ordered vs plain,
+with the roofline and the CPU baseline beside it.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c1] [--seed 0]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c3] [--seed 0]
 
-A step is one launch of the dense kernel over the whole synthetic workload
-(every edge against every environment), inputs resident in HBM.  Timing is
-hipEvents on the launch stream around exactly K launches after W untimed
-ones.  The roofline's algorithmic work is the reference's own source-level
-FP32 operation count for the same pairs in plain order, counted on the GPU by
-the reference-order replay kernel (bcc_work_dense).  The CPU baseline is the
-project's C restatement of the reference checker (the reference's CPU path,
-vamp, is not available), timed on the host cores on a bounded sample.
-Prints one JSON line.
+Default workload: c3 (panda, 1000 environments x 100000 edges, 16 cuboids,
+8 spheres and 8 cylinders per environment).  A step is one pass of the path
+over the whole synthetic batch with inputs resident in HBM:
+
+  learned step (`value`): the step-ordering model ranks the sample bins of
+      every (env, edge) pair (fused MFMA inference), then the ordered kernel
+      checks each pair's step-0 samples of its two top bins, several pairs per
+      wave, and runs the plain step loop for the pairs still undecided;
+  plain step (`plain_value`): the reference's order, 32 samples per step.
+
+Both are timed with hipEvents on the launch stream around exactly K steps
+after W untimed ones.  The model is trained first, on the GPU, on labels the
+exact checker writes for a separate seeded edge stream.  Verdicts of every
+variant are compared with the plain kernel's (0 mismatches, else exit 1).
+Beside the headline: the same ordered kernel with evenly spaced probe bins
+(the method without the network), with perfect bins taken from exact labels
+of the benchmark pairs themselves (the bound on any order), and how many
+colliding pairs each probe decides.
+
+The roofline's algorithmic work is the reference's own source-level FP32
+operation count for the same pairs in plain order (bcc_work_dense), divided
+by the dominant kernel's time.  The CPU baseline is the project's C
+restatement of the reference checker (the reference's CPU path, vamp, is not
+available), timed on the host cores on a bounded sample.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -27,21 +42,17 @@ import numpy as np
 REPO = os.path.dirname(os.path.abspath(__file__))
 FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet)
 BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 MFMA peak (datasheet)
+HBM_PEAK_GBS = 8000.0
 DIM = {"panda": 7, "fetch": 8, "baxter": 14}
+PROBE = 2                    # predicted bins probed per pair (kProbe1 in csrc/bcc_dense.h)
+METRIC = "edge-env checks/sec at bit-exact verdicts, learned-ordered vs plain, 1 GPU"
+RID = {"panda": 0, "fetch": 1, "baxter": 2}
 
 
 def mlp_flops(robot: str) -> int:
     """Algorithmic flops of one inference: 3 layers over the unpadded input
     (64-cell encoding + start and end)."""
     return 2 * ((64 + 2 * DIM[robot]) * 256 + 256 * 256 + 256 * 32)
-HBM_PEAK_GBS = 8000.0
-METRIC = "edge-env checks/sec at bit-exact verdicts, learned-ordered vs plain, 1 GPU"
-# HBM bytes per launch of the dense kernel, measured with rocprofv3 FETCH_SIZE
-# (x2.000) + WRITE_SIZE (x1.000), the scales read off a plain copy kernel;
-# commands and raw counts in profiles/dense_kernel_fetch_c1.md.  Only the
-# default workload is measured; re-measure after kernel changes.
-MEASURED_TRAFFIC = {"c1": (2 * 3.156683e6 * 1024 + 8.665456e6 * 1024,
-                           "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/dense_kernel_fetch_c1.md")}
 
 
 def load_pkg():
@@ -54,80 +65,99 @@ def load_pkg():
     return mod
 
 
-def cpu_baseline(robot, sc, edges, seconds=10.0, max_threads=16):
-    """Time the oracle (CPU restatement) on a seeded random sample of pairs."""
+def log(msg: str):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+# ------------------------------------------------------------ CPU baseline
+
+def host_cpu() -> dict:
+    """The host's CPU model, core count and the share this process may use
+    (its affinity mask, capped by OMP_NUM_THREADS where the host sets it)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        share = min(share, int(cap))
+    return {"model": model, "nproc": os.cpu_count() or 1, "threads": max(1, share)}
+
+
+def _time_oracle(run, n_start: int, n_max: int, seconds: float):
+    """Grow a sample until `run(n)` takes `seconds` (or covers n_max)."""
+    n = n_start
+    while True:
+        t0 = time.perf_counter()
+        run(n)
+        dt = time.perf_counter() - t0
+        if dt >= seconds or n >= n_max:
+            return n, dt
+        n = int(min(n_max, max(n * 2, n * seconds / max(dt, 1e-3) * 1.1)))
+
+
+def cpu_baseline(robot, sc, edges, pair_env=None, seconds=10.0, one_thread_seconds=4.0):
+    """The oracle (the project's C restatement of the reference checker) on
+    seeded random pairs of the same workload: all the host cores this process
+    may use, then one thread.  Dense when pair_env is None (pairs = (edge,
+    env)); ragged otherwise (pair p = (edge p, env pair_env[p]))."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib
     orc = oracle_lib.Oracle()
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(cores, max_threads))
-    E, M = sc.num_envs, edges.shape[0]
-    rid = {"panda": 0, "fetch": 1, "baxter": 2}[robot]
+    cpu = host_cpu()
+    rid = RID[robot]
     rng = np.random.default_rng(1234)
-    n = 2000
-    while True:
-        pe = rng.integers(0, E, n).astype(np.int32)
-        pg = rng.integers(0, M, n).astype(np.int32)
-        t0 = time.perf_counter()
-        orc.check_pairs(rid, sc, edges, pe, pg, threads=threads)
-        dt = time.perf_counter() - t0
-        if dt >= seconds or n >= E * M:
-            break
-        n = int(min(E * M, max(n * 2, n * seconds / max(dt, 1e-3) * 1.1)))
+    E, M = sc.num_envs, edges.shape[0]
+    total = M if pair_env is not None else E * M
+
+    def run(threads):
+        def go(n):
+            if pair_env is None:
+                pe = rng.integers(0, E, n).astype(np.int32)
+                pg = rng.integers(0, M, n).astype(np.int32)
+            else:
+                pg = rng.integers(0, M, n).astype(np.int32)
+                pe = pair_env[pg]
+            orc.check_pairs(rid, sc, edges, pe, pg, threads=threads)
+        return go
+
+    n_all, dt_all = _time_oracle(run(cpu["threads"]), 2000, total, seconds)
+    n_one, dt_one = _time_oracle(run(1), 200, total, one_thread_seconds)
     return {
-        "value": n / dt,
+        "value": n_all / dt_all,
         "unit": "edge-env checks/s",
-        "cores": threads,
+        "cores": cpu["threads"],
         "kind": "port",
-        "sample": f"{n} random (env, edge) pairs of the same workload, {dt:.1f} s, "
-                  f"C restatement of the reference checker (vamp unavailable), {threads} threads",
+        "sample": f"{n_all} random (env, edge) pairs of the same workload in {dt_all:.1f} s on {cpu['threads']} "
+                  f"threads; C restatement of the reference checker (vamp unavailable)",
+        "one_thread": {"value": n_one / dt_one, "sample": f"{n_one} random pairs in {dt_one:.1f} s"},
+        "host": {"cpu_model": cpu["model"], "nproc": cpu["nproc"], "threads_used": cpu["threads"]},
     }
 
 
-def cpu_baseline_pairs(robot, sc, edges, pair_env, seconds=10.0, max_threads=16):
-    """CPU baseline on a ragged batch: a seeded random sample of its pairs."""
-    sys.path.insert(0, os.path.join(REPO, "tests"))
-    import oracle_lib
-    orc = oracle_lib.Oracle()
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    threads = max(1, min(cores, max_threads))
-    total = edges.shape[0]
-    rid = {"panda": 0, "fetch": 1, "baxter": 2}[robot]
-    rng = np.random.default_rng(1234)
-    n = 500
-    while True:
-        idx = rng.integers(0, total, n).astype(np.int32)
-        t0 = time.perf_counter()
-        orc.check_pairs(rid, sc, edges, pair_env[idx], idx, threads=threads)
-        dt = time.perf_counter() - t0
-        if dt >= seconds or n >= total:
-            break
-        n = int(min(total, max(n * 2, n * seconds / max(dt, 1e-3) * 1.1)))
-    return {"value": n / dt, "unit": "edge-env checks/s", "cores": threads, "kind": "port",
-            "sample": f"{n} random pairs of the ragged batch, {dt:.1f} s, C restatement of the reference "
-                      f"checker (vamp unavailable), {threads} threads"}
+# ------------------------------------------------------------ learned step
 
-
-def learned_leg(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, plain_ms):
-    """Train the step-ordering model on labels the exact checker writes for
-    seeded (env, edge) pairs drawn with a separate edge stream, then time one
-    learned step = model inference for every pair + the ordered check."""
+def train_model(bcc, W, a, robot, scene, E, n_train_edges, edge_law):
+    """The step-ordering model trained on the GPU: step-0 labels (the samples
+    the ordered kernel's probe checks) of a.train_pairs seeded (env, edge)
+    pairs of a separate edge stream, written by the exact checker."""
     rng = np.random.default_rng([a.seed, 3])
-    n_train_edges = max(1, min(M, 100_000))
-    train_edges = W.uniform_edges(robot, n_train_edges, rng) if robot != "fetch" else \
-        W.lognormal_edges(robot, n_train_edges, rng)
+    train_edges = edge_law(robot, n_train_edges, rng)
     d_train = bcc.DeviceBuffer.from_array(train_edges)
     n = int(a.train_pairs)
     pe = rng.integers(0, E, n).astype(np.int32)
     pg = rng.integers(0, n_train_edges, n).astype(np.int32)
     t0 = time.perf_counter()
-    d_labels = bcc.label_pairs(robot, scene, d_train, pe, pg)
+    d_labels = bcc.label_pairs(robot, scene, d_train, pe, pg, steps=1)
     t_label = time.perf_counter() - t0
     model = bcc.Model(robot, seed=a.seed)
     t0 = time.perf_counter()
@@ -135,64 +165,117 @@ def learned_leg(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, plai
     losses = model.train(d_x, d_labels, n, epochs=a.train_epochs, batch=8192, lr=2e-3)
     bcc.synchronize()
     t_train = time.perf_counter() - t0
-    print(f"[bench] learned: {n} labels in {t_label:.1f} s, {a.train_epochs} epochs in {t_train:.1f} s, "
-          f"loss {losses[0]:.3f} -> {losses[-1]:.3f}", file=sys.stderr, flush=True)
-    del d_x
+    log(f"learned: {n} step-0 labels in {t_label:.1f} s, {a.train_epochs} epochs in {t_train:.1f} s, "
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    info = {"pairs": n, "epochs": a.train_epochs, "labels": "step-0 samples of each bin (steps=1)",
+            "label_s": t_label, "train_s": t_train, "loss_first_epoch": float(losses[0]),
+            "loss_last_epoch": float(losses[-1]), "edges": "separate seeded edge stream, same environments"}
+    return model, info
+
+
+def perfect_orders(labels: np.ndarray) -> np.ndarray:
+    """Order words whose first two bins hold colliding step-0 samples where
+    the pair has any (lowest bins first), padded with other distinct bins."""
+    L = labels.astype(np.uint64)
+
+    def ctz(x):
+        low = x & (~x + np.uint64(1))
+        return np.where(x != 0, np.log2(np.maximum(low, 1).astype(np.float64)).astype(np.int64), -1)
+
+    b0 = ctz(L)
+    b1 = ctz(L & (L - np.uint64(1)))
+    b0 = np.where(b0 < 0, 16, b0)
+    b1 = np.where(b1 < 0, (b0 + 16) % 32, b1)
+    b2 = np.zeros_like(b0)
+    for c in (2, 1, 0):
+        b2 = np.where((b0 != c) & (b1 != c), c, b2)
+    b3 = np.zeros_like(b0)
+    for c in (3, 2, 1, 0):
+        b3 = np.where((b0 != c) & (b1 != c) & (b2 != c), c, b3)
+    return (b0 | (b1 << 8) | (b2 << 16) | (b3 << 24)).astype(np.uint32)
+
+
+def probe_decided(bcc, robot, scene, d_edges, M, E, d_order, plain_verdicts):
+    """Fraction of colliding pairs the ordered kernel's probe decides."""
     d_out = bcc.DeviceBuffer(E * M)
-    # one learned step: order every pair, then the ordered check
-    for _ in range(max(1, a.warmup)):
-        d_order = model.order_dense(scene, d_edges, M)
-        bcc.check_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order)
-    bcc.synchronize()
-    t_order = []
-    for _ in range(a.steps):
-        t0 = time.perf_counter()
-        d_order = model.order_dense(scene, d_edges, M)
-        bcc.synchronize()
-        t_order.append((time.perf_counter() - t0) * 1e3)
-    order_ms = float(np.median(t_order))
-    check_ms = bcc.time_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order, a.warmup, a.steps)
+    d_rounds = bcc.DeviceBuffer(E * M)
+    bcc.check_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order, d_rounds)
+    r = d_rounds.to_array(np.uint8, (M, E))
+    coll = plain_verdicts == 1
+    return float((r[coll] == 0).mean()) if coll.any() else None
+
+
+def learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, plain_ms):
+    edge_law = W.lognormal_edges if robot == "fetch" else W.uniform_edges
+    model, train = train_model(bcc, W, a, robot, scene, E, max(1, min(M, 100_000)), edge_law)
+    d_order = bcc.DeviceBuffer(E * M * 4)
+    d_out = bcc.DeviceBuffer(E * M)
+    step_ms, order_ms = model.time_learned_dense(scene, d_edges, M, d_order, d_out, a.warmup, a.steps)
     got = d_out.to_array(np.uint8, (M, E))
     mismatches = int((got != plain_verdicts).sum())
-    # rounds per pair, plain vs learned
-    d_rounds = bcc.DeviceBuffer(E * M)
-    bcc.check_dense_ex(robot, scene, d_edges, M, 32, d_out, None, d_rounds)
-    r_plain = d_rounds.to_array(np.uint8, (M, E)).astype(np.float64)
-    bcc.check_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order, d_rounds)
-    r_learn = d_rounds.to_array(np.uint8, (M, E)).astype(np.float64)
-    coll = plain_verdicts == 1
-    print(f"[bench] learned step: order {order_ms:.2f} ms + ordered check {check_ms:.3f} ms "
-          f"(plain check {plain_ms:.3f} ms); verdict mismatches {mismatches}", file=sys.stderr, flush=True)
-    if mismatches:
-        print("[bench] learned-ordered verdicts differ from plain order", file=sys.stderr)
-        sys.exit(1)
-    mfma = mlp_flops(robot) * E * M / (order_ms / 1e3) / 1e12
-    return {
-        "ms_per_step": order_ms + check_ms,
-        "order_ms": order_ms,
+    check_ms = bcc.time_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order, a.warmup, a.steps)
+    mismatches += int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
+    log(f"learned step {step_ms:.3f} ms = inference {order_ms:.3f} ms + ordered check (alone {check_ms:.3f} ms); "
+        f"plain {plain_ms:.3f} ms; verdict mismatches {mismatches}")
+    res = {
+        "ms_per_step": step_ms,
+        "inference_ms": order_ms,
         "ordered_check_ms": check_ms,
-        "inference_mfma": {"achieved": mfma, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": mfma / BF16_PEAK_TFLOPS, "flops_per_pair": mlp_flops(robot),
-                           "note": "wall time of the inference call (fragment packing, encoder, fused kernel)"},
-        "speedup_vs_plain": plain_ms / (order_ms + check_ms),
-        "check_only_speedup_vs_plain": plain_ms / check_ms,
+        "speedup_vs_plain": plain_ms / step_ms,
         "verdict_mismatches_vs_plain": mismatches,
-        "mean_rounds_colliding_plain": float(r_plain[coll].mean()) if coll.any() else None,
-        "mean_rounds_colliding_learned": float(r_learn[coll].mean()) if coll.any() else None,
-        "mean_rounds_all_plain": float(r_plain.mean()),
-        "mean_rounds_all_learned": float(r_learn.mean()),
-        "train": {"pairs": n, "epochs": a.train_epochs, "label_s": t_label, "train_s": t_train,
-                  "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
-                  "edges": "separate seeded edge stream, same environments"},
+        "inference_mfma": {"achieved": mlp_flops(robot) * E * M / (order_ms / 1e3) / 1e12,
+                           "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "flops_per_pair": mlp_flops(robot),
+                           "note": "hipEvent time of the inference (fragment packing, environment encoding, "
+                                   "fused 3-layer BF16 MFMA kernel with the top-4 selection)"},
+        "probe_decided_colliding": {"trained": probe_decided(bcc, robot, scene, d_edges, M, E, d_order,
+                                                             plain_verdicts)},
+        "train": train,
     }
+    res["inference_mfma"]["frac"] = res["inference_mfma"]["achieved"] / BF16_PEAK_TFLOPS
+    # the method without the network: evenly spaced probe bins
+    d_none = bcc.DeviceBuffer.from_array(np.full(E * M, 0xffffffff, np.uint32))
+    none_ms = bcc.time_dense_ex(robot, scene, d_edges, M, 32, d_out, d_none, a.warmup, a.steps)
+    none_bad = int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
+    res["no_network"] = {"ms_per_step": none_ms, "speedup_vs_plain": plain_ms / none_ms,
+                         "verdict_mismatches_vs_plain": none_bad,
+                         "note": "the same ordered kernel probing evenly spaced bins, no inference"}
+    res["probe_decided_colliding"]["evenly_spaced"] = probe_decided(bcc, robot, scene, d_edges, M, E, d_none,
+                                                                    plain_verdicts)
+    mismatches += none_bad
+    del d_none
+    if a.bound:
+        # perfect bins from exact step-0 labels of every benchmark pair
+        t0 = time.perf_counter()
+        pe = np.tile(np.arange(E, dtype=np.int32), M)
+        pg = np.repeat(np.arange(M, dtype=np.int32), E)
+        lab = bcc.label_pairs(robot, scene, d_edges, pe, pg, steps=1).to_array(np.uint32, (E * M,))
+        del pe, pg
+        d_best = bcc.DeviceBuffer.from_array(perfect_orders(lab))
+        del lab
+        best_ms = bcc.time_dense_ex(robot, scene, d_edges, M, 32, d_out, d_best, a.warmup, a.steps)
+        best_bad = int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
+        res["oracle_bound"] = {"ordered_check_ms": best_ms, "speedup_vs_plain": plain_ms / best_ms,
+                               "verdict_mismatches_vs_plain": best_bad,
+                               "note": "ordered check with perfect probe bins from exact step-0 labels of the "
+                                       "benchmark pairs (no inference): the bound on any learned order"}
+        res["oracle_bound_speedup"] = plain_ms / best_ms
+        res["probe_decided_colliding"]["perfect"] = probe_decided(bcc, robot, scene, d_edges, M, E, d_best,
+                                                                  plain_verdicts)
+        mismatches += best_bad
+        log(f"no network {none_ms:.3f} ms, perfect bins {best_ms:.3f} ms ({time.perf_counter() - t0:.1f} s)")
+    if mismatches:
+        log("ordered verdicts differ from plain order")
+        sys.exit(1)
+    model.close()
+    return res
 
 
 def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
     """Learned leg of the mixed-robot batch: one model per robot, trained on
-    labels the exact checker writes for seeded (env, edge) pairs of that
-    robot's environments with a separate lognormal edge stream (the batch's
-    edge law); one learned step = inference for every pair of every robot
-    (bcc_model_order_pairs) + one replay of a group of ordered plans."""
+    step-0 labels of seeded (env, edge) pairs of that robot's environments
+    with a separate lognormal edge stream (the batch's edge law); one learned
+    step = inference for every pair of every robot (bcc_model_order_pairs) +
+    one replay of a group of ordered plans."""
     n = int(a.train_pairs)
     for k, r in enumerate(runs):
         rng = np.random.default_rng([a.seed, 3, k])
@@ -203,7 +286,7 @@ def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
         pe = rng.integers(0, E, n).astype(np.int32)
         pg = rng.integers(0, n_edges, n).astype(np.int32)
         t0 = time.perf_counter()
-        d_labels = bcc.label_pairs(r["robot"], r["scene"], d_train, pe, pg)
+        d_labels = bcc.label_pairs(r["robot"], r["scene"], d_train, pe, pg, steps=1)
         model = bcc.Model(r["robot"], seed=a.seed)
         d_x = model.features(r["scene"], d_train, pe, pg)
         losses = model.train(d_x, d_labels, n, epochs=a.train_epochs, batch=8192, lr=2e-3)
@@ -211,10 +294,8 @@ def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
         del d_x, d_labels, d_train
         r["model"], r["losses"] = model, losses
         r["pair_env"] = np.repeat(np.arange(E, dtype=np.int32), np.diff(r["off"]))
-        print(f"[bench] learned {r['robot']}: {n} labels, {a.train_epochs} epochs in "
-              f"{time.perf_counter() - t0:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}",
-              file=sys.stderr, flush=True)
-    # inference time per robot (order every pair of the part)
+        log(f"learned {r['robot']}: {n} step-0 labels, {a.train_epochs} epochs in "
+            f"{time.perf_counter() - t0:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
     order_ms = {}
     for r in runs:
         d_env = bcc.DeviceBuffer.from_array(r["pair_env"])
@@ -239,32 +320,31 @@ def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
         r["plan_l"].close()
         r["model"].close()
     total_order = sum(order_ms.values())
-    print(f"[bench] learned step: order {total_order:.2f} ms ("
-          + ", ".join(f"{k} {v:.2f}" for k, v in order_ms.items())
-          + f") + ordered group {check_ms:.3f} ms (plain group {plain_ms:.3f} ms); verdict mismatches "
-          f"{mismatches}", file=sys.stderr, flush=True)
+    log(f"learned step: order {total_order:.2f} ms (" + ", ".join(f"{k} {v:.2f}" for k, v in order_ms.items())
+        + f") + ordered group {check_ms:.3f} ms (plain group {plain_ms:.3f} ms); verdict mismatches {mismatches}")
     if mismatches:
-        print("[bench] learned-ordered verdicts differ from plain order", file=sys.stderr)
+        log("learned-ordered verdicts differ from plain order")
         sys.exit(1)
     flops = sum(mlp_flops(r["robot"]) * r["edges"].shape[0] for r in runs)
     mfma = flops / (total_order / 1e3) / 1e12
     return {
         "ms_per_step": total_order + check_ms,
-        "order_ms": total_order,
-        "order_ms_per_robot": order_ms,
+        "inference_ms": total_order,
+        "inference_ms_per_robot": order_ms,
         "inference_mfma": {"achieved": mfma, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                            "frac": mfma / BF16_PEAK_TFLOPS,
                            "note": "wall time of the inference calls (fragment packing, encoder, fused kernel)"},
         "ordered_check_ms": check_ms,
         "speedup_vs_plain": plain_ms / (total_order + check_ms),
-        "check_only_speedup_vs_plain": plain_ms / check_ms,
         "verdict_mismatches_vs_plain": mismatches,
-        "train": {"pairs_per_robot": n, "epochs": a.train_epochs,
+        "train": {"pairs_per_robot": n, "epochs": a.train_epochs, "labels": "step-0 samples of each bin",
                   "loss_first_epoch": {r["robot"]: float(r["losses"][0]) for r in runs},
                   "loss_last_epoch": {r["robot"]: float(r["losses"][-1]) for r in runs},
                   "edges": "separate seeded lognormal edge stream per robot, same environments"},
     }
 
+
+# ------------------------------------------------------------ workloads
 
 def bench_mixed(bcc, W, a) -> dict:
     """c4: panda, fetch and baxter environments interleaved; one ragged plan
@@ -272,8 +352,8 @@ def bench_mixed(bcc, W, a) -> dict:
     t0 = time.perf_counter()
     parts, env_robot, global_off = W.make_mixed(seed=a.seed, scale=a.scale)
     total = int(global_off[-1])
-    print(f"[bench] workload c4: {len(env_robot)} interleaved envs, {total} pairs "
-          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    log(f"workload c4: {len(env_robot)} interleaved envs, {total} pairs (generated in "
+        f"{time.perf_counter() - t0:.1f} s)")
     runs = []
     for robot, sc, off, edges, gidx in parts:
         scene = bcc.Scene(sc)
@@ -295,10 +375,10 @@ def bench_mixed(bcc, W, a) -> dict:
     alone = {r["robot"]: r["plan"].time(1, max(1, a.steps // 2)) for r in runs}
     for r in runs:
         if not np.array_equal(r["d_out"].to_array(np.uint8, (r["edges"].shape[0],)), r["v"]):
-            print("[bench] verdicts changed between launches", file=sys.stderr)
+            log("verdicts changed between launches")
             sys.exit(1)
-    print(f"[bench] {a.steps} timed group launches: {ms:.3f} ms each; plans alone: "
-          + ", ".join(f"{k} {v:.3f} ms" for k, v in alone.items()), file=sys.stderr, flush=True)
+    log(f"{a.steps} timed group launches: {ms:.3f} ms each; plans alone: "
+        + ", ".join(f"{k} {v:.3f} ms" for k, v in alone.items()))
     learned = learned_mixed(bcc, W, a, runs, ms) if a.learned else None
     roofline = None
     if not a.no_work:
@@ -307,7 +387,7 @@ def bench_mixed(bcc, W, a) -> dict:
             pair_env = np.repeat(np.arange(r["sc"].num_envs, dtype=np.int32), np.diff(r["off"]))
             work, trig, wv = bcc.work_pairs(r["robot"], r["scene"], r["d_edges"], pair_env, 32)
             if not np.array_equal(wv, r["v"]):
-                print("[bench] work-counter verdicts differ from the ragged kernel", file=sys.stderr)
+                log("work-counter verdicts differ from the ragged kernel")
                 sys.exit(1)
             flops += float(work.sum(dtype=np.float64))
             trigs += float(trig.sum(dtype=np.float64))
@@ -320,19 +400,17 @@ def bench_mixed(bcc, W, a) -> dict:
                             "reference-order replay; sin/cos counted separately"}
     cpu = None
     if not a.no_cpu_baseline:
-        n_done, t_done, threads = 0, 0.0, 1
+        n_done, t_done, c = 0, 0.0, None
         for r in runs:
             pair_env = np.repeat(np.arange(r["sc"].num_envs, dtype=np.int32), np.diff(r["off"]))
-            c = cpu_baseline_pairs(r["robot"], r["sc"], r["edges"], pair_env, seconds=4.0)
-            n_s = int(c["sample"].split()[0])
+            c = cpu_baseline(r["robot"], r["sc"], r["edges"], pair_env, seconds=4.0, one_thread_seconds=1.5)
             # weight each robot by its share of the batch
             n_done += r["edges"].shape[0]
             t_done += r["edges"].shape[0] / c["value"]
-            threads = c["cores"]
-        cpu = {"value": n_done / t_done, "unit": "edge-env checks/s", "cores": threads, "kind": "port",
-               "sample": f"per robot a random sample of its pairs (about 4 s each, e.g. last {n_s} pairs), "
-                         f"combined by each robot's share of the batch; C restatement of the reference "
-                         f"checker (vamp unavailable), {threads} threads"}
+        cpu = {"value": n_done / t_done, "unit": "edge-env checks/s", "cores": c["cores"], "kind": "port",
+               "sample": "per robot a random sample of its pairs (about 4 s each), combined by each robot's share "
+                         "of the batch; C restatement of the reference checker (vamp unavailable)",
+               "host": c["host"]}
     desc = (f"{len(env_robot)} envs interleaved panda/fetch/baxter, {total} pairs "
             f"(U{{32..512}} obstacles, lognormal edges)")
     rec = {
@@ -340,7 +418,7 @@ def bench_mixed(bcc, W, a) -> dict:
         "config": {"workload": f"c4: {desc}, seed {a.seed}", "robot": "panda+fetch+baxter",
                    "envs": int(len(env_robot)), "edges": total,
                    "layout": "three ragged CSR batches (one per robot), one hipGraph with a branch per robot",
-                   "pairs": total, "resolution": 32, "order": "plain", "collision_fraction": coll,
+                   "pairs": total, "resolution": 32, "collision_fraction": coll,
                    "per_robot": {r["robot"]: {"pairs": int(r["edges"].shape[0]),
                                               "collision_fraction": float(r["v"].mean()),
                                               "ms_alone": alone[r["robot"]],
@@ -363,14 +441,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "c4", "mbm"],
+    ap.add_argument("--config", default="c3", choices=["c0", "c1", "c2", "c3", "c4", "mbm"],
                     help="mbm = the README run: 100 bookshelf_small scenes x 1000 Halton edges (panda)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--scale", type=float, default=1.0, help="env/edge count multiplier (1.0 = full size)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-work", action="store_true", help="skip the reference-order work count")
     ap.add_argument("--no-learned", dest="learned", action="store_false",
-                    help="skip training the step-ordering model and timing the learned-ordered check")
+                    help="plain order only: skip training and the learned-ordered step")
+    ap.add_argument("--no-bound", dest="bound", action="store_false",
+                    help="skip the perfect-bin bound (exact step-0 labels of every benchmark pair)")
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -397,12 +477,18 @@ def main():
 
     if a.config == "c4":
         part = bench_mixed(bcc, W, a)
-        rec = {"metric": METRIC, "value": part["value"], "unit": "edge-env checks/s", "n_gpus": 1,
-               "steps": a.steps, "warmup": a.warmup, "ms_per_step": part["ms_per_step"], "higher_is_better": True,
+        learned = part.get("learned")
+        rec = {"metric": METRIC, "value": learned["value"] if learned else part["value"], "unit": "edge-env checks/s",
+               "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+               "ms_per_step": learned["ms_per_step"] if learned else part["ms_per_step"], "higher_is_better": True,
                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-               "config": part["config"], "roofline": part["roofline"], "cpu_baseline": part["cpu_baseline"]}
-        if "learned" in part:
-            rec["learned"] = part["learned"]
+               "config": {**part["config"], "order": "learned" if learned else "plain"},
+               "plain_value": part["value"], "plain_ms_per_step": part["ms_per_step"],
+               "roofline": part["roofline"], "cpu_baseline": part["cpu_baseline"]}
+        if learned:
+            rec["speedup_vs_plain"] = learned["speedup_vs_plain"]
+            rec["verdict_mismatches_vs_plain"] = learned["verdict_mismatches_vs_plain"]
+            rec["learned"] = learned
         print(json.dumps(rec))
         return
 
@@ -429,8 +515,7 @@ def main():
         pairs = E * M
         shape = (M, E)
         desc = f"{robot}, {E} envs x {M} edges"
-    print(f"[bench] workload {a.config}: {desc} = {pairs} pairs "
-          f"(generated in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    log(f"workload {a.config}: {desc} = {pairs} pairs (generated in {time.perf_counter() - t0:.1f} s)")
 
     scene = bcc.Scene(sc)
     d_edges = bcc.DeviceBuffer.from_array(edges)
@@ -452,18 +537,16 @@ def main():
     else:
         ms = bcc.time_dense(robot, scene, d_edges, M, 32, d_out, a.warmup, a.steps)
     bcc.synchronize()
-    wall = time.perf_counter() - wall0
-    print(f"[bench] {a.steps} timed launches: {ms:.3f} ms each (wall incl. warmup {wall:.2f} s)",
-          file=sys.stderr, flush=True)
-    again = d_out.to_array(np.uint8, shape)
-    if not np.array_equal(again, verdicts):
-        print("[bench] verdicts changed between launches", file=sys.stderr)
+    log(f"plain: {a.steps} timed launches {ms:.3f} ms each (wall incl. warmup {time.perf_counter() - wall0:.2f} s)")
+    if not np.array_equal(d_out.to_array(np.uint8, shape), verdicts):
+        log("verdicts changed between launches")
         sys.exit(1)
-    value = pairs / (ms / 1e3)
+    plain_value = pairs / (ms / 1e3)
 
     learned = None
     if a.learned and not ragged:
-        learned = learned_leg(bcc, W, a, robot, scene, sc, d_edges, E, M, verdicts, ms)
+        learned = learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, verdicts, ms)
+        learned["value"] = pairs / (learned["ms_per_step"] / 1e3)
 
     roofline = None
     if not a.no_work:
@@ -474,45 +557,51 @@ def main():
         else:
             work, trig, wv = bcc.work_dense(robot, scene, d_edges, M, 32)
         if not np.array_equal(wv, verdicts):
-            print("[bench] work-counter verdicts differ from the dense kernel", file=sys.stderr)
+            log("work-counter verdicts differ from the dense kernel")
             sys.exit(1)
         flops = float(work.sum(dtype=np.float64))
         trigs = float(trig.sum(dtype=np.float64))
-        achieved = flops / (ms / 1e3) / 1e12
+        # the dominant kernel of the measured step: the ordered check of the
+        # learned step, else the plain kernel
+        kms = learned["ordered_check_ms"] if learned else ms
+        achieved = flops / (kms / 1e3) / 1e12
         roofline = {
             "bound": "valu",
             "achieved": achieved,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": MEASURED_TRAFFIC[a.config][0] if a.config in MEASURED_TRAFFIC and a.scale == 1.0 else None,
-            "traffic_source": MEASURED_TRAFFIC[a.config][1] if a.config in MEASURED_TRAFFIC else None,
-            "kernel": "ragged_kernel (hipGraph replay)" if ragged else "dense_kernel",
+            "traffic": None,
+            "kernel": ("ordered_dense_kernel (the learned step's check)" if learned else
+                       ("ragged_kernel (hipGraph replay)" if ragged else "dense_kernel")),
+            "kernel_ms": kms,
             "algorithmic_flops_per_launch": flops,
             "sincos_per_launch": trigs,
-            "note": "reference source-level FP32 ops in plain order, counted on the GPU by the "
-                    "reference-order replay (bcc_work_dense); sin/cos counted separately",
+            "plain_kernel": {"ms": ms, "achieved": flops / (ms / 1e3) / 1e12,
+                             "frac": flops / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS},
+            "note": "algorithmic work = the reference's source-level FP32 ops for these pairs in plain order "
+                    "(counted on the GPU by the reference-order replay, bcc_work_dense; sin/cos counted "
+                    "separately) / the kernel's hipEvent time; the kernel executes less than that work "
+                    "(early exits, culling, probing), so this is an effective rate",
         }
-        print(f"[bench] work count: {flops:.4g} flops/launch ({time.perf_counter() - t1:.1f} s)",
-              file=sys.stderr, flush=True)
+        log(f"work count: {flops:.4g} flops/launch ({time.perf_counter() - t1:.1f} s)")
 
     cpu = None
-    if not a.no_cpu_baseline and not ragged:
-        cpu = cpu_baseline(robot, sc, edges)
-    elif not a.no_cpu_baseline:
-        cpu = cpu_baseline_pairs(robot, sc, edges, np.repeat(np.arange(E, dtype=np.int32), np.diff(env_off)))
-    if cpu is not None:
-        print(f"[bench] cpu baseline {cpu['value']:.4g} pairs/s on {cpu['cores']} threads", file=sys.stderr,
-              flush=True)
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(robot, sc, edges, np.repeat(np.arange(E, dtype=np.int32), np.diff(env_off))
+                           if ragged else None)
+        log(f"cpu baseline {cpu['value']:.4g} pairs/s on {cpu['cores']} threads "
+            f"({cpu['one_thread']['value']:.4g} on one; {cpu['host']['cpu_model']}, nproc {cpu['host']['nproc']})")
 
+    step_ms = learned["ms_per_step"] if learned else ms
     rec = {
         "metric": METRIC,
-        "value": value,
+        "value": pairs / (step_ms / 1e3),
         "unit": "edge-env checks/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": step_ms,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -526,16 +615,19 @@ def main():
             "layout": "ragged CSR, one persistent launch replayed as a hipGraph" if ragged else "dense E x M",
             "pairs": pairs,
             "resolution": 32,
-            "order": "plain",
+            "order": "learned (probe of the 2 top predicted bins, then plain steps)" if learned else "plain",
             "collision_fraction": coll,
         },
+        "plain_value": plain_value,
+        "plain_ms_per_step": ms,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     if learned is not None:
-        # `value` is the plain-order path; the learned-ordered step (model
-        # inference for every pair + ordered check) is reported beside it
-        learned["value"] = pairs / (learned["ms_per_step"] / 1e3)
+        rec["speedup_vs_plain"] = learned["speedup_vs_plain"]
+        rec["verdict_mismatches_vs_plain"] = learned["verdict_mismatches_vs_plain"]
+        if "oracle_bound_speedup" in learned:
+            rec["oracle_bound_speedup"] = learned["oracle_bound_speedup"]
         rec["learned"] = learned
     print(json.dumps(rec))
     scene.close()

@@ -242,6 +242,11 @@ int bcc_model_features(const bcc_model *model, const bcc_scene *scene, const flo
 int bcc_label_pairs(int robot, const bcc_scene *scene, const float *d_edges, int num_pairs,
                     const int32_t *d_pair_env, const int32_t *d_pair_edge, int resolution, uint32_t *d_labels,
                     void *stream);
+/* the same over the first `steps` samples of each bin only (steps = 1: the
+ * step-0 samples, the ones the ordered kernels' probe checks; 0: all) */
+int bcc_label_pairs_steps(int robot, const bcc_scene *scene, const float *d_edges, int num_pairs,
+                          const int32_t *d_pair_env, const int32_t *d_pair_edge, int resolution, int steps,
+                          uint32_t *d_labels, void *stream);
 /* one forward + backward pass (mean binary cross-entropy), FP32 gradients into d_grad, no update */
 int bcc_model_grads(bcc_model *model, const uint16_t *d_x, const uint32_t *d_labels, int rows, float *d_grad,
                     float *loss);
@@ -251,7 +256,10 @@ int bcc_model_train(bcc_model *model, const uint16_t *d_x, const uint32_t *d_lab
 int bcc_model_logits(bcc_model *model, const uint16_t *d_x, int64_t rows, float *d_logits);
 /* top-4 bins of every (edge, env) pair of a dense batch into d_order [num_edges][num_envs]
  * (one fused kernel: features, three MFMA layers and the selection; same
- * values as bcc_model_logits on bcc_model_features rows) */
+ * values as bcc_model_logits on bcc_model_features rows).  The bcc_model_order_*
+ * and _logits_ calls run asynchronously on `stream` (results are ready when
+ * the stream is); they allocate only when a scene has more environments than
+ * any the model ordered before. */
 int bcc_model_order_dense(bcc_model *model, const bcc_scene *scene, const float *d_edges, int num_edges,
                           uint32_t *d_order, void *stream);
 /* the same, also writing every pair's 32 logits into d_logits [num_edges][num_envs][32] */
@@ -263,6 +271,13 @@ int bcc_model_logits_dense(bcc_model *model, const bcc_scene *scene, const float
 int bcc_model_order_pairs(bcc_model *model, const bcc_scene *scene, const float *d_edges, int64_t num_pairs,
                           const int32_t *d_pair_env, const int32_t *d_pair_edge, uint32_t *d_order,
                           float *d_logits, void *stream);
+/* One learned-order step over a dense batch, timed: bcc_model_order_dense
+ * into d_order, then bcc_check_dense_ex with that order, `iters` times after
+ * `warmup`, hipEvents on `stream`; step_ms = average step, order_ms
+ * (optional) = average of the inference alone, timed the same way. */
+int bcc_time_learned_dense(bcc_model *model, const bcc_scene *scene, const float *d_edges, int num_edges,
+                           int resolution, uint32_t *d_order, uint8_t *d_results, int warmup, int iters,
+                           void *stream, double *step_ms, double *order_ms);
 
 /* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
 int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,

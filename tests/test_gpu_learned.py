@@ -112,22 +112,27 @@ def _training_set(bcc, scene, sc, edges, robot, n, seed):
     return de, pe, pg, dl
 
 
-def test_labels_match_the_oracle_bin_by_bin(bcc, oracle, gpu):
+@pytest.mark.parametrize("steps", [0, 1, 2])
+def test_labels_match_the_oracle_bin_by_bin(bcc, oracle, gpu, steps):
+    """Bit t: one of the first `steps` samples of bin t collides (0: any)."""
     from batch_cc_amd import workloads as W
     robot, sc, edges = W.make_config("c0", seed=5, scale=0.05)
     with bcc.Scene(sc) as scene:
-        de, pe, pg, dl = _training_set(bcc, scene, sc, edges, robot, 60, 1)
-        lab = dl.to_array(np.uint32, (60,))
+        de, pe, pg, _ = _training_set(bcc, scene, sc, edges, robot, 60, 1)
+        lab = bcc.label_pairs(robot, scene, de, pe, pg, steps=steps).to_array(np.uint32, (60,))
+    multi = 0
     for p in range(60):
         s, e = edges[pg[p], 0], edges[pg[p], 1]
         n, d = oracle.steps(s, e, 32)
+        multi += n > 1
         bits = 0
         for t in range(32):
-            for i in range(n):
+            for i in range(n if steps == 0 else min(n, steps)):
                 if oracle.config_collides(0, sc, int(pe[p]), oracle.sample(s, d, n, t, i)):
                     bits |= 1 << t
                     break
         assert lab[p] == bits, p
+    assert multi > 30
 
 
 def test_training_reduces_loss_and_model_files_round_trip(bcc, gpu, tmp_path):

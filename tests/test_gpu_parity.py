@@ -299,3 +299,69 @@ def test_high_resolution_matches_oracle(bcc, oracle, gpu, robot):
     assert np.array_equal(got_o, exp)
     assert np.array_equal(got_r, exp.T.reshape(-1))
     assert np.array_equal(got_ro, exp.T.reshape(-1))
+
+
+def test_full_size_c0_matches_the_oracle(bcc, oracle, gpu):
+    """c0 at full size (panda, 100 envs x 1000 edges): every pair against the oracle."""
+    from batch_cc_amd import workloads as W
+    robot, sc, edges = W.make_config("c0", seed=0, scale=1.0)
+    got = _compare_dense(bcc, oracle, robot, sc, edges, label="c0 full")
+    print(f"c0: {got.size} pairs, collision fraction {got.mean():.4f}")
+
+
+def test_full_size_c3_is_deterministic_and_matches_sampled_pairs(bcc, oracle, gpu):
+    """c3 at full size (panda, 1000 envs x 100000 edges = 1e8 pairs): two
+    launches agree bit for bit and a seeded sample of 4000 pairs matches the
+    oracle."""
+    from batch_cc_amd import workloads as W
+    robot, sc, edges = W.make_config("c3", seed=0, scale=1.0)
+    E, M = sc.num_envs, edges.shape[0]
+    with bcc.Scene(sc) as scene:
+        de = bcc.DeviceBuffer.from_array(edges)
+        da, db = bcc.DeviceBuffer(E * M), bcc.DeviceBuffer(E * M)
+        bcc.check_dense_device(robot, scene, de, M, 32, da)
+        bcc.check_dense_device(robot, scene, de, M, 32, db)
+        a = da.to_array(np.uint8, (M, E))
+        b = db.to_array(np.uint8, (M, E))
+    assert np.array_equal(a, b)
+    rng = np.random.default_rng(7)
+    pe = rng.integers(0, E, 4000).astype(np.int32)
+    pg = rng.integers(0, M, 4000).astype(np.int32)
+    exp, _, _, _ = oracle.check_pairs(0, sc, edges, pe, pg, threads=16)
+    assert np.array_equal(a[pg, pe], exp)
+    print(f"c3: {a.size} pairs, collision fraction {a.mean():.4f}")
+
+
+def test_c3_learned_leg_matches_plain_and_the_oracle(bcc, oracle, gpu):
+    """c3's learned leg on c3's own scenes (16 cuboids, 8 spheres, 8 cylinders
+    per environment) at a tenth of its size (100 envs x 10000 edges): a model
+    trained on step-0 labels of a separate edge stream orders the pairs; the
+    ordered verdicts equal the plain ones, and a sample equals the oracle."""
+    from batch_cc_amd import workloads as W
+    robot, sc, edges = W.make_config("c3", seed=0, scale=0.1)
+    E, M = sc.num_envs, edges.shape[0]
+    rng = np.random.default_rng([0, 3])
+    train = W.uniform_edges(robot, 20000, rng)
+    n = 200_000
+    pe = rng.integers(0, E, n).astype(np.int32)
+    pg = rng.integers(0, train.shape[0], n).astype(np.int32)
+    with bcc.Scene(sc) as scene:
+        dt = bcc.DeviceBuffer.from_array(train)
+        dl = bcc.label_pairs(robot, scene, dt, pe, pg, steps=1)
+        model = bcc.Model(robot, seed=0)
+        losses = model.train(model.features(scene, dt, pe, pg), dl, n, epochs=4, batch=8192, lr=2e-3)
+        de = bcc.DeviceBuffer.from_array(edges)
+        dp, do = bcc.DeviceBuffer(E * M), bcc.DeviceBuffer(E * M)
+        bcc.check_dense_device(robot, scene, de, M, 32, dp)
+        order = model.order_dense(scene, de, M)
+        bcc.check_dense_ex(robot, scene, de, M, 32, do, order)
+        plain = dp.to_array(np.uint8, (M, E))
+        learned = do.to_array(np.uint8, (M, E))
+    print(f"c3/10: loss {losses[0]:.3f} -> {losses[-1]:.3f}, collision fraction {plain.mean():.4f}")
+    assert losses[-1] < losses[0]
+    assert np.array_equal(learned, plain)
+    s = np.random.default_rng(8)
+    spe = s.integers(0, E, 2000).astype(np.int32)
+    spg = s.integers(0, M, 2000).astype(np.int32)
+    exp, _, _, _ = oracle.check_pairs(0, sc, edges, spe, spg, threads=16)
+    assert np.array_equal(learned[spg, spe], exp)
