@@ -2,11 +2,20 @@
 //
 // Matrices are row-major; BF16 values are stored as raw 16-bit patterns
 // (uint16_t) and converted with round-to-nearest-even.  Products accumulate
-// in FP32 on v_mfma_f32_16x16x16_bf16, whose operand layout for a 16x16x16
-// tile is:
-//   A (16x16): lane l holds A[l % 16][4*(l/16) .. 4*(l/16)+3]
-//   B (16x16): lane l holds B[4*(l/16) .. 4*(l/16)+3][l % 16]
-//   D (16x16): lane l holds D[4*(l/16) + v][l % 16], v = 0..3
+// in FP32 on v_mfma_f32_16x16x32_bf16 (gfx950), whose operands for one
+// 16 x 16 x 32 tile are
+//   A (16x32): lane l holds A[l % 16][slots 8 (l/16) .. 8 (l/16) + 7]
+//   B (32x16): lane l holds B[slots 8 (l/16) .. 8 (l/16) + 7][l % 16]
+//   D (16x16): lane l holds D[4 (l/16) + v][l % 16], v = 0..3
+// Slot 8 g + i (g = l/16, i = 0..7) of a 32-wide k block carries logical k
+//   kslot(g, i) = 4 g + i (i < 4),  16 + 4 g + (i - 4) (i >= 4),
+// the same in A and B (so the product is the plain sum over k).  With this
+// assignment the D rows a lane holds of two stacked 16-row output tiles
+// (rows 4 g + v of tile 2 j and of tile 2 j + 1) are exactly its B operand
+// for the next layer's k block j: activations chain from layer to layer
+// without lane exchanges.  Every MFMA in this file uses it, so the fused and
+// the layer-by-layer paths add the same products in the same slots and agree
+// bit for bit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,14 +23,31 @@
 
 namespace bcc {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));    // eight BF16 bit patterns
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));    // the same eight BF16, two per word
 
+typedef __bf16 bf16v2 __attribute__((ext_vector_type(2)));
+typedef float f32v2 __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16, round to nearest even (v_cvt_pk_bf16_f32 on gfx950).
 __device__ __forceinline__ uint16_t f2bf(float f)
 {
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
-    return (uint16_t)(u >> 16);
+    return __builtin_bit_cast(uint16_t, (__bf16)f);
+}
+
+// Two values at once: low half = a, high half = b.
+__device__ __forceinline__ uint32_t f2bf2(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32v2{a, b}), bf16v2));
+}
+
+// Eight FP32 values as one MFMA operand.
+__device__ __forceinline__ bf16x8 pack8(const float (&v)[8])
+{
+    const u32x4 w{f2bf2(v[0], v[1]), f2bf2(v[2], v[3]), f2bf2(v[4], v[5]), f2bf2(v[6], v[7])};
+    return __builtin_bit_cast(bf16x8, w);
 }
 
 __device__ __forceinline__ float bf2f(uint16_t h)
@@ -29,21 +55,38 @@ __device__ __forceinline__ float bf2f(uint16_t h)
     return __uint_as_float((uint32_t)h << 16);
 }
 
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
+__device__ __forceinline__ constexpr int kslot(int g, int i)
 {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    return i < 4 ? 4 * g + i : 16 + 4 * g + (i - 4);
+}
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a), __builtin_bit_cast(bf16v8, b), c, 0,
+                                                   0, 0);
+}
+
+// Eight slots of one row of a 32-wide k block held contiguously (k = 0..31).
+__device__ __forceinline__ bf16x8 load_slots(const uint16_t* row, int g)
+{
+    bf16x8 v;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        v[i] = (short)row[kslot(g, i)];
+    return v;
 }
 
 // Generic GEMM: C[M][N] = alpha * op(A)[M][K] * op(B)[K][N] + beta * C
 //   op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
 //   op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
 // 256 threads compute a 64x64 tile of C; each wave a 32x32 sub-tile
-// (2x2 MFMA tiles).  K advances 16 per step through LDS; out-of-range
+// (2x2 MFMA tiles).  K advances 32 per step through LDS; out-of-range
 // elements are read as zero.  Each operand is staged with consecutive
 // threads on its contiguous dimension.  Split K: block z covers
-// k in [z*kchunk, (z+1)*kchunk) and writes its tile to C + z*cstride (the
-// caller reduces the partial tiles).
+// k in [z*kchunk, (z+1)*kchunk) (kchunk a multiple of 32) and writes its tile
+// to C + z*cstride (the caller reduces the partial tiles).
 constexpr int kGemmTile = 64;
+constexpr int kGemmK = 32;
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restrict__ A, int lda,
@@ -51,8 +94,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
                                                        float* __restrict__ C, int ldc, int M, int N, int K,
                                                        float alpha, float beta, int kchunk, long long cstride)
 {
-    __shared__ uint16_t As[kGemmTile][16 + 4];   // [m][k]
-    __shared__ uint16_t Bs[kGemmTile][16 + 4];   // [n][k]
+    __shared__ uint16_t As[kGemmTile][kGemmK + 8];   // [m][k]
+    __shared__ uint16_t Bs[kGemmTile][kGemmK + 8];   // [n][k]
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -70,14 +113,14 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
         for (int j = 0; j < 2; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    for (int k0 = kbeg; k0 < kend; k0 += 16) {
-        // 64x16 elements of each operand, 4 per thread
+    for (int k0 = kbeg; k0 < kend; k0 += kGemmK) {
+        // 64x32 elements of each operand, 8 per thread
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int idx = tid + 256 * r;   // 0..1023
+        for (int r = 0; r < 8; ++r) {
+            const int idx = tid + 256 * r;   // 0..2047
             {
-                const int mm = TA ? (idx & 63) : (idx >> 4);
-                const int kk = TA ? (idx >> 6) : (idx & 15);
+                const int mm = TA ? (idx & 63) : (idx >> 5);
+                const int kk = TA ? (idx >> 6) : (idx & 31);
                 const int gm = m0 + mm, gk = k0 + kk;
                 uint16_t va = 0;
                 if (gm < M && gk < kend)
@@ -85,8 +128,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
                 As[mm][kk] = va;
             }
             {
-                const int nn = TB ? (idx >> 4) : (idx & 63);
-                const int kk = TB ? (idx & 15) : (idx >> 6);
+                const int nn = TB ? (idx >> 5) : (idx & 63);
+                const int kk = TB ? (idx & 31) : (idx >> 6);
                 const int gn = n0 + nn, gk = k0 + kk;
                 uint16_t vb = 0;
                 if (gn < N && gk < kend)
@@ -95,16 +138,14 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
             }
         }
         __syncthreads();
-        const int kq = 4 * (lane >> 4);
+        const int g = lane >> 4;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const uint16_t* ar = &As[wm + 16 * i + (lane & 15)][kq];
-            const bf16x4 a = bf16x4{(short)ar[0], (short)ar[1], (short)ar[2], (short)ar[3]};
+            const bf16x8 a = load_slots(&As[wm + 16 * i + (lane & 15)][0], g);
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                const uint16_t* br = &Bs[wn + 16 * j + (lane & 15)][kq];
-                const bf16x4 b = bf16x4{(short)br[0], (short)br[1], (short)br[2], (short)br[3]};
-                acc[i][j] = mfma16(a, b, acc[i][j]);
+                const bf16x8 b = load_slots(&Bs[wn + 16 * j + (lane & 15)][0], g);
+                acc[i][j] = mfma32(a, b, acc[i][j]);
             }
         }
         __syncthreads();
@@ -119,7 +160,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
                 const int gn = n0 + wn + 16 * j + (lane & 15);
                 if (gm < M && gn < N) {
                     float* c = C + (size_t)gm * ldc + gn;
-                    *c = alpha * acc[i][j][v] + (beta != 0.0f ? beta * *c : 0.0f);
+                    *c = beta != 0.0f ? alpha * acc[i][j][v] + beta * *c : alpha * acc[i][j][v];
                 }
             }
 }
@@ -136,7 +177,7 @@ __global__ void splitk_reduce_kernel(const float* __restrict__ part, int splits,
     for (int z = 0; z < splits; ++z)
         s += part[z * cstride + idx];
     float* c = C + (size_t)m * ldc + n;
-    *c = alpha * s + (beta != 0.0f ? beta * *c : 0.0f);
+    *c = beta != 0.0f ? alpha * s + beta * *c : alpha * s;
 }
 
 }  // namespace bcc

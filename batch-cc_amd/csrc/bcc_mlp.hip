@@ -86,12 +86,22 @@ __global__ void encode_env_kernel(const float* __restrict__ obs, const int* __re
 }
 
 // ------------------------------------------------------------ features
-// x[p] = [enc(env) | 2 (s - lo) / range - 1 | 2 (e - lo) / range - 1 | 0 ...]
+// x[p] = [enc(env) | (s - lo) * scale - 1 | (e - lo) * scale - 1 | 0 ...], scale = 2 / range
 // as BF16, rows for pairs p0 .. p0 + num_pairs - 1.  Pair p: env =
 // pair_env ? pair_env[p] : p % E, edge = pair_edge ? pair_edge[p] : p / E
 // (dense, edge-major).
+// Joint coordinate idx of an edge ([2][dim]: start then end) normalised to
+// [-1, 1] over the joint's limits: (q - lo) * scale - 1, scale = 2 / range.
+__device__ __forceinline__ float edge_feature(const float* __restrict__ edges, int dim, const float* __restrict__ lo,
+                                              const float* __restrict__ scale, long long edge, int idx)
+{
+    const int j = idx >= dim ? idx - dim : idx;
+    const float q = edges[(size_t)edge * 2 * dim + idx];
+    return (q - lo[j]) * scale[j] - 1.0f;
+}
+
 __global__ void features_kernel(const float* __restrict__ enc, const float* __restrict__ edges, int dim,
-                                const float* __restrict__ lo, const float* __restrict__ range,
+                                const float* __restrict__ lo, const float* __restrict__ scale,
                                 const int* __restrict__ pair_env, const int* __restrict__ pair_edge, int num_envs,
                                 long long p0, long long num_pairs, uint16_t* __restrict__ x)
 {
@@ -107,10 +117,7 @@ __global__ void features_kernel(const float* __restrict__ enc, const float* __re
     if (c < kEnc) {
         v = enc[(size_t)env * kEnc + c];
     } else if (c < kEnc + 2 * dim) {
-        const int j = (c - kEnc) % dim;
-        const int which = (c - kEnc) / dim;
-        const float q = edges[(size_t)edge * 2 * dim + which * dim + j];
-        v = 2.0f * (q - lo[j]) / range[j] - 1.0f;
+        v = edge_feature(edges, dim, lo, scale, edge, c - kEnc);
     }
     x[idx] = f2bf(v);
 }
@@ -212,45 +219,50 @@ __global__ void to_bf16_kernel(const float* __restrict__ a, long long n, uint16_
 // ------------------------------------------------------------ fused inference
 // Inference over a dense batch in one kernel, computed transposed,
 // Y^T = W^T X^T, so that an MFMA result tile (lane l: rows 4 (l/16) + v,
-// column l % 16) is already the next layer's B operand (lane l: k rows
-// 4 (l/16) + v, column l % 16) and activations never leave registers.
-// Columns are pairs, 16 per MFMA tile, 64 per wave.  Weights are read as
-// prepacked A fragments (staged through LDS, see below): fragment (rt, kt) of a layer with W [K][N] holds for
-// lane l the four values W[16 kt + 4 (l/16) + i][16 rt + l % 16], i = 0..3.
-// Every product, its k order and the bias/ReLU/rounding steps are those of
-// the layer-by-layer path (gemm_bf16_kernel + bias_act_kernel), so logits
-// and orders are identical to it.
-constexpr int kKT1 = kIn / 16;                   // k blocks of layer 1
-constexpr int kKT2 = kHid / 16;                  // k blocks of layers 2 and 3
+// column l % 16) of two stacked row tiles is already the next layer's B
+// operand for one 32-wide k block (see kslot in bcc_mlp.h) and activations
+// never leave registers.  Columns are pairs, 16 per MFMA tile, 48 per wave
+// (64 would need more than the 256 registers of two waves per SIMD).
+// Weights are read as prepacked A fragments: fragment (rt, kt) of a layer
+// with W [K][N] holds for lane l the eight values
+// W[32 kt + kslot(l/16, i)][16 rt + l % 16], i = 0..7.  Every product, its
+// slot and k-block order and the bias/ReLU/rounding steps are those of the
+// layer-by-layer path (gemm_bf16_kernel + bias_act_kernel), so logits and
+// orders are identical to it.
+constexpr int kKT1 = kIn / 32;                   // k blocks of layer 1
+constexpr int kKT2 = kHid / 32;                  // k blocks of layers 2 and 3
 constexpr int kRT12 = kHid / 16;                 // row tiles of layers 1 and 2
 constexpr int kRT3 = kOut / 16;                  // row tiles of layer 3
 constexpr int kFragL1 = 0;
 constexpr int kFragL2 = kFragL1 + kRT12 * kKT1;
 constexpr int kFragL3 = kFragL2 + kRT12 * kKT2;
 constexpr int kFrags = kFragL3 + kRT3 * kKT2;
-constexpr int kFuseCols = 4;                     // 16-pair column tiles per wave
+constexpr int kFuseCols = 3;                     // 16-pair column tiles per wave
+constexpr int kFusePairs = 16 * kFuseCols;       // pairs per wave
+constexpr int kFuseBlockPairs = 4 * kFusePairs;  // pairs per workgroup (four waves)
+static_assert(kIn % 32 == 0 && kHid % 32 == 0 && kOut % 16 == 0, "layer widths");
 
 // Weight staging: the workgroup's four waves share each weight fragment,
 // copied once per workgroup into LDS instead of being read from L2 by every
-// wave.  Stage rt holds layer 2's row tile rt (16 contiguous fragments) and
-// layer 3's two fragments at k block rt (layers 2 and 3 hold three quarters
-// of the weights).
-constexpr int kStageFrags = kKT2 + kRT3;                      // 18 fragments
+// wave.  Stage j holds layer 2's row tiles 2 j and 2 j + 1 (2 x 8 contiguous
+// fragments) and layer 3's fragments at k block j, which consumes exactly
+// those two row tiles (layers 2 and 3 hold three quarters of the weights).
+constexpr int kStageFrags = 2 * kKT2 + kRT3;                  // 18 fragments
 constexpr int kStageLoads = (kStageFrags * 64 + 255) / 256;   // per thread
 
-__device__ __forceinline__ void stage_load(const bf16x4* __restrict__ frag, int rt, bf16x4 (&pre)[kStageLoads])
+__device__ __forceinline__ void stage_load(const bf16x8* __restrict__ frag, int j, bf16x8 (&pre)[kStageLoads])
 {
 #pragma unroll
     for (int i = 0; i < kStageLoads; ++i) {
         const int e = threadIdx.x + 256 * i;
-        if (e < kKT2 * 64)
-            pre[i] = frag[(kFragL2 + rt * kKT2) * 64 + e];
+        if (e < 2 * kKT2 * 64)
+            pre[i] = frag[(kFragL2 + 2 * j * kKT2) * 64 + e];
         else if (e < kStageFrags * 64)
-            pre[i] = frag[(kFragL3 + ((e >> 6) - kKT2) * kKT2 + rt) * 64 + (e & 63)];
+            pre[i] = frag[(kFragL3 + ((e >> 6) - 2 * kKT2) * kKT2 + j) * 64 + (e & 63)];
     }
 }
 
-__device__ __forceinline__ void stage_store(bf16x4* buf, const bf16x4 (&pre)[kStageLoads])
+__device__ __forceinline__ void stage_store(bf16x8* buf, const bf16x8 (&pre)[kStageLoads])
 {
 #pragma unroll
     for (int i = 0; i < kStageLoads; ++i) {
@@ -264,11 +276,11 @@ __global__ void pack_fragments_kernel(const uint16_t* __restrict__ W1, const uin
                                       const uint16_t* __restrict__ W3, uint16_t* __restrict__ frag)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;   // (fragment, lane, i)
-    if (idx >= kFrags * 64 * 4)
+    if (idx >= kFrags * 64 * 8)
         return;
-    const int i = idx & 3;
-    const int lane = (idx >> 2) & 63;
-    const int f = idx >> 8;
+    const int i = idx & 7;
+    const int lane = (idx >> 3) & 63;
+    const int f = idx >> 9;
     const uint16_t* W;
     int N, KT, f0;
     if (f < kFragL2) {
@@ -279,39 +291,84 @@ __global__ void pack_fragments_kernel(const uint16_t* __restrict__ W1, const uin
         W = W3, N = kOut, KT = kKT2, f0 = kFragL3;
     }
     const int rt = (f - f0) / KT, kt = (f - f0) % KT;
-    frag[idx] = W[(size_t)(16 * kt + 4 * (lane >> 4) + i) * N + 16 * rt + (lane & 15)];
+    frag[idx] = W[(size_t)(32 * kt + kslot(lane >> 4, i)) * N + 16 * rt + (lane & 15)];
 }
 
-__device__ __forceinline__ float dense_feature(const float* __restrict__ enc, const float* __restrict__ edges,
-                                               int dim, const float* __restrict__ lo,
-                                               const float* __restrict__ range, int env, long long edge, int c)
+// The order word from a pair's 32 bin logits: four distinct bins, one byte
+// each.  The first is the highest logit; each later one the highest logit
+// among the bins at least kOrderSpread bins from every bin already chosen
+// (any unchosen bin when none is that far), ties to the lower bin.  The
+// probe checks the first bins' step-0 samples: neighbouring bins tend to
+// collide together, so spreading the picks along the edge makes a miss of
+// one less likely to repeat in the next.
+#ifndef BCC_ORDER_SPREAD
+#define BCC_ORDER_SPREAD 8
+#endif
+constexpr int kOrderSpread = BCC_ORDER_SPREAD;
+
+// The selection from the layer-3 result registers.  Lane l holds the logits
+// z[4 r3 + v] of bins 16 r3 + 4 g + v (r3 = 0, 1; v = 0..3; g = l / 16) of
+// one pair; the four lanes l % 16 + 16 g' hold its 32 bins.  Each round every
+// lane takes its best candidate, then the lanes 16 and 32 apart exchange
+// theirs; all four lanes end with the same word.
+__device__ __forceinline__ uint32_t spread_top4_lanes(const float (&z)[8], int g)
 {
-    if (c < kEnc)
-        return enc[(size_t)env * kEnc + c];
-    if (c < kEnc + 2 * dim) {
-        const int j = (c - kEnc) % dim;
-        const int which = (c - kEnc) / dim;
-        const float q = edges[(size_t)edge * 2 * dim + which * dim + j];
-        return 2.0f * (q - lo[j]) / range[j] - 1.0f;
+    uint32_t taken = 0, far_from = 0xffffffffu, packed = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t cand = (far_from & ~taken) ? (far_from & ~taken) : ~taken;
+        float bv = 0.0f;
+        int bb = 64;   // none yet
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int bin = 16 * (e >> 2) + 4 * g + (e & 3);
+            if (((cand >> bin) & 1u) && (bb == 64 || z[e] > bv || (z[e] == bv && bin < bb))) {
+                bv = z[e];
+                bb = bin;
+            }
+        }
+#pragma unroll
+        for (int off = 16; off <= 32; off <<= 1) {
+            const float ov = __shfl_xor(bv, off);
+            const int ob = __shfl_xor(bb, off);
+            if (ob != 64 && (bb == 64 || ov > bv || (ov == bv && ob < bb))) {
+                bv = ov;
+                bb = ob;
+            }
+        }
+        taken |= 1u << bb;
+        packed |= (uint32_t)bb << (8 * k);
+        if constexpr (kOrderSpread > 0) {
+            // bins within kOrderSpread - 1 of the pick are no longer "far"
+            const int lo = max(bb - (kOrderSpread - 1), 0);
+            const int hi = min(bb + (kOrderSpread - 1), 31);
+            const uint32_t width = (uint32_t)(hi - lo + 1);
+            far_from &= ~((width >= 32 ? 0xffffffffu : ((1u << width) - 1u)) << lo);
+        }
     }
-    return 0.0f;
+    return packed;
 }
 
-__global__ __launch_bounds__(256) void fused_order_kernel(
-    const bf16x4* __restrict__ frag, const float* __restrict__ b1, const float* __restrict__ b2,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void fused_order_kernel(
+    const bf16x8* __restrict__ frag, const float* __restrict__ b1, const float* __restrict__ b2,
     const float* __restrict__ b3, const float* __restrict__ enc, const float* __restrict__ edges, int dim,
-    const float* __restrict__ lo, const float* __restrict__ range, int num_envs, const int* __restrict__ pair_env,
+    const float* __restrict__ lo, const float* __restrict__ scale, int num_envs, const int* __restrict__ pair_env,
     const int* __restrict__ pair_edge, long long num_pairs, uint32_t* __restrict__ order, float* __restrict__ logits)
 {
-    __shared__ float zs[4][64][kOut + 1];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int col = lane & 15;
-    const int kq = 4 * (lane >> 4);
-    const long long base = ((long long)blockIdx.x * 4 + wave) * 64;
+    const int g = lane >> 4;
+    const int kq = 4 * g;
+    const long long base = ((long long)blockIdx.x * 4 + wave) * kFusePairs;
 
-    // layer-1 B operands: X^T blocks (6 k blocks x 4 column tiles)
-    bf16x4 xb[kKT1][kFuseCols];
+    // layer-1 B operands: X^T blocks (k blocks x column tiles).  Slot i of k
+    // block kt is feature 32 kt + kslot(g, i): k blocks 0 and 1 are the
+    // environment code (slots 0-3 and 4-7 are four consecutive cells each,
+    // one 16-byte load), k block 2 the edge coordinates and zero padding.
+    static_assert(kEnc == 64 && kKT1 == 3, "feature layout");
+    bf16x8 xb[kKT1][kFuseCols];
+    bool pair_ok[kFuseCols];
 #pragma unroll
     for (int ct = 0; ct < kFuseCols; ++ct) {
         const long long p = base + 16 * ct + col;
@@ -322,19 +379,29 @@ __global__ __launch_bounds__(256) void fused_order_kernel(
             env = pair_env ? pair_env[p] : (int)(p % num_envs);
             edge = pair_edge ? (long long)pair_edge[p] : (pair_env ? p : p / num_envs);
         }
+        pair_ok[ct] = ok;
+        const float4* e4 = (const float4*)(enc + (size_t)env * kEnc);
 #pragma unroll
-        for (int kt = 0; kt < kKT1; ++kt)
+        for (int kt = 0; kt < 2; ++kt) {
+            const float4 lo4 = ok ? e4[(32 * kt + kq) >> 2] : float4{0.f, 0.f, 0.f, 0.f};
+            const float4 hi4 = ok ? e4[(32 * kt + 16 + kq) >> 2] : float4{0.f, 0.f, 0.f, 0.f};
+            const float v[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+            xb[kt][ct] = pack8(v);
+        }
+        float v[8];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float v = ok ? dense_feature(enc, edges, dim, lo, range, env, edge, 16 * kt + kq + i) : 0.0f;
-                xb[kt][ct][i] = (short)f2bf(v);
-            }
+        for (int i = 0; i < 8; ++i) {
+            const int idx = kslot(g, i);   // coordinate idx of the edge's [2][dim]
+            v[i] = (ok && idx < 2 * dim) ? edge_feature(edges, dim, lo, scale, edge, idx) : 0.0f;
+        }
+        xb[2][ct] = pack8(v);
     }
 
-    // layer 1: Y1^T (256 x 64) kept as BF16 B operands of layer 2 (its
-    // weights are read directly: staging them too, inside this fully unrolled
-    // loop, costs the kernel its second resident workgroup)
-    bf16x4 y1[kKT2][kFuseCols];
+    // layer 1: Y1^T (256 x 48) kept as BF16 B operands of layer 2: row tiles
+    // 2 j and 2 j + 1 fill slots 0-3 and 4-7 of k block j (its weights are
+    // read directly: staging them too, inside this fully unrolled loop, costs
+    // the kernel its second resident workgroup)
+    u32x4 y1[kKT2][kFuseCols];
 #pragma unroll
     for (int rt = 0; rt < kRT12; ++rt) {
         f32x4 acc[kFuseCols];
@@ -343,105 +410,99 @@ __global__ __launch_bounds__(256) void fused_order_kernel(
             acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < kKT1; ++kt) {
-            const bf16x4 a = frag[(kFragL1 + rt * kKT1 + kt) * 64 + lane];
+            const bf16x8 a = frag[(kFragL1 + rt * kKT1 + kt) * 64 + lane];
 #pragma unroll
             for (int ct = 0; ct < kFuseCols; ++ct)
-                acc[ct] = mfma16(a, xb[kt][ct], acc[ct]);
+                acc[ct] = mfma32(a, xb[kt][ct], acc[ct]);
         }
+        const f32x4 bb = *(const f32x4*)(b1 + 16 * rt + kq);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const float bb = b1[16 * rt + kq + v];
+        for (int ct = 0; ct < kFuseCols; ++ct) {
+            float h[4];
 #pragma unroll
-            for (int ct = 0; ct < kFuseCols; ++ct) {
-                const float z = 1.0f * acc[ct][v] + 0.0f;
-                y1[rt][ct][v] = (short)f2bf(fmaxf(z + bb, 0.0f));
-            }
+            for (int v = 0; v < 4; ++v)
+                h[v] = fmaxf(acc[ct][v] + bb[v], 0.0f);
+            y1[rt >> 1][ct][2 * (rt & 1) + 0] = f2bf2(h[0], h[1]);
+            y1[rt >> 1][ct][2 * (rt & 1) + 1] = f2bf2(h[2], h[3]);
         }
     }
 
-    // layers 2 and 3: each layer-2 row tile is layer 3's next k block; its
-    // weights come through LDS, copied by the whole workgroup (the loads'
-    // latency is covered by the other resident workgroup's MFMAs)
-    __shared__ bf16x4 wbuf[kStageFrags * 64];
+    // layers 2 and 3: stage j computes layer 2's row tiles 2 j and 2 j + 1,
+    // which form layer 3's k block j; the stage's weights come through LDS,
+    // copied by the whole workgroup (the loads' latency is covered by the
+    // other resident workgroup's MFMAs)
+    __shared__ bf16x8 wbuf[kStageFrags * 64];
     f32x4 acc3[kRT3][kFuseCols];
 #pragma unroll
     for (int r3 = 0; r3 < kRT3; ++r3)
 #pragma unroll
         for (int ct = 0; ct < kFuseCols; ++ct)
             acc3[r3][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int rt = 0; rt < kRT12; ++rt) {
+#pragma unroll 1
+    for (int j = 0; j < kKT2; ++j) {
         {
-            bf16x4 pre[kStageLoads];
-            stage_load(frag, rt, pre);
+            bf16x8 pre[kStageLoads];
+            stage_load(frag, j, pre);
             __syncthreads();   // every wave is done with the previous stage
             stage_store(wbuf, pre);
         }
         __syncthreads();
-        f32x4 acc[kFuseCols];
+        u32x4 y2[kFuseCols];
 #pragma unroll
-        for (int ct = 0; ct < kFuseCols; ++ct)
-            acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kt = 0; kt < kKT2; ++kt) {
-            const bf16x4 a = wbuf[kt * 64 + lane];
+        for (int h2 = 0; h2 < 2; ++h2) {
+            f32x4 acc[kFuseCols];
 #pragma unroll
             for (int ct = 0; ct < kFuseCols; ++ct)
-                acc[ct] = mfma16(a, y1[kt][ct], acc[ct]);
-        }
-        bf16x4 y2[kFuseCols];
+                acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const float bb = b2[16 * rt + kq + v];
+            for (int kt = 0; kt < kKT2; ++kt) {
+                const bf16x8 a = wbuf[(h2 * kKT2 + kt) * 64 + lane];
+#pragma unroll
+                for (int ct = 0; ct < kFuseCols; ++ct)
+                    acc[ct] = mfma32(a, __builtin_bit_cast(bf16x8, y1[kt][ct]), acc[ct]);
+            }
+            const f32x4 bb = *(const f32x4*)(b2 + 16 * (2 * j + h2) + kq);
 #pragma unroll
             for (int ct = 0; ct < kFuseCols; ++ct) {
-                const float z = 1.0f * acc[ct][v] + 0.0f;
-                y2[ct][v] = (short)f2bf(fmaxf(z + bb, 0.0f));
+                float h[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    h[v] = fmaxf(acc[ct][v] + bb[v], 0.0f);
+                y2[ct][2 * h2 + 0] = f2bf2(h[0], h[1]);
+                y2[ct][2 * h2 + 1] = f2bf2(h[2], h[3]);
             }
         }
 #pragma unroll
         for (int r3 = 0; r3 < kRT3; ++r3) {
-            const bf16x4 a = wbuf[(kKT2 + r3) * 64 + lane];
+            const bf16x8 a = wbuf[(2 * kKT2 + r3) * 64 + lane];
 #pragma unroll
             for (int ct = 0; ct < kFuseCols; ++ct)
-                acc3[r3][ct] = mfma16(a, y2[ct], acc3[r3][ct]);
+                acc3[r3][ct] = mfma32(a, __builtin_bit_cast(bf16x8, y2[ct]), acc3[r3][ct]);
         }
     }
 
-    // logits to LDS (pair-major), then one lane per pair picks its top 4
+    // logits (bias added as in the layer-by-layer path), then the order word
 #pragma unroll
-    for (int r3 = 0; r3 < kRT3; ++r3)
+    for (int ct = 0; ct < kFuseCols; ++ct) {
+        float z[8];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int bin = 16 * r3 + kq + v;
-            const float bb = b3[bin];
+        for (int r3 = 0; r3 < kRT3; ++r3) {
+            const f32x4 bb = *(const f32x4*)(b3 + 16 * r3 + kq);
 #pragma unroll
-            for (int ct = 0; ct < kFuseCols; ++ct)
-                zs[wave][16 * ct + col][bin] = (1.0f * acc3[r3][ct][v] + 0.0f) + bb;
+            for (int v = 0; v < 4; ++v)
+                z[4 * r3 + v] = acc3[r3][ct][v] + bb[v];
         }
-    __syncthreads();
-    const long long p = base + lane;
-    if (p >= num_pairs)
-        return;
-    const float* z = zs[wave][lane];
-    uint32_t taken = 0, packed = 0;
-    for (int k = 0; k < 4; ++k) {
-        int best = -1;
-        float bv = 0.0f;
-        for (int c = 0; c < kOut; ++c) {
-            if ((taken >> c) & 1u)
-                continue;
-            if (best < 0 || z[c] > bv) {
-                best = c;
-                bv = z[c];
-            }
+        const uint32_t word = spread_top4_lanes(z, g);
+        const long long p = base + 16 * ct + col;
+        if (pair_ok[ct]) {
+            if (g == 0)
+                order[p] = word;
+            if (logits)
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    logits[p * kOut + 16 * (e >> 2) + kq + (e & 3)] = z[e];
         }
-        taken |= 1u << best;
-        packed |= (uint32_t)best << (8 * k);
     }
-    order[p] = packed;
-    if (logits)
-        for (int c = 0; c < kOut; ++c)
-            logits[p * kOut + c] = z[c];
 }
 
 }  // namespace bcc
@@ -493,7 +554,7 @@ hipError_t gemm(bool ta, bool tb, const uint16_t* A, int lda, const uint16_t* B,
         splits = std::min<long long>(std::min(32, K / 256), kSplitWs / ((long long)M * N));
         splits = std::max(splits, 1);
     }
-    const int kchunk = splits > 1 ? ((K + splits - 1) / splits + 15) / 16 * 16 : std::max(K, 1);
+    const int kchunk = splits > 1 ? ((K + splits - 1) / splits + kGemmK - 1) / kGemmK * kGemmK : std::max(K, 1);
     splits = (K + kchunk - 1) / kchunk;
     float* out = splits > 1 ? ws : C;
     const int ldo = splits > 1 ? N : ldc;
@@ -557,7 +618,7 @@ struct bcc_model {
     float* m = nullptr;       // Adam first moment
     float* v = nullptr;       // Adam second moment
     float* lo = nullptr;      // joint lower limits [dim]
-    float* range = nullptr;   // joint ranges [dim]
+    float* scale = nullptr;   // 2 / joint range [dim]
     uint16_t* frag = nullptr; // MFMA A fragments of the BF16 weights (fused inference)
     float* enc = nullptr;     // environment codes of the last scene ordered [enc_envs][64]
     int enc_envs = 0;
@@ -676,7 +737,7 @@ void destroy_model(bcc_model* M)
         return;
     free_scratch(M);
     for (void* p : {(void*)M->w, (void*)M->wb, (void*)M->g, (void*)M->m, (void*)M->v, (void*)M->lo,
-                    (void*)M->range, (void*)M->loss, (void*)M->frag, (void*)M->ws, (void*)M->enc})
+                    (void*)M->scale, (void*)M->loss, (void*)M->frag, (void*)M->ws, (void*)M->enc})
         (void)hipFree(p);
     delete M;
 }
@@ -695,14 +756,17 @@ int alloc_model(int robot, bcc_model** out)
     if (e == hipSuccess) e = hipMalloc((void**)&M->m, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMalloc((void**)&M->v, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMalloc((void**)&M->lo, sizeof(float) * 16);
-    if (e == hipSuccess) e = hipMalloc((void**)&M->range, sizeof(float) * 16);
+    if (e == hipSuccess) e = hipMalloc((void**)&M->scale, sizeof(float) * 16);
     if (e == hipSuccess) e = hipMalloc((void**)&M->loss, sizeof(float));
-    if (e == hipSuccess) e = hipMalloc((void**)&M->frag, sizeof(uint16_t) * kFrags * 64 * 4);
+    if (e == hipSuccess) e = hipMalloc((void**)&M->frag, sizeof(uint16_t) * kFrags * 64 * 8);
     if (e == hipSuccess) e = hipMalloc((void**)&M->ws, sizeof(float) * kSplitWs);
     if (e == hipSuccess) e = hipMemset(M->m, 0, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMemset(M->v, 0, sizeof(float) * P_N);
     if (e == hipSuccess) e = hipMemcpy(M->lo, kLo[robot], sizeof(float) * 16, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(M->range, kRange[robot], sizeof(float) * 16, hipMemcpyHostToDevice);
+    float scale[16] = {};
+    for (int j = 0; j < dim; ++j)
+        scale[j] = (float)(2.0 / (double)kRange[robot][j]);
+    if (e == hipSuccess) e = hipMemcpy(M->scale, scale, sizeof(float) * 16, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         destroy_model(M);
         return hip_fail(e, "model allocation");
@@ -857,7 +921,7 @@ int bcc_model_features(const bcc_model* M, const bcc_scene* scene, const float* 
     hipLaunchKernelGGL(encode_env_kernel, dim3(std::max(scene->num_envs, 1)), dim3(64), 0, s, scene->d_ref_obs,
                        scene->d_ref_meta, scene->num_envs, enc);
     hipLaunchKernelGGL(features_kernel, dim3(blocks_for(num_pairs * kIn)), dim3(256), 0, s, enc, d_edges, M->dim,
-                       M->lo, M->range, d_pair_env, d_pair_edge, scene->num_envs, 0LL, (long long)num_pairs, d_x);
+                       M->lo, M->scale, d_pair_env, d_pair_edge, scene->num_envs, 0LL, (long long)num_pairs, d_x);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipStreamSynchronize(s);
@@ -988,13 +1052,13 @@ int order_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges, long
         BCC_HIP(hipMalloc((void**)&M->enc, sizeof(float) * kEnc * scene->num_envs), "order scratch");
         M->enc_envs = scene->num_envs;
     }
-    hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks_for((long long)kFrags * 64 * 4)), dim3(256), 0, s,
+    hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks_for((long long)kFrags * 64 * 8)), dim3(256), 0, s,
                        M->wb + P_W1, M->wb + P_W2, M->wb + P_W3, M->frag);
     hipLaunchKernelGGL(encode_env_kernel, dim3(scene->num_envs), dim3(64), 0, s, scene->d_ref_obs, scene->d_ref_meta,
                        scene->num_envs, M->enc);
-    hipLaunchKernelGGL(fused_order_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s,
-                       (const bf16x4*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, M->enc, d_edges, M->dim, M->lo,
-                       M->range, scene->num_envs, pair_env, pair_edge, pairs, d_order, d_logits);
+    hipLaunchKernelGGL(fused_order_kernel, dim3((unsigned)((pairs + kFuseBlockPairs - 1) / kFuseBlockPairs)), dim3(256), 0, s,
+                       (const bf16x8*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, M->enc, d_edges, M->dim, M->lo,
+                       M->scale, scene->num_envs, pair_env, pair_edge, pairs, d_order, d_logits);
     if (hipError_t e = hipGetLastError(); e != hipSuccess)
         return hip_fail(e, "fused order");
     return BCC_OK;

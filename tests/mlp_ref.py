@@ -75,3 +75,19 @@ def loss_only(params, x, labels):
     H2 = np.maximum(H1 @ P["W2"] + P["b2"], 0)
     Z3 = H2 @ P["W3"] + P["b3"]
     return np.mean(np.sum(np.maximum(Z3, 0) - Z3 * Y + np.log1p(np.exp(-np.abs(Z3))), axis=1))
+
+
+def spread_top4(z, spread=8):
+    """Order word of one pair's 32 logits: the highest logit first, then each
+    time the highest among the bins at least `spread` bins from every bin
+    already chosen (any unchosen bin when none is), ties to the lower bin."""
+    taken = []
+    for _ in range(4):
+        far = [c for c in range(32) if c not in taken and all(abs(c - t) >= spread for t in taken)]
+        cand = far if far else [c for c in range(32) if c not in taken]
+        best = cand[0]
+        for c in cand[1:]:
+            if z[c] > z[best]:
+                best = c
+        taken.append(best)
+    return taken[0] | taken[1] << 8 | taken[2] << 16 | taken[3] << 24

@@ -72,12 +72,13 @@ def test_encoder_and_features_match_numpy(bcc, gpu):
         pg = rng.integers(0, edges.shape[0], 300).astype(np.int32)
         x = model.features(scene, de, pe, pg).to_array(np.uint16, (300, 96))
     lo, rg = W.LIMITS["panda"]
+    scale = np.float32(2.0 / rg.astype(np.float64))   # 2 / range, rounded once
     for i in range(300):
         q = edges[pg[i]]
         f = np.zeros(96, np.float32)
         f[:64] = enc[pe[i]]
-        f[64:71] = np.float32(2.0) * (q[0] - lo) / rg - np.float32(1.0)
-        f[71:78] = np.float32(2.0) * (q[1] - lo) / rg - np.float32(1.0)
+        f[64:71] = (q[0] - lo) * scale - np.float32(1.0)
+        f[71:78] = (q[1] - lo) * scale - np.float32(1.0)
         assert np.array_equal(x[i], mlp_ref.bf16_bits(f)), i
 
 
@@ -246,9 +247,10 @@ def test_trained_model_gain_over_untrained_and_plain(bcc, gpu):
 
 @pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
 def test_fused_inference_matches_layer_by_layer(bcc, gpu, robot):
-    """The fused kernel (features, three MFMA layers, top-4 in one launch)
-    gives the same logits, bit for bit, as the layer-by-layer GEMM path on the
-    same rows, and the top-4 of those logits with ties to the lower bin."""
+    """The fused kernel (features, three MFMA layers, order selection in one
+    launch) gives the same logits, bit for bit, as the layer-by-layer GEMM path
+    on the same rows, and the order words of the spread top-4 rule
+    (mlp_ref.spread_top4) on those logits."""
     from batch_cc_amd import workloads as W
     rng = np.random.default_rng(12)
     envs = [W.c0_env(rng, n_cub=3, n_sph=2, n_cyl=2) for _ in range(7)]
@@ -272,7 +274,6 @@ def test_fused_inference_matches_layer_by_layer(bcc, gpu, robot):
         ref = model.logits(x, M * E).reshape(M, E, 32)
         order2 = model.order_dense(scene, de, M).to_array(np.uint32, (M, E))
     assert np.array_equal(logits.view(np.uint32), ref.view(np.uint32))
-    top = np.argsort(-ref, axis=2, kind="stable")[:, :, :4].astype(np.uint32)
-    exp = top[..., 0] | top[..., 1] << 8 | top[..., 2] << 16 | top[..., 3] << 24
+    exp = np.array([[mlp_ref.spread_top4(ref[m, e]) for e in range(E)] for m in range(M)], np.uint32)
     assert np.array_equal(order, exp)
     assert np.array_equal(order2, order)
