@@ -70,6 +70,7 @@ def lib():
     _sig(L, "bcc_work_dense", I, [I, P, P, I, I, P, P, P, P])
     _sig(L, "bcc_work_pairs", I, [I, P, P, I, P, I, P, P, P, P])
     _sig(L, "bcc_check_ragged", I, [I, P, P, P, I, P])
+    _sig(L, "bcc_check_ragged_noalloc", I, [I, P, P, P, ctypes.c_int64, I, P, P])
     _sig(L, "bcc_ragged_plan_create", I, [I, P, P, P, I, P, I, ctypes.POINTER(P)])
     _sig(L, "bcc_ragged_plan_launch", I, [P, P])
     _sig(L, "bcc_ragged_plan_launch_eager", I, [P, P])
@@ -344,6 +345,21 @@ def check_ragged(robot, scene: Scene, env_off, edges, resolution: int = 32) -> n
     out = np.zeros(e.shape[0], np.uint8)
     _check(lib().bcc_check_ragged(robot_id(robot), scene.handle, _ptr(off), _ptr(e), int(resolution), _ptr(out)))
     return out
+
+
+def check_ragged_noalloc(robot, scene: Scene, d_env_off: DeviceBuffer, d_edges: DeviceBuffer, num_pairs: int,
+                         d_out: DeviceBuffer, resolution: int = 32, stream=None):
+    """Allocation-free ragged check on device buffers (asynchronous): the
+    tile list is cut on the device into the library's static scratch array.
+    d_env_off: int64 [num_envs + 1] CSR offsets on the device."""
+    n = int(num_pairs)
+    dim = lib().bcc_robot_dimension(robot_id(robot))
+    if d_env_off.nbytes < 8 * (scene.num_envs + 1):
+        raise BCCError(f"d_env_off holds {d_env_off.nbytes} bytes, {scene.num_envs} envs need {8 * (scene.num_envs + 1)}")
+    if d_edges.nbytes < n * 2 * dim * 4 or d_out.nbytes < n:
+        raise BCCError(f"d_edges/d_out are too small for {n} pairs")
+    _check(lib().bcc_check_ragged_noalloc(robot_id(robot), scene.handle, d_env_off.ptr, d_edges.ptr, n,
+                                          int(resolution), d_out.ptr, stream))
 
 
 class RaggedPlan:

@@ -1385,6 +1385,167 @@ int bcc_check_ragged(int robot, const bcc_scene* scene, const int64_t* env_offse
     return rc;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------------ allocation-free ragged check
+// bcc_check_ragged_noalloc runs the ragged plan's persistent kernel over a CSR
+// batch whose offsets are already on the device.  Its only memory beyond the
+// caller's inputs and outputs is `scratch`: a header of kNoAllocHeaderInts
+// ints (the persistent kernel's tile counter at [0]) followed by at most
+// kNoAllocMaxTiles tiles, which a one-workgroup kernel cuts from the offsets
+// on the device.  Each device holds its own copy for the life of the
+// process; every call on a device reuses it, so calls that overlap on one
+// device race (issue them on one stream).
+
+constexpr int kNoAllocHeaderInts = 64;   // 256 bytes
+constexpr int kNoAllocMaxTiles = 1 << 18;
+constexpr int kNoAllocTileInts = (int)(sizeof(bcc::RaggedTile) / sizeof(int));
+static_assert(sizeof(bcc::RaggedTile) % sizeof(int) == 0 && alignof(bcc::RaggedTile) <= 16, "tile layout");
+
+static __device__ __align__(256) int scratch[kNoAllocHeaderInts + kNoAllocMaxTiles * kNoAllocTileInts];
+
+namespace {
+
+constexpr int kNoAllocBlock = 1024;
+constexpr int kNoAllocMaxDevices = 64;
+
+// Device address of `scratch` per device, taken once with that device
+// current (each device's copy lives at its own address).
+std::mutex g_scratch_mu;
+int* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+int scratch_on_current_device(int** out)
+{
+    int dev = 0;
+    BCC_HIP(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(BCC_ERR_INVALID, "device ordinal " + std::to_string(dev) + " is beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        BCC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<int*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return BCC_OK;
+}
+
+// Cuts a CSR batch into tiles of at most `tile` pairs: one workgroup walks
+// the environments kNoAllocBlock at a time; each thread counts its
+// environment's tiles, a workgroup scan places them after those of the
+// environments before it, and the thread writes them.  Offsets are clamped
+// to [0, total] and writes to the first `cap` slots, so any offsets array
+// keeps every access in bounds; slots [count, cap) get empty tiles, which the
+// persistent kernel takes and skips.  Also zeroes the tile counter.
+__global__ __launch_bounds__(kNoAllocBlock) void noalloc_tiles_kernel(const int64_t* __restrict__ off, int num_envs,
+                                                                      int64_t total, int tile, int cap,
+                                                                      bcc::RaggedTile* __restrict__ tiles,
+                                                                      int* __restrict__ counter)
+{
+    constexpr int kWaves = kNoAllocBlock / 64;
+    __shared__ int wave_total[kWaves];
+    __shared__ int placed;   // tiles of the environments walked so far (capped at cap)
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    const int wave = t >> 6;
+    if (t == 0) {
+        placed = 0;
+        *counter = 0;
+    }
+    __syncthreads();
+    for (int e0 = 0; e0 < num_envs; e0 += kNoAllocBlock) {
+        const int e = e0 + t;
+        int64_t begin = 0, end = 0;
+        if (e < num_envs) {
+            begin = min(max(off[e], (int64_t)0), total);
+            end = min(max(off[e + 1], begin), total);
+        }
+        const int count = (int)min((end - begin + tile - 1) / tile, (int64_t)cap);
+        int incl = count;   // inclusive scan over the wave
+        for (int d = 1; d < 64; d <<= 1) {
+            const int y = __shfl_up(incl, d);
+            if (lane >= d)
+                incl += y;
+        }
+        if (lane == 63)
+            wave_total[wave] = incl;
+        __syncthreads();
+        int before = placed;
+        for (int w = 0; w < wave; ++w)
+            before += wave_total[w];
+        before += incl - count;
+        for (int j = 0; j < count; ++j) {
+            const int slot = before + j;
+            if (slot >= cap)
+                break;
+            const int64_t b = begin + (int64_t)j * tile;
+            tiles[slot] = bcc::RaggedTile{e, (int)b, (int)min(b + tile, end), 0};
+        }
+        __syncthreads();
+        if (t == 0) {
+            int sum = placed;
+            for (int w = 0; w < kWaves; ++w)
+                sum = min(sum + wave_total[w], cap);
+            placed = sum;
+        }
+        __syncthreads();
+    }
+    for (int slot = placed + t; slot < cap; slot += kNoAllocBlock)
+        tiles[slot] = bcc::RaggedTile{0, 0, 0, 0};
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcc_check_ragged_noalloc(int robot, const bcc_scene* scene, const int64_t* d_env_offsets, const float* d_edges,
+                             int64_t num_pairs, int resolution, uint8_t* d_results, void* stream)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, 0, resolution, d_results))
+        return rc;
+    if (num_pairs < 0 || num_pairs > 0x7fffffffLL)
+        return fail(BCC_ERR_INVALID, "num_pairs must be in [0, 2^31 - 1]");
+    if (num_pairs == 0)
+        return BCC_OK;
+    const int E = scene->num_envs;
+    if (E < 1)
+        return fail(BCC_ERR_INVALID, "pairs given for a scene without environments");
+    if (!d_env_offsets || !d_edges || !d_results)
+        return fail(BCC_ERR_INVALID, "d_env_offsets/d_edges/d_results is NULL");
+    // tile and grid as a ragged plan picks them; the pair weights of the
+    // obstacle mean are on the device, so the variant uses the scene's mean
+    const int variant = choose_variant(robot, mean_obstacles(scene));
+    const bcc_tune_entry& tune = tuned(robot);
+    const bcc::VariantKnobs& k = knobs(robot);
+    const int resident = std::min(tune.groups_per_cu, variant ? k.hi_waves : k.lo_waves);
+    const int cus = device_cus();
+    int64_t tile = tile_for(num_pairs, cus * resident * (bcc::kBlock / 64), tune.tiles_per_wave);
+    // at most E - 1 partial tiles beyond ceil(pairs / tile): grow the tile
+    // until the list fits the scratch array
+    auto bound = [&](int64_t t) { return (num_pairs + t - 1) / t + E; };
+    while (bound(tile) > kNoAllocMaxTiles && tile < num_pairs)
+        tile *= 2;
+    tile = std::min(tile, num_pairs);   // one tile per environment is the coarsest list
+    if (bound(tile) > kNoAllocMaxTiles)
+        return fail(BCC_ERR_INVALID, "an allocation-free ragged batch holds at most " +
+                                         std::to_string(kNoAllocMaxTiles - 1) + " environments, got " +
+                                         std::to_string(E));
+    const int cap = (int)bound(tile);
+    int* base = nullptr;
+    if (int rc = scratch_on_current_device(&base))
+        return rc;
+    int* counter = base;
+    bcc::RaggedTile* tiles = reinterpret_cast<bcc::RaggedTile*>(base + kNoAllocHeaderInts);
+    const hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(noalloc_tiles_kernel, dim3(1), dim3(kNoAllocBlock), 0, s, d_env_offsets, E, num_pairs,
+                       (int)tile, cap, tiles, counter);
+    BCC_HIP(hipGetLastError(), "tile kernel launch");
+    bcc::RaggedArgs a{scene->d_obs, scene->d_meta, d_edges, tiles, cap, counter, resolution,
+                      cus * tune.groups_per_cu, d_results};
+    a.variant = variant;
+    return dispatch_ragged(robot, a, s);
+}
+
 int bcc_debug_dense_chunk(int edges)
 {
     if (edges < 0 || edges > 4096)

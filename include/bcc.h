@@ -147,6 +147,23 @@ int bcc_check_ragged(int robot, const bcc_scene *scene, const int64_t *env_offse
                      int resolution, uint8_t *results);
 
 /*
+ * Allocation-free ragged check.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops).  Same verdicts as a ragged plan over the same CSR
+ * batch; everything is device-resident and asynchronous on `stream`:
+ * d_env_offsets int64 [num_envs + 1] (offsets[0] = 0, non-decreasing,
+ * offsets[num_envs] = num_pairs), d_edges [num_pairs][2][dim], d_results
+ * [num_pairs].  The tile list lives in a static per-device scratch array of
+ * 4 MiB + 256 bytes, kept until the process ends, so a batch holds at most
+ * 2^18 - 1 environments and 2^31 - 1 pairs (larger: BCC_ERR_INVALID).  Calls
+ * that overlap on one device race on that array: issue them on one stream.
+ * Offsets that break the rules above give unspecified verdicts for the
+ * affected pairs, never an out-of-bounds access.
+ */
+int bcc_check_ragged_noalloc(int robot, const bcc_scene *scene, const int64_t *d_env_offsets, const float *d_edges,
+                             int64_t num_pairs, int resolution, uint8_t *d_results, void *stream);
+
+/*
  * A ragged plan owns the tile list (env, pair range) of one CSR layout and a
  * captured hipGraph (tile-counter reset + one persistent kernel launch) over
  * caller-owned device buffers d_edges / d_results.  tile_pairs = 0 picks the
