@@ -117,6 +117,7 @@ def lib():
     _sig(L, "bcc_debug_samples", I, [I, P, I, I, P, P, P])
     _sig(L, "bcc_debug_prims", I, [P, I, P])
     _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
+    _sig(L, "bcc_debug_valu", I, [I, I, I, P])
     _sig(L, "bcc_debug_force_variant", I, [I])
     _sig(L, "bcc_debug_cull_min", I, [I])
     _sig(L, "bcc_debug_scene_cull_min", I, [P, ctypes.POINTER(I)])
@@ -530,6 +531,13 @@ def debug_half_boxes(inp: np.ndarray, mask: int) -> np.ndarray:
 
 def debug_copy(d_src: DeviceBuffer, d_dst: DeviceBuffer, nbytes: int):
     _check(lib().bcc_debug_copy(d_src.ptr, d_dst.ptr, int(nbytes)))
+
+
+def debug_valu(blocks: int, iters: int, active_lanes: int, d_out: DeviceBuffer):
+    """Known FP32 work: per wave, lanes below active_lanes run 8 x iters FMAs."""
+    if d_out.nbytes < 4 * 256 * int(blocks):
+        raise BCCError("d_out must hold blocks * 256 floats")
+    _check(lib().bcc_debug_valu(int(blocks), int(iters), int(active_lanes), d_out.ptr))
 
 
 # ------------------------------------------------------------ learned ordering

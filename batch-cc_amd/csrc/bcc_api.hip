@@ -501,6 +501,34 @@ __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ d
         dst[i] = src[i];
 }
 
+// Known FP32 VALU work for calibrating the instruction and FLOP counters:
+// lanes below `active` of every wave run kValuChains independent chains of
+// `iters` fmaf each (then kValuChains - 1 adds and one store); the other
+// lanes only store.  Lane values come from the thread index and the kernel
+// arguments, so nothing folds.
+constexpr int kValuChains = 8;
+
+__global__ __launch_bounds__(256) void valu_kernel(float* __restrict__ out, int iters, int active, float m, float c)
+{
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    float r = 0.0f;
+    if ((threadIdx.x & 63) < active) {
+        float x[kValuChains];
+#pragma unroll
+        for (int k = 0; k < kValuChains; ++k)
+            x[k] = (float)(tid + k) * 1e-7f;
+        for (int i = 0; i < iters; ++i)
+#pragma unroll
+            for (int k = 0; k < kValuChains; ++k)
+                x[k] = fmaf(x[k], m, c);
+        r = x[0];
+#pragma unroll
+        for (int k = 1; k < kValuChains; ++k)
+            r += x[k];
+    }
+    out[tid] = r;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1671,6 +1699,16 @@ int bcc_debug_copy(const void* d_src, void* d_dst, size_t bytes)
     hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, 0, (const uint4*)d_src, (uint4*)d_dst, n);
     BCC_HIP(hipGetLastError(), "copy kernel");
     BCC_HIP(hipDeviceSynchronize(), "copy kernel");
+    return BCC_OK;
+}
+
+int bcc_debug_valu(int blocks, int iters, int active_lanes, float* d_out)
+{
+    if (blocks < 1 || iters < 0 || active_lanes < 0 || active_lanes > 64 || !d_out)
+        return fail(BCC_ERR_INVALID, "bad VALU calibration arguments");
+    hipLaunchKernelGGL(valu_kernel, dim3(blocks), dim3(256), 0, 0, d_out, iters, active_lanes, 0.999f, 1e-3f);
+    BCC_HIP(hipGetLastError(), "VALU calibration kernel");
+    BCC_HIP(hipDeviceSynchronize(), "VALU calibration kernel");
     return BCC_OK;
 }
 

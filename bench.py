@@ -22,11 +22,18 @@ Beside the headline: the same ordered kernel with evenly spaced probe bins
 of the benchmark pairs themselves (the bound on any order), and how many
 colliding pairs each probe decides.
 
-The roofline's algorithmic work is the reference's own source-level FP32
-operation count for the same pairs in plain order (bcc_work_dense), divided
-by the dominant kernel's time.  The CPU baseline is the project's C
-restatement of the reference checker (the reference's CPU path, vamp, is not
-available), timed on the host cores on a bounded sample.  Prints one JSON line.
+Roofline of the step's dominant kernel: before it touches the GPU, the run
+re-runs its own workload (one step of each kernel) three times in child
+processes under `rocprofv3 --pmc` (SQ counters, FETCH_SIZE, WRITE_SIZE);
+`roofline.achieved` is the executed FP32 rate (SQ_INSTS_VALU_FLOPS_FP32 in
+units measured on a known-FMA kernel, over the kernel's hipEvent time here)
+and `traffic` the HBM bytes per launch (scaled on a known-size copy).  The
+reference's own source-level FP32 operation count for the same pairs in plain
+order (bcc_work_dense) over the same time is `effective_achieved`.  Without
+rocprofv3 (or with --no-counters) the effective figure stands in.  The CPU
+baseline is the project's C restatement of the reference checker (the
+reference's CPU path, vamp, is not available), timed on the host cores on a
+bounded sample.  Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -344,11 +351,115 @@ def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
     }
 
 
+
+def make_workload(W, a):
+    """(robot, scene, edges, env_off or None, E, M, pairs, verdict shape,
+    description) of a dense or ragged config."""
+    if a.config == "c2":
+        robot, sc, env_off, edges = W.make_ragged(a.config, seed=a.seed, scale=a.scale)
+        E, M = sc.num_envs, edges.shape[0]
+        return (robot, sc, edges, env_off, E, M, M, (M,),
+                f"{robot}, {E} envs, {M} ragged pairs (Zipf(1.2) edges per env)")
+    if a.config == "mbm":
+        from batch_cc_amd import mbm
+        robot = "panda"
+        problems = os.path.join(REPO, "tests", "golden", "mbm_panda.json")
+        sc, edges = mbm.readme_run(problems, robot, max(1, int(100 * a.scale)), max(1, int(1000 * a.scale)))
+        E, M = sc.num_envs, edges.shape[0]
+        return (robot, sc, edges, None, E, M, E * M, (M, E),
+                f"{robot}, {E} MotionBenchMaker bookshelf_small envs x {M} Halton edges")
+    robot, sc, edges = W.make_config(a.config, seed=a.seed, scale=a.scale)
+    E, M = sc.num_envs, edges.shape[0]
+    return robot, sc, edges, None, E, M, E * M, (M, E), f"{robot}, {E} envs x {M} edges"
+
+
+# ------------------------------------------------------------ counters
+
+def child_args(a) -> list:
+    """Arguments of a counter child: the same workload, one step of each kernel."""
+    args = ["--config", a.config, "--seed", str(a.seed), "--scale", str(a.scale), "--train-pairs",
+            str(a.train_pairs), "--train-epochs", str(a.train_epochs), "--counter-child"]
+    if not a.learned:
+        args.append("--no-learned")
+    return args
+
+
+def counter_child(bcc, W, C, a):
+    """Runs under rocprofv3 --pmc: the known-work kernels, then one launch of
+    the workload's plain kernel and, for learned configs, the model trained as
+    in the measured run and one learned step (inference + ordered check)."""
+    C.run_known_work(bcc)
+    if a.config == "c4":
+        a.steps, a.warmup, a.no_work, a.no_cpu_baseline = 1, 0, True, True
+        bench_mixed(bcc, W, a, None, {})
+        return
+    robot, sc, edges, env_off, E, M, pairs, shape, desc = make_workload(W, a)
+    scene = bcc.Scene(sc)
+    d_edges = bcc.DeviceBuffer.from_array(edges)
+    d_out = bcc.DeviceBuffer(pairs)
+    if env_off is not None:
+        plan = bcc.RaggedPlan(robot, scene, env_off, d_edges, d_out)
+        plan.launch()
+        bcc.synchronize()
+        plan.close()
+    else:
+        bcc.check_dense_device(robot, scene, d_edges, M, 32, d_out)
+        bcc.synchronize()
+        if a.learned:
+            edge_law = W.lognormal_edges if robot == "fetch" else W.uniform_edges
+            model, _ = train_model(bcc, W, a, robot, scene, E, max(1, min(M, 100_000)), edge_law)
+            d_order = bcc.DeviceBuffer(E * M * 4)
+            model.time_learned_dense(scene, d_edges, M, d_order, d_out, 0, 1)
+            bcc.synchronize()
+            model.close()
+    scene.close()
+
+
+def build_roofline(C, hw, cal, match, kernel: str, kms: float, ref_flops=None, ref_trigs=None) -> dict:
+    """Roofline of the step's dominant kernel(s): executed FP32 rate from the
+    counters (unit from the known-work kernel) when they were collected, HBM
+    traffic from FETCH_SIZE + WRITE_SIZE (scaled on the known-size copy), and
+    the reference-order work rate beside it as the effective figure."""
+    eff = ref_flops / (kms / 1e3) / 1e12 if ref_flops else None
+    c = C.kernel_counters(hw, cal, match, kms, FP32_PEAK_TFLOPS, HBM_PEAK_GBS) if hw else {}
+    rf = {"bound": "valu", "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "kernel": kernel, "kernel_ms": kms}
+    if "executed_tflops" in c:
+        rf["achieved"] = c["executed_tflops"]
+        rf["frac"] = c["executed_frac"]
+        rf["source"] = ("executed FP32 operations (SQ_INSTS_VALU_FLOPS_FP32 of the kernel's one-step rocprofv3 "
+                        "pass x the unit measured on a known-FMA kernel) / the kernel's hipEvent time")
+    else:
+        rf["achieved"] = eff
+        rf["frac"] = eff / FP32_PEAK_TFLOPS if eff is not None else None
+        rf["source"] = "reference-order work / hipEvent time (no counters collected)"
+    rf["traffic"] = c.get("traffic_bytes_per_launch")
+    rf["effective_achieved"] = eff
+    rf["effective_frac"] = eff / FP32_PEAK_TFLOPS if eff is not None else None
+    rf["algorithmic_flops_per_launch"] = ref_flops
+    rf["sincos_per_launch"] = ref_trigs
+    if c:
+        rf["counters"] = {k: v for k, v in c.items() if k not in ("executed_tflops", "executed_frac")}
+        rf["calibration"] = cal
+    rf["note"] = ("effective = the reference's source-level FP32 ops for these pairs in plain order (counted on "
+                  "the GPU by the reference-order replay) / the kernel time; the kernel executes less than that "
+                  "(early exits, culling, probing), so effective_frac can exceed 1")
+    return rf
+
+
+def is_kernel(prefix: str):
+    return lambda name: name.startswith(prefix)
+
+
+def bcc_counters():
+    from batch_cc_amd import counters
+    return counters
+
 # ------------------------------------------------------------ workloads
 
-def bench_mixed(bcc, W, a) -> dict:
+def bench_mixed(bcc, W, a, hw, cal) -> dict:
     """c4: panda, fetch and baxter environments interleaved; one ragged plan
-    per robot, all replayed as one hipGraph (bcc_ragged_group)."""
+    per robot, all replayed as one hipGraph (bcc_ragged_group).  hw / cal:
+    the counter passes and their calibration (or None / {})."""
     t0 = time.perf_counter()
     parts, env_robot, global_off = W.make_mixed(seed=a.seed, scale=a.scale)
     total = int(global_off[-1])
@@ -380,7 +491,7 @@ def bench_mixed(bcc, W, a) -> dict:
     log(f"{a.steps} timed group launches: {ms:.3f} ms each; plans alone: "
         + ", ".join(f"{k} {v:.3f} ms" for k, v in alone.items()))
     learned = learned_mixed(bcc, W, a, runs, ms) if a.learned else None
-    roofline = None
+    flops = trigs = None
     if not a.no_work:
         flops = trigs = 0.0
         for r in runs:
@@ -391,13 +502,28 @@ def bench_mixed(bcc, W, a) -> dict:
                 sys.exit(1)
             flops += float(work.sum(dtype=np.float64))
             trigs += float(trig.sum(dtype=np.float64))
-        achieved = flops / (ms / 1e3) / 1e12
-        roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                    "kernel": "ragged_kernel x3 robots (one hipGraph)",
-                    "algorithmic_flops_per_launch": flops, "sincos_per_launch": trigs,
-                    "note": "reference source-level FP32 ops in plain order, counted on the GPU by the "
-                            "reference-order replay; sin/cos counted separately"}
+    # the learned step's check (learn_finish_kernel per robot) dominates it;
+    # the plain group's ragged_kernel per robot is reported beside it
+    plain_rf = build_roofline(bcc_counters(), hw, cal, is_kernel("bcc::ragged_kernel<"),
+                              "ragged_kernel x3 robots (one hipGraph, plain order)", ms, flops, trigs)
+    if learned:
+        roofline = build_roofline(bcc_counters(), hw, cal, is_kernel("bcc::learn_finish_kernel<"),
+                                  "learn_finish_kernel x3 robots (one hipGraph, the learned step's check)",
+                                  learned["ordered_check_ms"], flops, trigs)
+        plain_rf.pop("calibration", None)
+        roofline["plain_kernel"] = plain_rf
+        if hw and "sq" in hw:
+            mops = bcc_counters().per_step(hw["sq"], is_kernel("bcc::fused_order_kernel"),
+                                           "SQ_INSTS_VALU_MFMA_MOPS_BF16")
+            if mops:
+                im = learned["inference_mfma"]
+                im["executed_flops_per_launch"] = 512.0 * mops * len(runs)
+                im["executed_tflops"] = im["executed_flops_per_launch"] / (learned["inference_ms"] / 1e3) / 1e12
+                im["executed_frac"] = im["executed_tflops"] / BF16_PEAK_TFLOPS
+    else:
+        roofline = plain_rf
+    if roofline["achieved"] is None:
+        roofline = None
     cpu = None
     if not a.no_cpu_baseline:
         n_done, t_done, c = 0, 0.0, None
@@ -455,6 +581,9 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration) and exit")
+    ap.add_argument("--no-counters", dest="counters", action="store_false",
+                    help="skip the rocprofv3 counter passes (roofline from reference-order work only)")
+    ap.add_argument("--counter-child", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args()
 
     if a.copy_calibration:
@@ -474,9 +603,22 @@ def main():
 
     bcc = load_pkg()
     from batch_cc_amd import workloads as W
+    from batch_cc_amd import counters as C
+
+    if a.counter_child:
+        counter_child(bcc, W, C, a)
+        return
+
+    # counter passes first, while this process has not touched the GPU
+    hw, cal = None, {}
+    if a.counters:
+        t0 = time.perf_counter()
+        hw = C.collect(os.path.abspath(__file__), child_args(a), log)
+        cal = C.calibration(hw) if hw else {}
+        log(f"counters: {len(hw or {})} passes in {time.perf_counter() - t0:.1f} s; calibration {cal}")
 
     if a.config == "c4":
-        part = bench_mixed(bcc, W, a)
+        part = bench_mixed(bcc, W, a, hw, cal)
         learned = part.get("learned")
         rec = {"metric": METRIC, "value": learned["value"] if learned else part["value"], "unit": "edge-env checks/s",
                "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -493,28 +635,8 @@ def main():
         return
 
     t0 = time.perf_counter()
-    ragged = a.config == "c2"
-    if ragged:
-        robot, sc, env_off, edges = W.make_ragged(a.config, seed=a.seed, scale=a.scale)
-        E, M = sc.num_envs, edges.shape[0]
-        pairs = M
-        shape = (M,)
-        desc = f"{robot}, {E} envs, {M} ragged pairs (Zipf(1.2) edges per env)"
-    elif a.config == "mbm":
-        from batch_cc_amd import mbm
-        robot = "panda"
-        problems = os.path.join(REPO, "tests", "golden", "mbm_panda.json")
-        sc, edges = mbm.readme_run(problems, robot, max(1, int(100 * a.scale)), max(1, int(1000 * a.scale)))
-        E, M = sc.num_envs, edges.shape[0]
-        pairs = E * M
-        shape = (M, E)
-        desc = f"{robot}, {E} MotionBenchMaker bookshelf_small envs x {M} Halton edges"
-    else:
-        robot, sc, edges = W.make_config(a.config, seed=a.seed, scale=a.scale)
-        E, M = sc.num_envs, edges.shape[0]
-        pairs = E * M
-        shape = (M, E)
-        desc = f"{robot}, {E} envs x {M} edges"
+    robot, sc, edges, env_off, E, M, pairs, shape, desc = make_workload(W, a)
+    ragged = env_off is not None
     log(f"workload {a.config}: {desc} = {pairs} pairs (generated in {time.perf_counter() - t0:.1f} s)")
 
     scene = bcc.Scene(sc)
@@ -548,7 +670,7 @@ def main():
         learned = learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, verdicts, ms)
         learned["value"] = pairs / (learned["ms_per_step"] / 1e3)
 
-    roofline = None
+    flops = trigs = None
     if not a.no_work:
         t1 = time.perf_counter()
         if ragged:
@@ -561,30 +683,31 @@ def main():
             sys.exit(1)
         flops = float(work.sum(dtype=np.float64))
         trigs = float(trig.sum(dtype=np.float64))
-        # the dominant kernel of the measured step: the ordered check of the
-        # learned step, else the plain kernel
-        kms = learned["ordered_check_ms"] if learned else ms
-        achieved = flops / (kms / 1e3) / 1e12
-        roofline = {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel": ("ordered_dense_kernel (the learned step's check)" if learned else
-                       ("ragged_kernel (hipGraph replay)" if ragged else "dense_kernel")),
-            "kernel_ms": kms,
-            "algorithmic_flops_per_launch": flops,
-            "sincos_per_launch": trigs,
-            "plain_kernel": {"ms": ms, "achieved": flops / (ms / 1e3) / 1e12,
-                             "frac": flops / (ms / 1e3) / 1e12 / FP32_PEAK_TFLOPS},
-            "note": "algorithmic work = the reference's source-level FP32 ops for these pairs in plain order "
-                    "(counted on the GPU by the reference-order replay, bcc_work_dense; sin/cos counted "
-                    "separately) / the kernel's hipEvent time; the kernel executes less than that work "
-                    "(early exits, culling, probing), so this is an effective rate",
-        }
+        del work, trig, wv
         log(f"work count: {flops:.4g} flops/launch ({time.perf_counter() - t1:.1f} s)")
+    # the dominant kernel of the measured step: the ordered check of the
+    # learned step, else the plain kernel
+    if learned:
+        roofline = build_roofline(C, hw, cal, is_kernel("bcc::ordered_dense_kernel<"),
+                                  "ordered_dense_kernel (the learned step's check)", learned["ordered_check_ms"],
+                                  flops, trigs)
+        roofline["plain_kernel"] = build_roofline(C, hw, cal, is_kernel("bcc::dense_kernel<"),
+                                                  "dense_kernel (plain order)", ms, flops, trigs)
+        roofline["plain_kernel"].pop("calibration", None)
+        if hw and "sq" in hw:
+            mops = C.per_step(hw["sq"], is_kernel("bcc::fused_order_kernel"), "SQ_INSTS_VALU_MFMA_MOPS_BF16")
+            if mops:
+                im = learned["inference_mfma"]
+                im["executed_flops_per_launch"] = 512.0 * mops
+                im["executed_tflops"] = 512.0 * mops / (learned["inference_ms"] / 1e3) / 1e12
+                im["executed_frac"] = im["executed_tflops"] / BF16_PEAK_TFLOPS
+                im["executed_note"] = ("SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 of fused_order_kernel (padded K: 96 "
+                                       "inputs per layer-1 row) over the whole inference time")
+    else:
+        roofline = build_roofline(C, hw, cal, is_kernel("bcc::ragged_kernel<" if ragged else "bcc::dense_kernel<"),
+                                  "ragged_kernel (hipGraph replay)" if ragged else "dense_kernel", ms, flops, trigs)
+    if roofline["achieved"] is None:
+        roofline = None
 
     cpu = None
     if not a.no_cpu_baseline:

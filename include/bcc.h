@@ -356,6 +356,11 @@ int bcc_debug_prims(const float *d_in, int n, float *d_out);
 /* Plain device copy of `bytes` (a multiple of 16): a known-traffic kernel used
  * to calibrate hardware byte counters. */
 int bcc_debug_copy(const void *d_src, void *d_dst, size_t bytes);
+/* Known FP32 VALU work: `blocks` workgroups of 256 threads; in every wave the
+ * lanes below active_lanes (0..64) run 8 independent chains of `iters` FMAs
+ * (then 7 adds); every thread writes one float to d_out[blocks * 256].  A
+ * known-work kernel used to calibrate the hardware VALU/FLOP counters. */
+int bcc_debug_valu(int blocks, int iters, int active_lanes, float *d_out);
 
 #ifdef __cplusplus
 }
