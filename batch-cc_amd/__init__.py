@@ -117,7 +117,7 @@ def lib():
     _sig(L, "bcc_debug_samples", I, [I, P, I, I, P, P, P])
     _sig(L, "bcc_debug_prims", I, [P, I, P])
     _sig(L, "bcc_debug_copy", I, [P, P, ctypes.c_size_t])
-    _sig(L, "bcc_debug_valu", I, [I, I, I, P])
+    _sig(L, "bcc_debug_valu", I, [I, I, I, P, ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_debug_force_variant", I, [I])
     _sig(L, "bcc_debug_cull_min", I, [I])
     _sig(L, "bcc_debug_scene_cull_min", I, [P, ctypes.POINTER(I)])
@@ -533,11 +533,14 @@ def debug_copy(d_src: DeviceBuffer, d_dst: DeviceBuffer, nbytes: int):
     _check(lib().bcc_debug_copy(d_src.ptr, d_dst.ptr, int(nbytes)))
 
 
-def debug_valu(blocks: int, iters: int, active_lanes: int, d_out: DeviceBuffer):
-    """Known FP32 work: per wave, lanes below active_lanes run 8 x iters FMAs."""
+def debug_valu(blocks: int, iters: int, active_lanes: int, d_out: DeviceBuffer) -> float:
+    """Known FP32 work: per wave, lanes below active_lanes run 8 x iters FMAs.
+    Returns the kernel's hipEvent time in ms."""
     if d_out.nbytes < 4 * 256 * int(blocks):
         raise BCCError("d_out must hold blocks * 256 floats")
-    _check(lib().bcc_debug_valu(int(blocks), int(iters), int(active_lanes), d_out.ptr))
+    ms = ctypes.c_double(0.0)
+    _check(lib().bcc_debug_valu(int(blocks), int(iters), int(active_lanes), d_out.ptr, ctypes.byref(ms)))
+    return ms.value
 
 
 # ------------------------------------------------------------ learned ordering

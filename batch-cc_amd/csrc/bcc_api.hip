@@ -1702,13 +1702,32 @@ int bcc_debug_copy(const void* d_src, void* d_dst, size_t bytes)
     return BCC_OK;
 }
 
-int bcc_debug_valu(int blocks, int iters, int active_lanes, float* d_out)
+int bcc_debug_valu(int blocks, int iters, int active_lanes, float* d_out, double* ms)
 {
     if (blocks < 1 || iters < 0 || active_lanes < 0 || active_lanes > 64 || !d_out)
         return fail(BCC_ERR_INVALID, "bad VALU calibration arguments");
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (ms) {
+        BCC_HIP(hipEventCreate(&t0), "hipEventCreate");
+        BCC_HIP(hipEventCreate(&t1), "hipEventCreate");
+        BCC_HIP(hipEventRecord(t0, 0), "hipEventRecord");
+    }
     hipLaunchKernelGGL(valu_kernel, dim3(blocks), dim3(256), 0, 0, d_out, iters, active_lanes, 0.999f, 1e-3f);
-    BCC_HIP(hipGetLastError(), "VALU calibration kernel");
-    BCC_HIP(hipDeviceSynchronize(), "VALU calibration kernel");
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && ms)
+        e = hipEventRecord(t1, 0);
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    float t = 0.0f;
+    if (e == hipSuccess && ms)
+        e = hipEventElapsedTime(&t, t0, t1);
+    if (t0)
+        (void)hipEventDestroy(t0);
+    if (t1)
+        (void)hipEventDestroy(t1);
+    BCC_HIP(e, "VALU calibration kernel");
+    if (ms)
+        *ms = t;
     return BCC_OK;
 }
 

@@ -446,6 +446,20 @@ def build_roofline(C, hw, cal, match, kernel: str, kms: float, ref_flops=None, r
     return rf
 
 
+def fp32_fma_rate(bcc) -> dict:
+    """Measured FP32 VALU rate of the known-FMA kernel (16 waves per SIMD,
+    independent chains, non-packed FMAs as every kernel here is built): the
+    practical ceiling beside the datasheet peak."""
+    from batch_cc_amd import counters as C
+    blocks, iters = 4096, 1024
+    out = bcc.DeviceBuffer(4 * 256 * blocks)
+    bcc.debug_valu(blocks, iters, 64, out)                    # warm-up
+    ms = min(bcc.debug_valu(blocks, iters, 64, out) for _ in range(3))
+    flops = (blocks * 4) * 64 * (2 * C.VALU_CHAINS * iters + C.VALU_CHAINS - 1)
+    return {"tflops": flops / (ms / 1e3) / 1e12, "ms": ms, "flops": float(flops),
+            "kernel": f"valu_kernel: {blocks} x 256 threads, 8 chains x {iters} v_fma_f32 per lane"}
+
+
 def is_kernel(prefix: str):
     return lambda name: name.startswith(prefix)
 
@@ -616,10 +630,14 @@ def main():
         hw = C.collect(os.path.abspath(__file__), child_args(a), log)
         cal = C.calibration(hw) if hw else {}
         log(f"counters: {len(hw or {})} passes in {time.perf_counter() - t0:.1f} s; calibration {cal}")
+    fma = fp32_fma_rate(bcc)
+    log(f"known-FMA kernel: {fma['tflops']:.1f} TFLOP/s FP32 (non-packed FMAs)")
 
     if a.config == "c4":
         part = bench_mixed(bcc, W, a, hw, cal)
         learned = part.get("learned")
+        if part["roofline"]:
+            part["roofline"]["measured_fp32_fma"] = fma
         rec = {"metric": METRIC, "value": learned["value"] if learned else part["value"], "unit": "edge-env checks/s",
                "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
                "ms_per_step": learned["ms_per_step"] if learned else part["ms_per_step"], "higher_is_better": True,
@@ -708,6 +726,8 @@ def main():
                                   "ragged_kernel (hipGraph replay)" if ragged else "dense_kernel", ms, flops, trigs)
     if roofline["achieved"] is None:
         roofline = None
+    else:
+        roofline["measured_fp32_fma"] = fma
 
     cpu = None
     if not a.no_cpu_baseline:

@@ -359,8 +359,9 @@ int bcc_debug_copy(const void *d_src, void *d_dst, size_t bytes);
 /* Known FP32 VALU work: `blocks` workgroups of 256 threads; in every wave the
  * lanes below active_lanes (0..64) run 8 independent chains of `iters` FMAs
  * (then 7 adds); every thread writes one float to d_out[blocks * 256].  A
- * known-work kernel used to calibrate the hardware VALU/FLOP counters. */
-int bcc_debug_valu(int blocks, int iters, int active_lanes, float *d_out);
+ * known-work kernel used to calibrate the hardware VALU/FLOP counters and to
+ * measure the FP32 FMA rate; ms (optional) receives its hipEvent time. */
+int bcc_debug_valu(int blocks, int iters, int active_lanes, float *d_out, double *ms);
 
 #ifdef __cplusplus
 }
