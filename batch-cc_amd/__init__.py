@@ -70,6 +70,7 @@ def lib():
     _sig(L, "bcc_work_dense", I, [I, P, P, I, I, P, P, P, P])
     _sig(L, "bcc_work_pairs", I, [I, P, P, I, P, I, P, P, P, P])
     _sig(L, "bcc_check_ragged", I, [I, P, P, P, I, P])
+    _sig(L, "bcc_audit_clearance", I, [I, P, P, I, P, I, P, P])
     _sig(L, "bcc_check_ragged_noalloc", I, [I, P, P, P, ctypes.c_int64, I, P, P])
     _sig(L, "bcc_ragged_plan_create", I, [I, P, P, P, I, P, I, ctypes.POINTER(P)])
     _sig(L, "bcc_ragged_plan_launch", I, [P, P])
@@ -333,6 +334,28 @@ def work_pairs(robot, scene: Scene, d_edges: DeviceBuffer, pair_env: np.ndarray,
                                 dt.ptr, do.ptr, None))
     synchronize()
     return dw.to_array(np.uint64, (n,)), dt.to_array(np.uint64, (n,)), do.to_array(np.uint8, (n,))
+
+
+def audit_clearance(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, pair_env=None,
+                    resolution: int = 32) -> np.ndarray:
+    """Smallest gated clearance (metres, FP32) of every pair over all of its
+    samples: [num_edges, num_envs] for a dense batch, [num_pairs] with
+    pair_env (pair p = edge p in environment pair_env[p])."""
+    n = int(num_edges)
+    if pair_env is not None:
+        pe = np.ascontiguousarray(pair_env, np.int32)
+        if pe.size != n:
+            raise BCCError("pair_env must have one entry per pair")
+        d_pe = DeviceBuffer.from_array(pe)
+        shape = (n,)
+    else:
+        d_pe = None
+        shape = (n, scene.num_envs)
+    d_c = DeviceBuffer(4 * max(1, int(np.prod(shape))))
+    _check(lib().bcc_audit_clearance(robot_id(robot), scene.handle, d_edges.ptr, n, d_pe.ptr if d_pe else None,
+                                     int(resolution), d_c.ptr, None))
+    synchronize()
+    return d_c.to_array(np.float32, shape)
 
 
 def check_ragged(robot, scene: Scene, env_off, edges, resolution: int = 32) -> np.ndarray:

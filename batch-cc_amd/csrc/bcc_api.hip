@@ -883,6 +883,32 @@ static int work_common(int robot, const bcc_scene* scene, const float* d_edges, 
     return BCC_OK;
 }
 
+int bcc_audit_clearance(int robot, const bcc_scene* scene, const float* d_edges, int num_edges,
+                        const int32_t* d_pair_env, int resolution, float* d_clearance, void* stream)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_clearance))
+        return rc;
+    const size_t pairs = d_pair_env ? (size_t)num_edges : (size_t)num_edges * (size_t)scene->num_envs;
+    if (pairs == 0)
+        return BCC_OK;
+    bcc::ModelDev md;
+    if (int rc = device_model(robot, &md))
+        return rc;
+    bcc::AuditArgs a{scene->d_ref_obs, scene->d_ref_meta, d_edges, num_edges, scene->num_envs, resolution, md,
+                     d_pair_env, d_clearance};
+    const hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_audit_panda(a, s); break;
+    case BCC_FETCH: e = bcc::launch_audit_fetch(a, s); break;
+    case BCC_BAXTER: e = bcc::launch_audit_baxter(a, s); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id");
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "audit kernel launch");
+    return BCC_OK;
+}
+
 }  // extern "C"
 
 // A plan holds one batch's tile list, buffers and captured graph.  Two

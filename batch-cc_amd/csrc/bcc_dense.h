@@ -865,6 +865,170 @@ __global__ __launch_bounds__(kBlock) void work_kernel(const float* __restrict__ 
     }
 }
 
+// ------------------------------------------------------------ clearance audit
+// Signed clearance (metres, negative = penetration) of one robot sphere of
+// padded radius rp against one primitive: the distance the reference's
+// squared-distance test compares, minus the radii.  FP32; the audit uses it
+// only to find pairs near contact, which the FP64 oracle then decides.
+__device__ __forceinline__ float clr_sphere(cfptr o, float x, float y, float z, float rp)
+{
+    const float xs = o[0] - x, ys = o[1] - y, zs = o[2] - z;
+    return sqrtf(fmaf(zs, zs, fmaf(ys, ys, xs * xs))) - (o[3] + rp);
+}
+
+__device__ __forceinline__ float clr_capsule(cfptr c, float x, float y, float z, float rp, bool z_axis)
+{
+    const float dot = z_axis ? (z - c[2]) * c[5] : fmaf(z - c[2], c[5], fmaf(y - c[1], c[4], (x - c[0]) * c[3]));
+    const float t = fmaxf(fminf(dot * c[7], 1.0f), 0.0f);
+    const float xs = z_axis ? x - c[0] : x - fmaf(c[3], t, c[0]);
+    const float ys = z_axis ? y - c[1] : y - fmaf(c[4], t, c[1]);
+    const float zs = z - fmaf(c[5], t, c[2]);
+    return sqrtf(fmaf(zs, zs, fmaf(ys, ys, xs * xs))) - (rp + c[6]);
+}
+
+__device__ __forceinline__ float clr_cuboid(cfptr b, float x, float y, float z, float rp, bool z_axis)
+{
+    const float xs = x - b[0], ys = y - b[1], zs = z - b[2];
+    const float d1 = z_axis ? fmaf(b[4], ys, b[3] * xs) : fmaf(b[5], zs, fmaf(b[4], ys, b[3] * xs));
+    const float d2 = z_axis ? fmaf(b[7], ys, b[6] * xs) : fmaf(b[8], zs, fmaf(b[7], ys, b[6] * xs));
+    const float d3 = z_axis ? zs : fmaf(b[11], zs, fmaf(b[10], ys, b[9] * xs));
+    const float a1 = fmaxf(0.0f, fabsf(d1) - b[12]);
+    const float a2 = fmaxf(0.0f, fabsf(d2) - b[13]);
+    const float a3 = fmaxf(0.0f, fabsf(d3) - b[14]);
+    return sqrtf(fmaf(a3, a3, fmaf(a2, a2, a1 * a1))) - rp;
+}
+
+// Smallest clearance of one padded sphere over the environment, in the
+// reference's lists (src/utils.cuh:213-245: no cylinders).
+__device__ __forceinline__ float env_clearance(const RefEnvView& E, float x, float y, float z, float rp)
+{
+    float best = INFINITY;
+    for (int k = 0; k < E.ns; ++k)
+        best = fminf(best, clr_sphere(E.sph + 4 * k, x, y, z, rp));
+    for (int k = 0; k < E.nc; ++k)
+        best = fminf(best, clr_capsule(E.cap + 8 * k, x, y, z, rp, false));
+    for (int k = 0; k < E.nzc; ++k)
+        best = fminf(best, clr_capsule(E.zcap + 8 * k, x, y, z, rp, true));
+    for (int k = 0; k < E.ncb; ++k)
+        best = fminf(best, clr_cuboid(E.cub + 16 * k, x, y, z, rp, false));
+    for (int k = 0; k < E.nzcb; ++k)
+        best = fminf(best, clr_cuboid(E.zcub + 16 * k, x, y, z, rp, true));
+    return best;
+}
+
+__device__ __forceinline__ float self_clearance(const float* a, const float* b)
+{
+    const float xs = a[0] - b[0], ys = a[1] - b[1], zs = a[2] - b[2];
+    return sqrtf(fmaf(zs, zs, fmaf(ys, ys, xs * xs))) - (a[3] + b[3]);
+}
+
+// Gated clearance of one sample: the minimum over the check program's items
+// of (lone item: its gate) or max(gate, smallest fine clearance) -- negative
+// exactly when a gate and one of its fine spheres (pairs) both collide.  A
+// gate no lower than the running minimum cannot lower it, so its fine spheres
+// are skipped (the value is unchanged).
+template <class Robot>
+__device__ __forceinline__ float ref_sample_clearance(const float (&q)[Robot::dim], const RefEnvView& R,
+                                                      const ModelDev& m)
+{
+    float P[Robot::num_spheres * 4];
+    Robot::spheres(q, P);
+    float best = INFINITY;
+    for (int it = 0; it < m.num_program; ++it) {
+        const int g = m.prog_index[it];
+        if (m.prog_kind[it] == 0) {
+            const float* c = P + 4 * m.env_gate[g];
+            const float cg = env_clearance(R, c[0], c[1], c[2], (float)((double)c[3] + 0.005));
+            if (m.env_lone[g] || cg >= best) {
+                best = fminf(best, cg);
+                continue;
+            }
+            float fmin = INFINITY;
+            for (int f = m.env_fine_off[g]; f < m.env_fine_off[g + 1]; ++f) {
+                const float* cf = P + 4 * m.env_fine[f];
+                fmin = fminf(fmin, env_clearance(R, cf[0], cf[1], cf[2], (float)((double)cf[3] + 0.005)));
+            }
+            best = fminf(best, fmaxf(cg, fmin));
+        } else {
+            const float cg = self_clearance(P + 4 * m.self_gate_a[g], P + 4 * m.self_gate_b[g]);
+            if (m.self_lone[g] || cg >= best) {
+                best = fminf(best, cg);
+                continue;
+            }
+            float fmin = INFINITY;
+            for (int ia = m.self_a_off[g]; ia < m.self_a_off[g + 1]; ++ia)
+                for (int ib = m.self_b_off[g]; ib < m.self_b_off[g + 1]; ++ib)
+                    fmin = fminf(fmin, self_clearance(P + 4 * m.self_a[ia], P + 4 * m.self_b[ib]));
+            best = fminf(best, fmaxf(cg, fmin));
+        }
+    }
+    return best;
+}
+
+// Clearance audit: per pair, the minimum gated clearance over all 32 n
+// samples (no early exit), one half-wave per pair as the work counter
+// (grid-stride; dense: pair p = edge * num_envs + env, with pair_env:
+// env = pair_env[p], edge = p).  A pair collides exactly when some sample's
+// gated clearance is negative, so FP32 verdicts can differ from exact
+// arithmetic only where |clearance| is within FP32 rounding of zero.
+template <class Robot>
+__global__ __launch_bounds__(kBlock) void audit_kernel(const float* __restrict__ ref_obs,
+                                                       const int* __restrict__ ref_meta,
+                                                       const float* __restrict__ edges, int num_edges, int num_envs,
+                                                       int resolution, ModelDev model,
+                                                       const int* __restrict__ pair_env,
+                                                       float* __restrict__ clearance)
+{
+    constexpr int D = Robot::dim;
+    const int lane = threadIdx.x & 63;
+    const int t = lane & 31;
+    const long long pairs = pair_env ? (long long)num_edges : (long long)num_edges * num_envs;
+    const long long halves = ((long long)gridDim.x * kBlock) >> 5;
+    for (long long p = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 5; p < pairs; p += halves) {
+        const int edge = pair_env ? (int)p : (int)(p / num_envs);
+        const int env = pair_env ? pair_env[p] : (int)(p - (long long)edge * num_envs);
+        const RefEnvView R = load_ref_env(ref_obs, ref_meta, env);
+        float q[D], d[D];
+        const int n = setup_pair<D>(edges + (size_t)edge * 2 * D, resolution, t, q, d);
+        float best = INFINITY;
+        for (int i = 0; i < n; ++i) {
+            best = fminf(best, ref_sample_clearance<Robot>(q, R, model));
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                q[j] = q[j] + d[j];
+        }
+#pragma unroll
+        for (int o = 16; o > 0; o >>= 1)
+            best = fminf(best, __shfl_xor(best, o));
+        if (t == 0)
+            clearance[p] = best;
+    }
+}
+
+struct AuditArgs {
+    const float* ref_obs;
+    const int* ref_meta;
+    const float* edges;
+    int num_edges;
+    int num_envs;
+    int resolution;
+    ModelDev model;
+    const int* pair_env;
+    float* clearance;
+};
+
+template <class Robot>
+hipError_t launch_audit(const AuditArgs& a, hipStream_t stream)
+{
+    if (a.num_edges == 0 || a.num_envs == 0)
+        return hipSuccess;
+    const long long halves = a.pair_env ? (long long)a.num_edges : (long long)a.num_edges * a.num_envs;
+    const long long blocks = std::min<long long>((halves + 7) / 8, 65536);
+    hipLaunchKernelGGL(audit_kernel<Robot>, dim3((unsigned)blocks), dim3(kBlock), 0, stream, a.ref_obs, a.ref_meta,
+                       a.edges, a.num_edges, a.num_envs, a.resolution, a.model, a.pair_env, a.clearance);
+    return hipGetLastError();
+}
+
 // Training labels: for each pair, bit t of labels[p] is set when one of the
 // first `steps` samples of bin t (lane t's samples t n + i, i < steps; steps
 // = 0: all n) collides.  Every such sample is evaluated (no early exit) with
@@ -1039,6 +1203,7 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
     hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream);                                 \
     hipError_t launch_learn_##name(const LearnArgs& a, hipStream_t stream);                                   \
     hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream);                                   \
+    hipError_t launch_audit_##name(const AuditArgs& a, hipStream_t stream);                                   \
     extern const VariantKnobs knobs_##name;
 
 BCC_DECLARE_ROBOT(panda)

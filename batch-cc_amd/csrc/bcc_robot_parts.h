@@ -9,7 +9,8 @@
 //                   the high- and low-occupancy kernel variants
 // BCC_PART selects what this translation unit instantiates:
 //   unset  everything
-//   0      host dispatchers and the small kernels (FK, samples, work, labels)
+//   0      host dispatchers and the small kernels (FK, samples, work, labels,
+//          clearance audit)
 //   1, 7   ordered dense kernel, low / high occupancy
 //   2, 3   plain dense kernel, high / low occupancy
 //   4, 5   ragged kernel, high / low occupancy
@@ -90,6 +91,11 @@ hipError_t BCC_CAT2(launch_work_, BCC_ROBOT)(const WorkArgs& a, hipStream_t stre
 hipError_t BCC_CAT2(launch_label_, BCC_ROBOT)(const LabelArgs& a, hipStream_t stream)
 {
     return launch_label<robot_parts::RobotT>(a, stream);
+}
+
+hipError_t BCC_CAT2(launch_audit_, BCC_ROBOT)(const AuditArgs& a, hipStream_t stream)
+{
+    return launch_audit<robot_parts::RobotT>(a, stream);
 }
 #endif
 

@@ -137,6 +137,22 @@ int bcc_work_pairs(int robot, const bcc_scene *scene, const float *d_edges, int 
                    int resolution, uint64_t *d_work, uint64_t *d_trig, uint8_t *d_results, void *stream);
 
 /*
+ * Clearance audit (no reference counterpart; the reference's validator only
+ * diffs verdicts, scripts/batch_cc_validate.cpp:1074-1103): per pair, the
+ * smallest gated clearance in metres over all 32 n samples, evaluated without
+ * early exit in FP32 -- per sample, the minimum over the check program's
+ * items of (lone item: its gate) or max(gate, smallest fine clearance), where
+ * an environment clearance is the distance to the nearest primitive minus the
+ * padded radius (r + 0.005) and a self clearance the distance minus both
+ * radii.  Negative exactly when the pair collides; pairs within FP32 rounding
+ * of zero are the ones whose verdict can depend on the arithmetic.  Dense when
+ * d_pair_env is NULL (d_clearance[edge * num_envs + env]); else pair p checks
+ * edge p against environment d_pair_env[p] (num_edges = number of pairs).
+ */
+int bcc_audit_clearance(int robot, const bcc_scene *scene, const float *d_edges, int num_edges,
+                        const int32_t *d_pair_env, int resolution, float *d_clearance, void *stream);
+
+/*
  * Ragged batches (no reference counterpart: the reference only takes the
  * dense cross product, src/batch_cc.cu:244-261).  Environment e owns pairs
  * [env_offsets[e], env_offsets[e+1]) (CSR, host array of num_envs + 1);
