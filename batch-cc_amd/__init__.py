@@ -66,6 +66,7 @@ def lib():
     _sig(L, "bcc_scene_num_envs", I, [P])
     _sig(L, "bcc_check_dense", I, [I, P, P, I, I, P])
     _sig(L, "bcc_check_dense_async", I, [I, P, P, I, I, P, P])
+    _sig(L, "bcc_check_dense_endpoints", I, [I, P, P, I, I, P, P])
     _sig(L, "bcc_time_dense", I, [I, P, P, I, I, P, I, I, P, ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_work_dense", I, [I, P, P, I, I, P, P, P, P])
     _sig(L, "bcc_work_pairs", I, [I, P, P, I, P, I, P, P, P, P])
@@ -299,6 +300,20 @@ def check_dense_device(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: in
                        d_out: DeviceBuffer, stream=None):
     _check(lib().bcc_check_dense_async(robot_id(robot), scene.handle, d_edges.ptr, int(num_edges), int(resolution),
                                        d_out.ptr, stream))
+
+
+def check_dense_endpoints(robot, scene: Scene, edges, resolution: int = 32) -> np.ndarray:
+    """Verdicts [M, E] with the reference validator's vamp semantics: the
+    edge collides when the motion s -> e or the configuration e collides
+    (s is the motion's own first sample)."""
+    e = _edges_array(robot, edges)
+    M, E = e.shape[0], scene.num_envs
+    d_e = DeviceBuffer.from_array(e)
+    d_o = DeviceBuffer(max(1, M * E))
+    _check(lib().bcc_check_dense_endpoints(robot_id(robot), scene.handle, d_e.ptr, M, int(resolution), d_o.ptr,
+                                           None))
+    synchronize()
+    return d_o.to_array(np.uint8, (M, E))
 
 
 def time_dense(robot, scene: Scene, d_edges: DeviceBuffer, num_edges: int, resolution: int, d_out: DeviceBuffer,

@@ -495,6 +495,23 @@ __global__ void half_boxes_kernel(const float* __restrict__ in, uint64_t mask, f
         }
 }
 
+// Degenerate edges (e, e) of a batch: out[m] = (end of edge m, end of edge m).
+__global__ void end_edges_kernel(const float* __restrict__ edges, int num_edges, int dim, float* __restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;   // (edge, half, joint)
+    if (i >= (long long)num_edges * 2 * dim)
+        return;
+    const long long m = i / (2 * dim);
+    const int j = (int)(i % dim);
+    out[i] = edges[m * 2 * dim + dim + j];
+}
+
+__global__ void or_bytes_kernel(uint8_t* __restrict__ acc, const uint8_t* __restrict__ v, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        acc[i] = (uint8_t)(acc[i] | v[i]);
+}
+
 __global__ void copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -646,6 +663,52 @@ int bcc_check_dense_async(int robot, const bcc_scene* scene, const float* d_edge
         return rc;
     bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
     return dispatch_dense(robot, scene, a, (hipStream_t)stream);
+}
+
+int bcc_check_dense_endpoints(int robot, const bcc_scene* scene, const float* d_edges, int num_edges,
+                              int resolution, uint8_t* d_results, void* stream)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    const size_t pairs = (size_t)num_edges * (size_t)scene->num_envs;
+    if (pairs == 0)
+        return BCC_OK;
+    const hipStream_t s = (hipStream_t)stream;
+    // the motion s -> e (its first sample, lane 0's fmaf(d, 0, s), is s
+    // itself, so the reference's separate s -> s check adds nothing) ...
+    bcc::DenseArgs a{scene->d_obs, scene->d_meta, d_edges, num_edges, scene->num_envs, resolution, d_results};
+    if (int rc = dispatch_dense(robot, scene, a, s))
+        return rc;
+    // ... OR the motion e -> e: n = 1, every lane samples e exactly
+    const int dim = bcc_robot_dimension(robot);
+    const long long nf = (long long)num_edges * 2 * dim;
+    float* d_end = nullptr;
+    uint8_t* d_tmp = nullptr;
+    hipError_t e = hipMallocAsync((void**)&d_end, sizeof(float) * (size_t)nf, s);
+    if (e == hipSuccess)
+        e = hipMallocAsync((void**)&d_tmp, pairs, s);
+    int rc = BCC_OK;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(end_edges_kernel, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, d_edges,
+                           num_edges, dim, d_end);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        bcc::DenseArgs b{scene->d_obs, scene->d_meta, d_end, num_edges, scene->num_envs, resolution, d_tmp};
+        rc = dispatch_dense(robot, scene, b, s);
+        if (rc == BCC_OK) {
+            hipLaunchKernelGGL(or_bytes_kernel, dim3(1024), dim3(256), 0, s, d_results, d_tmp, (long long)pairs);
+            e = hipGetLastError();
+        }
+    }
+    if (d_end)
+        (void)hipFreeAsync(d_end, s);
+    if (d_tmp)
+        (void)hipFreeAsync(d_tmp, s);
+    if (rc != BCC_OK)
+        return rc;
+    BCC_HIP(e, "endpoint check");
+    return BCC_OK;
 }
 
 int bcc_check_dense(int robot, const bcc_scene* scene, const float* edges, int num_edges, int resolution,

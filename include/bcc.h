@@ -113,6 +113,17 @@ int bcc_check_dense_async(int robot, const bcc_scene *scene, const float *d_edge
                           uint8_t *d_results, void *stream);
 
 /*
+ * Endpoint-inclusive dense check on device buffers (the reference
+ * validator's vamp semantics, scripts/batch_cc_validate.cpp:627-646: an edge
+ * is valid when the motions s -> e, s -> s and e -> e all validate):
+ * d_results = plain verdict of (s, e) OR plain verdict of (e, e).  (s, s)
+ * needs no pass of its own: the first sample of (s, e) is s exactly.
+ * Asynchronous on `stream`; allocates its temporaries stream-ordered.
+ */
+int bcc_check_dense_endpoints(int robot, const bcc_scene *scene, const float *d_edges, int num_edges,
+                              int resolution, uint8_t *d_results, void *stream);
+
+/*
  * Average device time (ms) of `iters` launches of the dense kernel after
  * `warmup` untimed ones, measured with hipEvents on the launch stream.
  */
