@@ -125,6 +125,12 @@ def lib():
     _sig(L, "bcc_debug_scene_cull_min", I, [P, ctypes.POINTER(I)])
     _sig(L, "bcc_debug_dense_chunk", I, [I])
     _sig(L, "bcc_debug_half_boxes", I, [P, ctypes.c_uint64, P])
+    _sig(L, "bcc_check_dense_learned", I, [I, P, P, P, I, I, P, P, P, P, P])
+    _sig(L, "bcc_time_dense_learned", I, [I, P, P, P, I, I, P, I, I, P, ctypes.POINTER(ctypes.c_double)])
+    _sig(L, "bcc_ragged_plan_create_learned", I, [I, P, P, P, P, I, P, I, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_debug_learned_tile", I, [I])
+    _sig(L, "bcc_debug_learned_far", I, [I])
+    _sig(L, "bcc_debug_learned_passes", I, [I])
     _lib = L
     return L
 
@@ -404,11 +410,14 @@ def check_ragged_noalloc(robot, scene: Scene, d_env_off: DeviceBuffer, d_edges: 
 class RaggedPlan:
     """Tile list + captured hipGraph for repeated ragged launches on device buffers.
     d_order (optional): a learned visiting order per pair (uint32 per pair,
-    e.g. from Model.order_pairs), read at every launch."""
+    e.g. from Model.order_pairs), read at every launch.  learned=True: the
+    fused learned kernel (probe, inference of the undecided pairs with
+    `model`, probe, plain steps) over a snapshot of `model` taken now
+    (model None: fixed second-probe bins)."""
 
     def __init__(self, robot, scene: Scene, env_off, d_edges: DeviceBuffer, d_out: DeviceBuffer,
                  resolution: int = 32, tile_pairs: int = 0, groups_per_cu: int = 0,
-                 d_order: DeviceBuffer | None = None):
+                 d_order: DeviceBuffer | None = None, learned: bool = False, model: "Model | None" = None):
         self.off = np.ascontiguousarray(env_off, np.int64)
         self.robot = robot
         self.scene, self.d_edges, self.d_out, self.d_order = scene, d_edges, d_out, d_order
@@ -427,7 +436,14 @@ class RaggedPlan:
             raise BCCError(f"d_out holds {d_out.nbytes} bytes, {total} pairs need {total}")
         if d_order is not None and d_order.nbytes < 4 * total:
             raise BCCError(f"d_order holds {d_order.nbytes} bytes, {total} pairs need {4 * total}")
-        if d_order is None:
+        if learned:
+            if d_order is not None:
+                raise BCCError("a learned plan infers its own order: d_order must be None")
+            _check(lib().bcc_ragged_plan_create_learned(robot_id(robot), model.handle if model else None,
+                                                        scene.handle, _ptr(self.off), d_edges.ptr, int(resolution),
+                                                        d_out.ptr, int(tile_pairs), int(groups_per_cu),
+                                                        ctypes.byref(self.handle)))
+        elif d_order is None:
             _check(lib().bcc_ragged_plan_create_tuned(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
                                                       int(resolution), d_out.ptr, int(tile_pairs),
                                                       int(groups_per_cu), ctypes.byref(self.handle)))
@@ -740,6 +756,60 @@ class Model:
             self.close()
         except Exception:
             pass
+
+
+def check_dense_learned(robot, model: "Model | None", scene: Scene, d_edges: DeviceBuffer, num_edges: int,
+                        resolution: int = 32, d_out: DeviceBuffer | None = None, stage: bool = False,
+                        words: bool = False, stats: bool = False) -> dict:
+    """One fused learned step over a dense batch (bcc_check_dense_learned).
+    Returns {"verdicts": bool [M, E]} plus, on request, "stage" (uint8 [M,
+    E]: p = decided by probe p + 1, else passes - 1 + plain steps; passes =
+    2 unless set with debug_learned_passes),
+    "words" (uint32 [M, E], 0xffffffff where nothing was inferred) and
+    "stats" (pairs decided by probes 1, 2 and 3, inferred, sent to the step
+    loop)."""
+    M, E = int(num_edges), scene.num_envs
+    n = M * E
+    d = d_out if d_out is not None else DeviceBuffer(max(n, 1))
+    if d.nbytes < n:
+        raise BCCError("result buffer too small")
+    d_stage = DeviceBuffer(max(n, 1)) if stage else None
+    d_words = DeviceBuffer.from_array(np.full(max(n, 1), 0xffffffff, np.uint32)) if words else None
+    d_stats = DeviceBuffer.from_array(np.zeros(5, np.int32)) if stats else None
+    _check(lib().bcc_check_dense_learned(robot_id(robot), model.handle if model else None, scene.handle,
+                                         d_edges.ptr, M, int(resolution), d.ptr, d_stage.ptr if d_stage else None,
+                                         d_words.ptr if d_words else None, d_stats.ptr if d_stats else None, None))
+    synchronize()
+    out = {"verdicts": d.to_array(np.uint8, (M, E)).astype(bool)}
+    if stage:
+        out["stage"] = d_stage.to_array(np.uint8, (M, E))
+    if words:
+        out["words"] = d_words.to_array(np.uint32, (max(n, 1),))[:n].reshape(M, E)
+    if stats:
+        out["stats"] = d_stats.to_array(np.int32, (5,))
+    return out
+
+
+def time_dense_learned(robot, model: "Model | None", scene: Scene, d_edges: DeviceBuffer, num_edges: int,
+                       d_out: DeviceBuffer, warmup: int, iters: int, resolution: int = 32) -> float:
+    """Average ms of a fused learned step (hipEvents on the launch stream)."""
+    ms = ctypes.c_double(0.0)
+    _check(lib().bcc_time_dense_learned(robot_id(robot), model.handle if model else None, scene.handle, d_edges.ptr,
+                                        int(num_edges), int(resolution), d_out.ptr, int(warmup), int(iters), None,
+                                        ctypes.byref(ms)))
+    return ms.value
+
+
+def debug_learned_tile(pairs: int):
+    _check(lib().bcc_debug_learned_tile(int(pairs)))
+
+
+def debug_learned_far(radius: int):
+    _check(lib().bcc_debug_learned_far(int(radius)))
+
+
+def debug_learned_passes(passes: int):
+    _check(lib().bcc_debug_learned_passes(int(passes)))
 
 
 def debug_gemm(ta: bool, tb: bool, a: np.ndarray, b: np.ndarray, m: int, n: int, k: int, alpha=1.0, beta=0.0,

@@ -29,9 +29,9 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-# (source, extra macros, object name): each robot's kernels compile as nine
+# (source, extra macros, object name): each robot's kernels compile as eleven
 # parts in parallel (see csrc/bcc_robot_parts.h)
-ROBOT_PARTS = 9
+ROBOT_PARTS = 11
 SOURCES = [("bcc_api.hip", (), "bcc_api"), ("bcc_mlp.hip", (), "bcc_mlp")] + [
     (f"robot_{r}.hip", (f"BCC_PART={k}",), f"robot_{r}_p{k}") for r in ROBOTS for k in range(ROBOT_PARTS)]
 

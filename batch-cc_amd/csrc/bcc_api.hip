@@ -810,6 +810,198 @@ int bcc_time_dense_ex(int robot, const bcc_scene* scene, const float* d_edges, i
 
 }  // extern "C"
 
+// ------------------------------------------------------------ fused learned step
+namespace {
+
+std::atomic<int> g_learned_tile{0};   // > 0: forced pairs per dense tile (tuning hook)
+std::atomic<int> g_learned_far{0};    // first-probe exclusion radius of the second probe's picks
+std::atomic<int> g_learned_passes{bcc::kLearnPassesDefault};   // probe passes (tuning hook)
+
+// Pairs per tile of a learned launch (a wave's tile): about 16 tiles per
+// resident wave, a power of two in [32, 128] (MI355X: c1 runs best at 64 and
+// c3 at 128 of 16 ... 256).
+int learned_tile(long long pairs, long long resident_waves)
+{
+    if (const int forced = g_learned_tile.load(); forced > 0)
+        return std::min(forced, bcc::kLearnTile);
+    const double per = (double)pairs / (16.0 * (double)std::max<long long>(resident_waves, 1));
+    int tile = 32;
+    while (tile < 128 && 2.0 * tile <= per)
+        tile *= 2;
+    return tile;
+}
+
+// Bins at distance >= radius from every bin of the first probe: the second
+// probe's picks prefer them.
+uint32_t learned_far_mask(int radius)
+{
+    uint32_t far = 0xffffffffu;
+    for (int k = 0; k < bcc::kProbe1; ++k) {
+        const int b = (32 * k + bcc::fixed_offset(0)) / bcc::kProbe1;
+        for (int d = -(radius - 1); d <= radius - 1; ++d)
+            if (b + d >= 0 && b + d < 32)
+                far &= ~(1u << (b + d));
+    }
+    return far;
+}
+
+int dispatch_learned(int robot, const bcc::LearnedArgs& a, hipStream_t s)
+{
+    hipError_t e;
+    switch (robot) {
+    case BCC_PANDA: e = bcc::launch_learned_panda(a, s); break;
+    case BCC_FETCH: e = bcc::launch_learned_fetch(a, s); break;
+    case BCC_BAXTER: e = bcc::launch_learned_baxter(a, s); break;
+    default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
+    }
+    if (e != hipSuccess)
+        return hip_fail(e, "learned kernel launch");
+    return BCC_OK;
+}
+
+// One dense learned step on stream s: the model's operands (fragments and
+// environment codes, two small kernels), then the fused learned kernel,
+// persistent over the dense tiles (a wave that ends a tile takes the next
+// one from a counter, so no workgroup waits for its slowest wave); the
+// counter is a stream-ordered allocation of the call.
+int dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const float* d_edges, int num_edges,
+                  int resolution, uint8_t* d_results, uint8_t* d_stage, uint32_t* d_words, int32_t* d_stats,
+                  hipStream_t s)
+{
+    if (num_edges == 0 || scene->num_envs == 0)
+        return BCC_OK;
+    bcc::MlpArgs m{};
+    if (model) {
+        if (int rc = bcc_detail::model_mlp_args(model, scene, s, &m))
+            return rc;
+    }
+    m.far0 = learned_far_mask(g_learned_far.load());
+    m.passes = g_learned_passes.load();
+    const int variant = choose_variant(robot, mean_obstacles(scene));
+    const bcc::VariantKnobs& k = knobs(robot);
+    // one workgroup per resident slot: the variant's waves per SIMD
+    const long long slots = (long long)device_cus() * (variant ? k.hi_waves : k.lo_waves);
+    const int tile = learned_tile((long long)scene->num_envs * num_edges, slots * (bcc::kBlock / 64));
+    const int chunks = (num_edges + tile - 1) / tile;
+    const long long tiles = (long long)scene->num_envs * chunks;
+    if (tiles > 0x7fffffffLL - 65536)
+        return fail(BCC_ERR_INVALID, "too many tiles for one dense learned launch");
+    int* d_counter = nullptr;
+    BCC_HIP(hipMallocAsync((void**)&d_counter, sizeof(int), s), "learned tile counter");
+    hipLaunchKernelGGL(bcc::reset_counter_kernel, dim3(1), dim3(64), 0, s, d_counter);
+    bcc::LearnedArgs a{};
+    a.obs = scene->d_obs;
+    a.meta = scene->d_meta;
+    a.edges = d_edges;
+    a.tiles = bcc::LearnTiles{nullptr, (int)tiles, d_counter, chunks, num_edges, tile};
+    a.resolution = resolution;
+    a.variant = variant;
+    a.grid = (int)std::min<long long>(slots, (tiles + bcc::kBlock / 64 - 1) / (bcc::kBlock / 64));
+    a.mlp = m;
+    a.out = bcc::LearnedOut{bcc::PairSlots{scene->num_envs, 1}, d_results, d_stage, d_words, d_stats};
+    int rc = BCC_OK;
+    if (hipError_t e = hipGetLastError(); e != hipSuccess)
+        rc = hip_fail(e, "learned tile counter reset");
+    if (rc == BCC_OK)
+        rc = dispatch_learned(robot, a, s);
+    (void)hipFreeAsync(d_counter, s);
+    return rc;
+}
+
+int learned_args(int robot, const bcc_model* model, const bcc_scene* scene, const void* d_edges, int num_edges,
+                 int resolution, const void* d_results)
+{
+    if (int rc = check_dense_args(robot, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    if (model && bcc_detail::model_robot(model) != robot)
+        return fail(BCC_ERR_INVALID, "the model was created for another robot");
+    return BCC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bcc_check_dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const float* d_edges,
+                            int num_edges, int resolution, uint8_t* d_results, uint8_t* d_stage, uint32_t* d_words,
+                            int32_t* d_stats, void* stream)
+{
+    if (int rc = learned_args(robot, model, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    return dense_learned(robot, model, scene, d_edges, num_edges, resolution, d_results, d_stage, d_words, d_stats,
+                         (hipStream_t)stream);
+}
+
+int bcc_time_dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const float* d_edges,
+                           int num_edges, int resolution, uint8_t* d_results, int warmup, int iters, void* stream,
+                           double* avg_ms)
+{
+    if (int rc = learned_args(robot, model, scene, d_edges, num_edges, resolution, d_results))
+        return rc;
+    if (!avg_ms || iters < 1 || warmup < 0)
+        return fail(BCC_ERR_INVALID, "bad timing arguments");
+    hipStream_t s = (hipStream_t)stream;
+    auto step = [&]() {
+        return dense_learned(robot, model, scene, d_edges, num_edges, resolution, d_results, nullptr, nullptr,
+                             nullptr, s);
+    };
+    for (int i = 0; i < warmup; ++i)
+        if (int rc = step())
+            return rc;
+    hipEvent_t t0, t1;
+    BCC_HIP(hipEventCreate(&t0), "hipEventCreate");
+    hipError_t e = hipEventCreate(&t1);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    int rc = BCC_OK;
+    e = hipEventRecord(t0, s);
+    for (int i = 0; i < iters && e == hipSuccess && rc == BCC_OK; ++i)
+        rc = step();
+    if (e == hipSuccess)
+        e = hipEventRecord(t1, s);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(t1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms, t0, t1);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc != BCC_OK)
+        return rc;
+    if (e != hipSuccess)
+        return hip_fail(e, "timing events");
+    *avg_ms = (double)ms / iters;
+    return BCC_OK;
+}
+
+int bcc_debug_learned_tile(int pairs)
+{
+    if (pairs < 0 || pairs > bcc::kLearnTile)
+        return fail(BCC_ERR_INVALID, "learned tile must be in [0, 256]");
+    g_learned_tile.store(pairs);
+    return BCC_OK;
+}
+
+int bcc_debug_learned_passes(int passes)
+{
+    if (passes < 0 || passes > bcc::kLearnPassesMax)
+        return fail(BCC_ERR_INVALID, "learned probe passes must be in [0, 3]");
+    g_learned_passes.store(passes == 0 ? bcc::kLearnPassesDefault : passes);
+    return BCC_OK;
+}
+
+int bcc_debug_learned_far(int radius)
+{
+    if (radius < 0 || radius > 32)
+        return fail(BCC_ERR_INVALID, "radius must be in [0, 32]");
+    g_learned_far.store(radius);
+    return BCC_OK;
+}
+
+}  // extern "C"
+
 namespace bcc_detail {
 
 const bcc_robot_model* host_model(int robot)
@@ -978,8 +1170,10 @@ int bcc_audit_clearance(int robot, const bcc_scene* scene, const float* d_edges,
 // kinds:
 //   plain    the ragged kernel over every tile;
 //   ordered  a learned order given for every pair: learn_finish_kernel over
-//            segments that are the tiles themselves (identity queue).
-enum PlanKind { kPlanPlain = 0, kPlanOrdered = 1 };
+//            segments that are the tiles themselves (identity queue);
+//   learned  the fused learned kernel, persistent over the tiles, with a
+//            snapshot of the model taken when the plan is made.
+enum PlanKind { kPlanPlain = 0, kPlanOrdered = 1, kPlanLearned = 2 };
 
 struct bcc_ragged_plan {
     int robot = 0;
@@ -999,6 +1193,9 @@ struct bcc_ragged_plan {
     int preset_segs = 0;              // segments in the queue: every tile
     int* d_queue_idx = nullptr;       // [num_pairs] identity
     bcc::QueueSeg* d_segs = nullptr;  // [num_tiles]
+    // learned plans
+    bcc::LearnedArgs fused{};
+    void* d_mlp = nullptr;            // the model snapshot (null: no model)
 };
 
 struct bcc_ragged_group {
@@ -1073,6 +1270,12 @@ int run_plan(const bcc_ragged_plan* p, hipStream_t s)
 {
     if (p->kind == kPlanPlain)
         return dispatch_ragged(p->robot, p->args, s);
+    if (p->kind == kPlanLearned) {
+        hipLaunchKernelGGL(bcc::reset_counter_kernel, dim3(1), dim3(64), 0, s, p->d_counter);
+        if (hipError_t e = hipGetLastError(); e != hipSuccess)
+            return hip_fail(e, "plan counter reset");
+        return dispatch_learned(p->robot, p->fused, s);
+    }
     hipLaunchKernelGGL(bcc::reset_pipeline_kernel, dim3(1), dim3(64), 0, s, p->d_counter, p->preset_segs);
     if (hipError_t e = hipGetLastError(); e != hipSuccess)
         return hip_fail(e, "plan counter reset");
@@ -1098,6 +1301,7 @@ void bcc_ragged_plan_destroy_impl(bcc_ragged_plan* p)
     (void)hipFree(p->d_counter);
     (void)hipFree(p->d_queue_idx);
     (void)hipFree(p->d_segs);
+    (void)hipFree(p->d_mlp);
     delete p;
 }
 
@@ -1156,15 +1360,18 @@ int capture_plan(bcc_ragged_plan* p, bcc_ragged_plan** out)
 
 // Plan of either kind over a batch.  d_order: ordered plans, one word per
 // pair (index i).
+// Learned plans: mlp holds the model's operands in the allocation mlp_owned,
+// which the plan takes over (also on failure).
 int create_plan_kind(int robot, const bcc_scene* scene, PlanBatch& b, const float* d_edges, int resolution,
                      int kind, const uint32_t* d_order, uint8_t* d_results, int tile_pairs, int groups_per_cu,
-                     bcc_ragged_plan** out)
+                     bcc_ragged_plan** out, const bcc::MlpArgs* mlp = nullptr, void* mlp_owned = nullptr)
 {
     const int variant = choose_variant(robot, b.mean_obstacles);
     const bcc_tune_entry& tune = tuned(robot);
     if (groups_per_cu == 0)
         groups_per_cu = tune.groups_per_cu;
     bcc_ragged_plan* p = new bcc_ragged_plan;
+    p->d_mlp = mlp_owned;
     p->robot = robot;
     p->kind = kind;
     p->scene = scene;
@@ -1179,7 +1386,7 @@ int create_plan_kind(int robot, const bcc_scene* scene, PlanBatch& b, const floa
         err = hipMalloc((void**)&p->d_counter, sizeof(int) * bcc::kCtrInts);
     if (err == hipSuccess && !b.tiles.empty())
         err = hipMemcpy(p->d_tiles, b.tiles.data(), sizeof(bcc::RaggedTile) * b.tiles.size(), hipMemcpyHostToDevice);
-    if (err == hipSuccess && kind != kPlanPlain) {
+    if (err == hipSuccess && kind == kPlanOrdered) {
         err = hipMalloc((void**)&p->d_queue_idx, sizeof(int) * n);
         if (err == hipSuccess)
             err = hipMalloc((void**)&p->d_segs, sizeof(bcc::QueueSeg) * nt);
@@ -1208,6 +1415,19 @@ int create_plan_kind(int robot, const bcc_scene* scene, PlanBatch& b, const floa
         p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
                                   resolution, grid, d_results};
         p->args.variant = variant;
+    } else if (kind == kPlanLearned) {
+        bcc::LearnedArgs& F = p->fused;
+        F.obs = scene->d_obs;
+        F.meta = scene->d_meta;
+        F.edges = d_edges;
+        F.tiles = bcc::LearnTiles{p->d_tiles, p->num_tiles, p->d_counter, 0, 0, 0};
+        F.resolution = resolution;
+        F.grid = grid;
+        F.variant = variant;
+        F.mlp = mlp ? *mlp : bcc::MlpArgs{};
+        F.mlp.far0 = learned_far_mask(g_learned_far.load());
+        F.mlp.passes = g_learned_passes.load();
+        F.out = bcc::LearnedOut{b.slots, d_results, nullptr, nullptr, nullptr};
     } else {
         bcc::LearnArgs& L = p->learn;
         L.obs = scene->d_obs;
@@ -1315,6 +1535,31 @@ int bcc_ragged_plan_create_ordered(int robot, const bcc_scene* scene, const int6
 {
     return create_plan(robot, scene, env_offsets, d_edges, resolution, true, d_order, d_results, tile_pairs,
                        groups_per_cu, out);
+}
+
+int bcc_ragged_plan_create_learned(int robot, const bcc_model* model, const bcc_scene* scene,
+                                   const int64_t* env_offsets, const float* d_edges, int resolution,
+                                   uint8_t* d_results, int tile_pairs, int groups_per_cu, bcc_ragged_plan** out)
+{
+    if (!out)
+        return fail(BCC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (int rc = learned_args(robot, model, scene, d_edges, 0, resolution, d_results))
+        return rc;
+    PlanBatch b;
+    // the fused learned kernel queues a tile's undecided pairs in LDS
+    if (int rc = ragged_batch(scene, env_offsets, robot, tile_pairs, groups_per_cu, bcc::kLearnTile, b, &tile_pairs))
+        return rc;
+    if (b.total > 0 && (!d_edges || !d_results))
+        return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
+    bcc::MlpArgs m{};
+    void* owned = nullptr;
+    if (model) {
+        if (int rc = bcc_detail::model_mlp_snapshot(model, scene, &m, &owned))
+            return rc;
+    }
+    return create_plan_kind(robot, scene, b, d_edges, resolution, kPlanLearned, nullptr, d_results, tile_pairs,
+                            groups_per_cu, out, &m, owned);
 }
 
 int bcc_ragged_plan_launch(bcc_ragged_plan* plan, void* stream)

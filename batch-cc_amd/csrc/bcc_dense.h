@@ -11,6 +11,7 @@
 #include <algorithm>
 
 #include "bcc_device.h"
+#include "bcc_mlp.h"
 
 namespace bcc {
 
@@ -253,17 +254,25 @@ __device__ __forceinline__ int probe_bin(uint32_t top, int k)
     return top == 0xffffffffu ? (32 * k + 16) / K : (int)((top >> (8 * k)) & 31u);
 }
 
+// The step-0 sample of bin b of a pair, start + delta * (b n) (the plain
+// step-0 sample of lane b, same arithmetic), into Q.
+template <int D>
+__device__ __forceinline__ void probe_at(const float* __restrict__ se, int resolution, int bin, const LdsRow& Q)
+{
+    float s[D], d[D];
+    const int n = setup_pair_sd<D>(se, resolution, s, d);
+    const float bn = (float)(bin * n);
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        Q.at(j) = fmaf(d[j], bn, s[j]);
+}
+
 // Lane k of a probe group: step 0 of the pair's k-th probed bin into Q.
 template <int D, int K>
 __device__ __forceinline__ void probe_sample(const float* __restrict__ se, int resolution, uint32_t top, int k,
                                              const LdsRow& Q)
 {
-    float s[D], d[D];
-    const int n = setup_pair_sd<D>(se, resolution, s, d);
-    const float bn = (float)(probe_bin<K>(top, k) * n);
-#pragma unroll
-    for (int j = 0; j < D; ++j)
-        Q.at(j) = fmaf(d[j], bn, s[j]);
+    probe_at<D>(se, resolution, probe_bin<K>(top, k), Q);
 }
 
 // Plain step loop of one wave over a queue of pairs (the pairs a probe left
@@ -698,6 +707,280 @@ hipError_t launch_learn(const LearnArgs& a, hipStream_t stream)
 {
     return a.variant ? launch_learn_finish_variant<Robot, HW, HU>(a, stream)
                      : launch_learn_finish_variant<Robot, LW, LU>(a, stream);
+}
+
+// ------------------------------------------------------------ fused learned step
+// The whole learned step of a batch in one kernel: inference runs inside the
+// checking kernel, and only for the pairs that need it.  Each wave owns a
+// tile (one environment, up to kLearnTile pairs; persistent waves take tiles
+// from a counter) and runs four wave-local phases, with no workgroup
+// barriers, so a wave that ends early takes its next tile at once:
+//   1. probe: every pair's step-0 samples at K evenly spaced bins,
+//      (32 k + 16) / K, 64 / K pairs per batch; a hit decides the pair, the
+//      others are queued in LDS;
+//   2. inference, for the queued pairs only: the MLP on the matrix cores, 16
+//      pairs per MFMA column tile, giving each pair its order word (the
+//      spread top-4 among the bins not yet probed) in LDS;
+//   3. probes of the queued pairs at the word's first K bins (and, with
+//      three passes, then at its next K); without a model: fixed bins between
+//      the ones probed before; each pass compacts the queue in place;
+//   4. the plain step loop, a half-wave per pair, over the pairs still
+//      undecided.
+// The probes only visit some of the reference's samples first; every verdict
+// is still the OR over the whole sample set, so it equals plain order's for
+// any model.  While one wave runs its MFMA phase the other waves on its SIMD
+// run check code.
+constexpr int kLearnTile = 256;    // pairs per tile
+constexpr int kLearnPassesMax = 3;  // probe passes: the first, then up to 4 / K predicted ones
+#ifndef BCC_LEARN_PASSES
+#define BCC_LEARN_PASSES 2
+#endif
+constexpr int kLearnPassesDefault = BCC_LEARN_PASSES;
+
+// Fixed bins of probe pass p (the first pass always; the later ones without a
+// model): (32 k + offset) / K, k = 0..K-1 -- for K = 2: 8 and 24, 0 and 16,
+// 4 and 20.
+__host__ __device__ __forceinline__ constexpr int fixed_offset(int pass)
+{
+    return pass == 0 ? 16 : pass == 1 ? 0 : 8;
+}
+
+// Model operands of the fused learned kernel.
+struct MlpArgs {
+    const bf16x8* frag;   // packed weight fragments (kFrags x 64 lanes); null: no model
+    const float* b1;
+    const float* b2;
+    const float* b3;
+    const float* enc;     // [num_envs][kEnc] environment codes
+    const float* lo;      // joint lower limits [dim]
+    const float* scale;   // 2 / joint range [dim]
+    uint32_t far0;        // bins that count as far from the first probe's bins
+    int passes;           // probe passes, 1 .. kLearnPassesMax
+};
+
+// Where tiles come from: an explicit list (ragged batches), or dense tiles,
+// tile t = environment t / dense_chunks and edges [(t % dense_chunks) *
+// dense_tile, + dense_tile): waves that take neighbouring tiles share an
+// environment, so its obstacle records stay in the scalar cache.
+struct LearnTiles {
+    const RaggedTile* list;
+    int num_tiles;
+    int* counter;   // next-tile counter, zeroed before the launch
+    int dense_chunks, dense_edges, dense_tile;
+};
+
+// Outputs of the fused learned kernel; pair index i is the edge row, its
+// verdict lives at out[slots.slot(i, env)] (stage and words likewise).
+struct LearnedOut {
+    PairSlots slots;
+    uint8_t* out;
+    uint8_t* stage;    // optional: p = decided by probe pass p, else passes - 1 + plain steps run (<= 255)
+    uint32_t* words;   // optional: the order word inferred for each pair that reached phase 2
+    int* stats;        // optional [5]: pairs decided by probe passes 0, 1, 2, inferred, sent to the step loop
+};
+
+struct LearnedPairs {
+    LearnedOut o;
+    int env;
+    int passes;
+    __device__ __forceinline__ size_t edge(int i) const { return (size_t)i; }
+    __device__ __forceinline__ void finish(int i, bool h, int steps) const
+    {
+        const size_t sl = o.slots.slot(i, env);
+        o.out[sl] = h ? 1 : 0;
+        if (o.stage)
+            o.stage[sl] = (uint8_t)min(passes - 1 + steps, 255);
+    }
+};
+
+// Order word of the pair in column l % 16 of a 16-pair tile (ok: the
+// column holds a pair): operands, logits, and the spread top picks among the
+// bins not in `probed`.  Not inlined: its registers (fragment offsets,
+// activations) are allocated apart from the check program's.
+template <int D>
+__device__ __attribute__((noinline)) uint32_t infer_word(const MlpArgs& mlp, const float* __restrict__ code,
+                                                         const float* __restrict__ edges, int i, bool ok,
+                                                         uint32_t probed)
+{
+    bf16x8 xb[kKT1];
+    mlp_pair_operands(code, edges, D, mlp.lo, mlp.scale, i, ok, xb);
+    float z[8];
+    mlp_tile_logits(mlp.frag, mlp.b1, mlp.b2, mlp.b3, xb, z);
+    return spread_top4_lanes_from(z, (threadIdx.x & 63) >> 4, probed, mlp.far0);
+}
+
+template <class Robot, int K, int kWaves, int kUnroll>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void learned_kernel(
+    const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges, LearnTiles tl,
+    int resolution, MlpArgs mlp, LearnedOut res)
+{
+    constexpr int D = Robot::dim;
+    constexpr int kBlockWaves = kBlock / 64;
+    constexpr int kPairs = 64 / K;
+    static_assert(K * (kLearnPassesMax - 1) <= 4, "the predicted passes use the order word's four bins");
+    __shared__ int next_queued[kBlockWaves];
+    __shared__ int queue[kBlockWaves][kLearnTile];
+    __shared__ uint32_t word[kBlockWaves][kLearnTile];
+    __shared__ float qs[D][kBlock];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int grp = lane / K;
+    const int k = lane - grp * K;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const LdsRow Q{&qs[0][threadIdx.x]};
+    const int passes = min(max(mlp.passes, 1), kLearnPassesMax);
+    uint32_t probed = 0;   // the first probe's bins
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk)
+        probed |= 1u << ((32 * kk + fixed_offset(0)) / K);
+
+    for (;;) {
+        // -- take a tile (wave-uniform)
+        int t = 0;
+        if (lane == 0)
+            t = atomicAdd(tl.counter, 1);
+        const int tile = __builtin_amdgcn_readfirstlane(__shfl(t, 0));
+        if (tile >= tl.num_tiles)
+            return;
+        int env, begin, end;
+        if (tl.list) {
+            const RaggedTile T = tl.list[tile];
+            env = T.env;
+            begin = T.begin;
+            end = T.end;
+        } else {
+            env = tile / tl.dense_chunks;
+            begin = (tile - env * tl.dense_chunks) * tl.dense_tile;
+            end = min(tl.dense_edges, begin + tl.dense_tile);
+        }
+        env = __builtin_amdgcn_readfirstlane(env);
+        begin = __builtin_amdgcn_readfirstlane(begin);
+        end = __builtin_amdgcn_readfirstlane(end);
+        const EnvView E = load_env(obs, meta, env);
+        int nq = 0;   // queued pairs (wave-uniform)
+        int hits[kLearnPassesMax] = {0, 0, 0}, inferred = 0;
+#pragma unroll 1
+        for (int pass = 0; pass < passes; ++pass) {
+            const int cnt = pass == 0 ? end - begin : nq;
+            if (pass > 0 && cnt == 0)
+                break;
+            if (pass == 1 && mlp.frag) {
+                // -- phase 2: inference over the queue, 16 pairs per column tile
+                const float* code = mlp.enc + (size_t)env * kEnc;
+                for (int c0 = 0; c0 < cnt; c0 += 16) {
+                    const int j = c0 + (lane & 15);
+                    const bool ok = j < cnt;
+                    const int i = ok ? queue[wave][j] : 0;
+                    const uint32_t w = infer_word<D>(mlp, code, edges, i, ok, probed);
+                    if (ok && lane < 16) {
+                        word[wave][j] = w;
+                        if (res.words)
+                            res.words[res.slots.slot(i, env)] = w;
+                    }
+                }
+                inferred = cnt;
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            }
+            // -- phases 1 and 3: probe K bins of each pair, queue the misses
+            // (later passes compact in place: the entries a batch writes lie
+            // below the end of the batch, which it has already read)
+            nq = 0;
+            for (int j0 = 0; j0 < cnt; j0 += kPairs) {
+                const int j = j0 + grp;
+                const bool valid = j < cnt;
+                int i = 0;
+                uint32_t w = 0;   // the pair's order word (later passes, with a model)
+                if (valid) {
+                    int bin;
+                    if (pass == 0) {
+                        i = begin + j;
+                        bin = (32 * k + fixed_offset(0)) / K;
+                    } else {
+                        i = queue[wave][j];
+                        if (mlp.frag) {
+                            w = word[wave][j];
+                            bin = (int)((w >> (8 * (K * (pass - 1) + k))) & 31u);
+                        } else {
+                            bin = (32 * k + fixed_offset(pass)) / K;
+                        }
+                    }
+                    probe_at<D>(edges + (size_t)i * 2 * D, resolution, bin, Q);
+                }
+                const uint64_t live = __ballot(valid);
+                const uint64_t hit = Robot::template check<kUnroll, K>(Q, E, live);
+                const bool h = (hit >> lane) & 1ull;
+                const bool lead = valid && k == 0;
+                const uint64_t miss = __ballot(lead && !h);
+                if (lead) {
+                    if (h) {
+                        const size_t sl = res.slots.slot(i, env);
+                        res.out[sl] = 1;
+                        if (res.stage)
+                            res.stage[sl] = (uint8_t)pass;
+                    } else {
+                        // the word moves with its pair
+                        const int pos = nq + __popcll(miss & below);
+                        queue[wave][pos] = i;
+                        word[wave][pos] = w;
+                    }
+                }
+                const int decided = __popcll(__ballot(lead && h));
+                if (pass == 0)
+                    hits[0] += decided;
+                else if (pass == 1)
+                    hits[1] += decided;
+                else
+                    hits[2] += decided;
+                nq += __popcll(miss);
+            }
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        }
+        // -- phase 4: plain steps over the pairs still undecided
+        if (lane == 0)
+            next_queued[wave] = 0;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        plain_queue<Robot, kUnroll>(E, edges, resolution, queue[wave], nq, &next_queued[wave], Q,
+                                    LearnedPairs{res, env, passes});
+        if (res.stats && lane == 0) {
+            atomicAdd(res.stats + 0, hits[0]);
+            atomicAdd(res.stats + 1, hits[1]);
+            atomicAdd(res.stats + 2, hits[2]);
+            atomicAdd(res.stats + 3, inferred);
+            atomicAdd(res.stats + 4, nq);
+        }
+    }
+}
+
+struct LearnedArgs {
+    const float* obs;
+    const int* meta;
+    const float* edges;
+    LearnTiles tiles;
+    int resolution;
+    int grid;      // persistent workgroups
+    int variant;   // 1: high-occupancy kernel variant, 0: low
+    MlpArgs mlp;
+    LearnedOut out;
+};
+
+template <class Robot, int W, int U>
+hipError_t launch_learned_variant(const LearnedArgs& a, hipStream_t stream)
+{
+    hipLaunchKernelGGL((learned_kernel<Robot, kProbe1, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta,
+                       a.edges, a.tiles, a.resolution, a.mlp, a.out);
+    return hipGetLastError();
+}
+
+template <class Robot, int HW, int HU, int LW, int LU>
+hipError_t launch_learned(const LearnedArgs& a, hipStream_t stream)
+{
+    if (a.tiles.num_tiles == 0)
+        return hipSuccess;
+    return a.variant ? launch_learned_variant<Robot, HW, HU>(a, stream)
+                     : launch_learned_variant<Robot, LW, LU>(a, stream);
 }
 
 // ------------------------------------------------------------ test hooks
@@ -1202,6 +1485,7 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
     hipError_t launch_work_##name(const WorkArgs& a, hipStream_t stream);                                     \
     hipError_t launch_ragged_##name(const RaggedArgs& a, hipStream_t stream);                                 \
     hipError_t launch_learn_##name(const LearnArgs& a, hipStream_t stream);                                   \
+    hipError_t launch_learned_##name(const LearnedArgs& a, hipStream_t stream);                               \
     hipError_t launch_label_##name(const LabelArgs& a, hipStream_t stream);                                   \
     hipError_t launch_audit_##name(const AuditArgs& a, hipStream_t stream);                                   \
     extern const VariantKnobs knobs_##name;

@@ -29,6 +29,19 @@ int hip_fail(hipError_t e, const char* what);
 const bcc_robot_model* host_model(int robot);
 int device_model(int robot, bcc::ModelDev* out);
 
+// Robot a model was created for.
+int model_robot(const bcc_model* model);
+
+// Inference operands of `model` for the environments of `scene`, in the
+// model's own buffers: packs the weight fragments and encodes the scene on
+// stream s (allocating the code buffer, after a synchronisation, only when
+// the scene has more environments than any before).
+int model_mlp_args(bcc_model* model, const bcc_scene* scene, hipStream_t s, bcc::MlpArgs* out);
+
+// The same operands as one device allocation *owned (hipFree it), filled and
+// synchronised: a snapshot of the model as it is now, for plans.
+int model_mlp_snapshot(const bcc_model* model, const bcc_scene* scene, bcc::MlpArgs* out, void** owned);
+
 }  // namespace bcc_detail
 
 #define BCC_HIP(call, what)                                  \

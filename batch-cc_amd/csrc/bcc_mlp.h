@@ -76,6 +76,218 @@ __device__ __forceinline__ bf16x8 load_slots(const uint16_t* row, int g)
     return v;
 }
 
+// ------------------------------------------------------------ model shape
+constexpr int kEnc = 64;      // occupancy cells (4 x 4 x 4)
+constexpr int kIn = 96;       // padded input width: 64 + 2 * dim (dim <= 16)
+constexpr int kHid = 256;
+constexpr int kOut = 32;
+
+// Weight fragments of the fused inference kernels: fragment (rt, kt) of a
+// layer with W [K][N] holds for lane l the eight values
+// W[32 kt + kslot(l/16, i)][16 rt + l % 16], i = 0..7 (one MFMA A operand).
+constexpr int kKT1 = kIn / 32;                   // k blocks of layer 1
+constexpr int kKT2 = kHid / 32;                  // k blocks of layers 2 and 3
+constexpr int kRT12 = kHid / 16;                 // row tiles of layers 1 and 2
+constexpr int kRT3 = kOut / 16;                  // row tiles of layer 3
+constexpr int kFragL1 = 0;
+constexpr int kFragL2 = kFragL1 + kRT12 * kKT1;
+constexpr int kFragL3 = kFragL2 + kRT12 * kKT2;
+constexpr int kFrags = kFragL3 + kRT3 * kKT2;
+
+// Joint coordinate idx of an edge ([2][dim]: start then end) normalised to
+// [-1, 1] over the joint's limits: (q - lo) * scale - 1, scale = 2 / range.
+__device__ __forceinline__ float edge_feature(const float* __restrict__ edges, int dim, const float* __restrict__ lo,
+                                              const float* __restrict__ scale, long long edge, int idx)
+{
+    const int j = idx >= dim ? idx - dim : idx;
+    const float q = edges[(size_t)edge * 2 * dim + idx];
+    return (q - lo[j]) * scale[j] - 1.0f;
+}
+
+// The order word from a pair's 32 bin logits: four distinct bins, one byte
+// each.  The first is the highest logit; each later one the highest logit
+// among the bins at least kOrderSpread bins from every bin already chosen
+// (any unchosen bin when none is that far), ties to the lower bin.  The
+// probe checks the first bins' step-0 samples: neighbouring bins tend to
+// collide together, so spreading the picks along the edge makes a miss of
+// one less likely to repeat in the next.
+#ifndef BCC_ORDER_SPREAD
+#define BCC_ORDER_SPREAD 8
+#endif
+constexpr int kOrderSpread = BCC_ORDER_SPREAD;
+
+// The selection from the layer-3 result registers.  Lane l holds the logits
+// z[4 r3 + v] of bins 16 r3 + 4 g + v (r3 = 0, 1; v = 0..3; g = l / 16) of
+// one pair; the four lanes l % 16 + 16 g' hold its 32 bins.  Each round every
+// lane takes its best candidate, then the lanes 16 and 32 apart exchange
+// theirs; all four lanes end with the same word.  Bins in taken0 are never
+// picked (bins a probe has already visited) and the bins outside far0 count
+// as near a chosen bin from the start.
+__device__ __forceinline__ uint32_t spread_top4_lanes_from(const float (&z)[8], int g, uint32_t taken0,
+                                                           uint32_t far0)
+{
+    uint32_t taken = taken0, far_from = far0, packed = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t cand = (far_from & ~taken) ? (far_from & ~taken) : ~taken;
+        float bv = 0.0f;
+        int bb = 64;   // none yet
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int bin = 16 * (e >> 2) + 4 * g + (e & 3);
+            if (((cand >> bin) & 1u) && (bb == 64 || z[e] > bv || (z[e] == bv && bin < bb))) {
+                bv = z[e];
+                bb = bin;
+            }
+        }
+#pragma unroll
+        for (int off = 16; off <= 32; off <<= 1) {
+            const float ov = __shfl_xor(bv, off);
+            const int ob = __shfl_xor(bb, off);
+            if (ob != 64 && (bb == 64 || ov > bv || (ov == bv && ob < bb))) {
+                bv = ov;
+                bb = ob;
+            }
+        }
+        taken |= 1u << bb;
+        packed |= (uint32_t)bb << (8 * k);
+        if constexpr (kOrderSpread > 0) {
+            // bins within kOrderSpread - 1 of the pick are no longer "far"
+            const int lo = max(bb - (kOrderSpread - 1), 0);
+            const int hi = min(bb + (kOrderSpread - 1), 31);
+            const uint32_t width = (uint32_t)(hi - lo + 1);
+            far_from &= ~((width >= 32 ? 0xffffffffu : ((1u << width) - 1u)) << lo);
+        }
+    }
+    return packed;
+}
+
+__device__ __forceinline__ uint32_t spread_top4_lanes(const float (&z)[8], int g)
+{
+    return spread_top4_lanes_from(z, g, 0u, 0xffffffffu);
+}
+
+// Layer-1 B operands (X^T, three 32-wide k blocks) of the pair in column
+// l % 16 of a 16-pair tile: slot i of k block kt is feature 32 kt + kslot(g,
+// i); k blocks 0 and 1 are the environment code (slots 0-3 and 4-7 are four
+// consecutive cells each, one 16-byte load), k block 2 the normalised edge
+// coordinates and zero padding.  ok = false gives a zero column.
+__device__ __forceinline__ void mlp_pair_operands(const float* __restrict__ code, const float* __restrict__ edges,
+                                                  int dim, const float* __restrict__ lo,
+                                                  const float* __restrict__ scale, long long edge, bool ok,
+                                                  bf16x8 (&xb)[kKT1])
+{
+    static_assert(kEnc == 64 && kKT1 == 3, "feature layout");
+    const int g = (threadIdx.x & 63) >> 4;
+    const int kq = 4 * g;
+    const float4* e4 = (const float4*)code;
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) {
+        const float4 lo4 = ok ? e4[(32 * kt + kq) >> 2] : float4{0.f, 0.f, 0.f, 0.f};
+        const float4 hi4 = ok ? e4[(32 * kt + 16 + kq) >> 2] : float4{0.f, 0.f, 0.f, 0.f};
+        const float v[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+        xb[kt] = pack8(v);
+    }
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int idx = kslot(g, i);   // coordinate idx of the edge's [2][dim]
+        v[i] = (ok && idx < 2 * dim) ? edge_feature(edges, dim, lo, scale, edge, idx) : 0.0f;
+    }
+    xb[2] = pack8(v);
+}
+
+// Raw buffer resource over `bytes` bytes at p.  Loads through it take a
+// per-lane VGPR offset plus a wave-uniform SGPR or immediate offset, so a run
+// of loads at constant distances from one base costs one VGPR of address
+// (global loads keep a 64-bit VGPR address live per distinct offset).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ u32x4 load16(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff)
+{
+    return __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+}
+
+// Logits of one 16-pair column tile, computed transposed (Y^T = W^T X^T) as
+// in the fused inference kernel: the same MFMAs in the same k order, the
+// same bias, ReLU and BF16 roundings, so the logits are bit-identical to it
+// and to the layer-by-layer path.  Lane l ends with z[4 r3 + v] = logit of
+// bin 16 r3 + 4 (l/16) + v of the pair in column l % 16.  Weight fragments
+// are read straight from memory (L2-resident, 192 KiB) through a buffer
+// resource; activations stay in registers: layer 1's 256 x 16 BF16 outputs
+// are 32 VGPRs, layer 2 is produced two row tiles at a time and consumed at
+// once by layer 3.  Every lane of the wave must be active (MFMA).
+__device__ __forceinline__ void mlp_tile_logits(const bf16x8* __restrict__ frag, const float* __restrict__ b1,
+                                                const float* __restrict__ b2, const float* __restrict__ b3,
+                                                const bf16x8 (&xb)[kKT1], float (&z)[8])
+{
+    const int lane = threadIdx.x & 63;
+    const __amdgpu_buffer_rsrc_t rf = raw_rsrc(frag, kFrags * 64 * 16);
+    const __amdgpu_buffer_rsrc_t rb1 = raw_rsrc(b1, kHid * 4);
+    const __amdgpu_buffer_rsrc_t rb2 = raw_rsrc(b2, kHid * 4);
+    const __amdgpu_buffer_rsrc_t rb3 = raw_rsrc(b3, kOut * 4);
+    const uint32_t vf = (uint32_t)lane * 16;          // lane's 16 bytes of a fragment
+    const uint32_t vb = (uint32_t)(lane >> 4) * 16;   // bias elements 4 (l/16) .. +3 of a row tile
+    auto A = [&](int f) { return __builtin_bit_cast(bf16x8, load16(rf, vf, f * 64 * 16)); };
+    u32x4 y1[kKT2];
+#pragma unroll
+    for (int rt = 0; rt < kRT12; ++rt) {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < kKT1; ++kt)
+            acc = mfma32(A(kFragL1 + rt * kKT1 + kt), xb[kt], acc);
+        const f32x4 bb = __builtin_bit_cast(f32x4, load16(rb1, vb, 64 * rt));
+        float h[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            h[v] = fmaxf(acc[v] + bb[v], 0.0f);
+        y1[rt >> 1][2 * (rt & 1) + 0] = f2bf2(h[0], h[1]);
+        y1[rt >> 1][2 * (rt & 1) + 1] = f2bf2(h[2], h[3]);
+        // loads of later row tiles stay below this point: the kernels this
+        // runs in are sized for their check program's registers
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    f32x4 acc3[kRT3];
+#pragma unroll
+    for (int r3 = 0; r3 < kRT3; ++r3)
+        acc3[r3] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+    for (int j = 0; j < kKT2; ++j) {
+        u32x4 y2;
+#pragma unroll
+        for (int h2 = 0; h2 < 2; ++h2) {
+            f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < kKT2; ++kt) {
+                acc = mfma32(A(kFragL2 + (2 * j + h2) * kKT2 + kt), __builtin_bit_cast(bf16x8, y1[kt]), acc);
+                if (kt % 4 == 3)
+                    __builtin_amdgcn_sched_barrier(0);   // at most four fragments in flight
+            }
+            const f32x4 bb = __builtin_bit_cast(f32x4, load16(rb2, vb, 64 * (2 * j + h2)));
+            float h[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                h[v] = fmaxf(acc[v] + bb[v], 0.0f);
+            y2[2 * h2 + 0] = f2bf2(h[0], h[1]);
+            y2[2 * h2 + 1] = f2bf2(h[2], h[3]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int r3 = 0; r3 < kRT3; ++r3)
+            acc3[r3] = mfma32(A(kFragL3 + r3 * kKT2 + j), __builtin_bit_cast(bf16x8, y2), acc3[r3]);
+    }
+#pragma unroll
+    for (int r3 = 0; r3 < kRT3; ++r3) {
+        const f32x4 bb = __builtin_bit_cast(f32x4, load16(rb3, vb, 64 * r3));
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            z[4 * r3 + v] = acc3[r3][v] + bb[v];
+    }
+}
+
 // Generic GEMM: C[M][N] = alpha * op(A)[M][K] * op(B)[K][N] + beta * C
 //   op(A)[m][k] = TA ? A[k*lda + m] : A[m*lda + k]
 //   op(B)[k][n] = TB ? B[n*ldb + k] : B[k*ldb + n]
@@ -166,7 +378,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(const uint16_t* __restri
 }
 
 // Sum of split-K partial tiles in split order, then C = alpha * sum + beta * C.
-__global__ void splitk_reduce_kernel(const float* __restrict__ part, int splits, long long cstride, int M, int N,
+static __global__ void splitk_reduce_kernel(const float* __restrict__ part, int splits, long long cstride, int M, int N,
                                      float* __restrict__ C, int ldc, float alpha, float beta)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -19,11 +19,6 @@
 
 namespace bcc {
 
-constexpr int kEnc = 64;      // occupancy cells (4 x 4 x 4)
-constexpr int kIn = 96;       // padded input width: 64 + 2 * dim (dim <= 16)
-constexpr int kHid = 256;
-constexpr int kOut = 32;
-
 // ------------------------------------------------------------ encoder
 // 64 cells of the box [-1,1] x [-1,1] x [0,1.5]; every primitive contributes
 // the fraction of each cell its axis-aligned bounding box covers; the
@@ -90,16 +85,6 @@ __global__ void encode_env_kernel(const float* __restrict__ obs, const int* __re
 // as BF16, rows for pairs p0 .. p0 + num_pairs - 1.  Pair p: env =
 // pair_env ? pair_env[p] : p % E, edge = pair_edge ? pair_edge[p] : p / E
 // (dense, edge-major).
-// Joint coordinate idx of an edge ([2][dim]: start then end) normalised to
-// [-1, 1] over the joint's limits: (q - lo) * scale - 1, scale = 2 / range.
-__device__ __forceinline__ float edge_feature(const float* __restrict__ edges, int dim, const float* __restrict__ lo,
-                                              const float* __restrict__ scale, long long edge, int idx)
-{
-    const int j = idx >= dim ? idx - dim : idx;
-    const float q = edges[(size_t)edge * 2 * dim + idx];
-    return (q - lo[j]) * scale[j] - 1.0f;
-}
-
 __global__ void features_kernel(const float* __restrict__ enc, const float* __restrict__ edges, int dim,
                                 const float* __restrict__ lo, const float* __restrict__ scale,
                                 const int* __restrict__ pair_env, const int* __restrict__ pair_edge, int num_envs,
@@ -229,14 +214,6 @@ __global__ void to_bf16_kernel(const float* __restrict__ a, long long n, uint16_
 // slot and k-block order and the bias/ReLU/rounding steps are those of the
 // layer-by-layer path (gemm_bf16_kernel + bias_act_kernel), so logits and
 // orders are identical to it.
-constexpr int kKT1 = kIn / 32;                   // k blocks of layer 1
-constexpr int kKT2 = kHid / 32;                  // k blocks of layers 2 and 3
-constexpr int kRT12 = kHid / 16;                 // row tiles of layers 1 and 2
-constexpr int kRT3 = kOut / 16;                  // row tiles of layer 3
-constexpr int kFragL1 = 0;
-constexpr int kFragL2 = kFragL1 + kRT12 * kKT1;
-constexpr int kFragL3 = kFragL2 + kRT12 * kKT2;
-constexpr int kFrags = kFragL3 + kRT3 * kKT2;
 constexpr int kFuseCols = 3;                     // 16-pair column tiles per wave
 constexpr int kFusePairs = 16 * kFuseCols;       // pairs per wave
 constexpr int kFuseBlockPairs = 4 * kFusePairs;  // pairs per workgroup (four waves)
@@ -294,61 +271,6 @@ __global__ void pack_fragments_kernel(const uint16_t* __restrict__ W1, const uin
     frag[idx] = W[(size_t)(32 * kt + kslot(lane >> 4, i)) * N + 16 * rt + (lane & 15)];
 }
 
-// The order word from a pair's 32 bin logits: four distinct bins, one byte
-// each.  The first is the highest logit; each later one the highest logit
-// among the bins at least kOrderSpread bins from every bin already chosen
-// (any unchosen bin when none is that far), ties to the lower bin.  The
-// probe checks the first bins' step-0 samples: neighbouring bins tend to
-// collide together, so spreading the picks along the edge makes a miss of
-// one less likely to repeat in the next.
-#ifndef BCC_ORDER_SPREAD
-#define BCC_ORDER_SPREAD 8
-#endif
-constexpr int kOrderSpread = BCC_ORDER_SPREAD;
-
-// The selection from the layer-3 result registers.  Lane l holds the logits
-// z[4 r3 + v] of bins 16 r3 + 4 g + v (r3 = 0, 1; v = 0..3; g = l / 16) of
-// one pair; the four lanes l % 16 + 16 g' hold its 32 bins.  Each round every
-// lane takes its best candidate, then the lanes 16 and 32 apart exchange
-// theirs; all four lanes end with the same word.
-__device__ __forceinline__ uint32_t spread_top4_lanes(const float (&z)[8], int g)
-{
-    uint32_t taken = 0, far_from = 0xffffffffu, packed = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t cand = (far_from & ~taken) ? (far_from & ~taken) : ~taken;
-        float bv = 0.0f;
-        int bb = 64;   // none yet
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int bin = 16 * (e >> 2) + 4 * g + (e & 3);
-            if (((cand >> bin) & 1u) && (bb == 64 || z[e] > bv || (z[e] == bv && bin < bb))) {
-                bv = z[e];
-                bb = bin;
-            }
-        }
-#pragma unroll
-        for (int off = 16; off <= 32; off <<= 1) {
-            const float ov = __shfl_xor(bv, off);
-            const int ob = __shfl_xor(bb, off);
-            if (ob != 64 && (bb == 64 || ov > bv || (ov == bv && ob < bb))) {
-                bv = ov;
-                bb = ob;
-            }
-        }
-        taken |= 1u << bb;
-        packed |= (uint32_t)bb << (8 * k);
-        if constexpr (kOrderSpread > 0) {
-            // bins within kOrderSpread - 1 of the pick are no longer "far"
-            const int lo = max(bb - (kOrderSpread - 1), 0);
-            const int hi = min(bb + (kOrderSpread - 1), 31);
-            const uint32_t width = (uint32_t)(hi - lo + 1);
-            far_from &= ~((width >= 32 ? 0xffffffffu : ((1u << width) - 1u)) << lo);
-        }
-    }
-    return packed;
-}
-
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void fused_order_kernel(
     const bf16x8* __restrict__ frag, const float* __restrict__ b1, const float* __restrict__ b2,
     const float* __restrict__ b3, const float* __restrict__ enc, const float* __restrict__ edges, int dim,
@@ -362,11 +284,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int kq = 4 * g;
     const long long base = ((long long)blockIdx.x * 4 + wave) * kFusePairs;
 
-    // layer-1 B operands: X^T blocks (k blocks x column tiles).  Slot i of k
-    // block kt is feature 32 kt + kslot(g, i): k blocks 0 and 1 are the
-    // environment code (slots 0-3 and 4-7 are four consecutive cells each,
-    // one 16-byte load), k block 2 the edge coordinates and zero padding.
-    static_assert(kEnc == 64 && kKT1 == 3, "feature layout");
+    // layer-1 B operands: X^T blocks (k blocks x column tiles)
     bf16x8 xb[kKT1][kFuseCols];
     bool pair_ok[kFuseCols];
 #pragma unroll
@@ -380,21 +298,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             edge = pair_edge ? (long long)pair_edge[p] : (pair_env ? p : p / num_envs);
         }
         pair_ok[ct] = ok;
-        const float4* e4 = (const float4*)(enc + (size_t)env * kEnc);
+        bf16x8 x[kKT1];
+        mlp_pair_operands(enc + (size_t)env * kEnc, edges, dim, lo, scale, edge, ok, x);
 #pragma unroll
-        for (int kt = 0; kt < 2; ++kt) {
-            const float4 lo4 = ok ? e4[(32 * kt + kq) >> 2] : float4{0.f, 0.f, 0.f, 0.f};
-            const float4 hi4 = ok ? e4[(32 * kt + 16 + kq) >> 2] : float4{0.f, 0.f, 0.f, 0.f};
-            const float v[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
-            xb[kt][ct] = pack8(v);
-        }
-        float v[8];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int idx = kslot(g, i);   // coordinate idx of the edge's [2][dim]
-            v[i] = (ok && idx < 2 * dim) ? edge_feature(edges, dim, lo, scale, edge, idx) : 0.0f;
-        }
-        xb[2][ct] = pack8(v);
+        for (int kt = 0; kt < kKT1; ++kt)
+            xb[kt][ct] = x[kt];
     }
 
     // layer 1: Y1^T (256 x 48) kept as BF16 B operands of layer 2: row tiles
@@ -1030,35 +938,100 @@ int bcc_model_logits(bcc_model* M, const uint16_t* d_x, int64_t rows, float* d_l
 
 }  // extern "C"
 
-namespace {
+namespace bcc_detail {
 
-// Fused inference: pack the weight fragments, encode the environments, one
-// kernel for all layers and the top-4 selection, all on stream s (no
-// synchronization).  Pairs: every (edge, env) pair of a dense batch,
-// edge-major (pair_env null), or the list (pair_env[p], pair_edge ?
-// pair_edge[p] : p).  d_logits (optional) receives [pairs][32].  The
-// environment codes live in a model-owned buffer, allocated only when a scene
-// with more environments than before is ordered.
-int order_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges, long long pairs, const int* pair_env,
-                const int* pair_edge, uint32_t* d_order, float* d_logits, hipStream_t s)
+int model_robot(const bcc_model* M)
 {
-    if (pairs == 0 || scene->num_envs == 0)
-        return BCC_OK;
+    return M->robot;
+}
+
+int model_mlp_args(bcc_model* M, const bcc_scene* scene, hipStream_t s, MlpArgs* out)
+{
     if (M->enc_envs < scene->num_envs) {
         BCC_HIP(hipStreamSynchronize(s), "order scratch");
         (void)hipFree(M->enc);
         M->enc = nullptr;
         M->enc_envs = 0;
-        BCC_HIP(hipMalloc((void**)&M->enc, sizeof(float) * kEnc * scene->num_envs), "order scratch");
+        BCC_HIP(hipMalloc((void**)&M->enc, sizeof(float) * kEnc * std::max(scene->num_envs, 1)), "order scratch");
         M->enc_envs = scene->num_envs;
     }
     hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks_for((long long)kFrags * 64 * 8)), dim3(256), 0, s,
                        M->wb + P_W1, M->wb + P_W2, M->wb + P_W3, M->frag);
-    hipLaunchKernelGGL(encode_env_kernel, dim3(scene->num_envs), dim3(64), 0, s, scene->d_ref_obs, scene->d_ref_meta,
-                       scene->num_envs, M->enc);
+    if (scene->num_envs > 0)
+        hipLaunchKernelGGL(encode_env_kernel, dim3(scene->num_envs), dim3(64), 0, s, scene->d_ref_obs,
+                           scene->d_ref_meta, scene->num_envs, M->enc);
+    BCC_HIP(hipGetLastError(), "model operands");
+    *out = MlpArgs{(const bf16x8*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, M->enc, M->lo, M->scale,
+                   0xffffffffu};
+    return BCC_OK;
+}
+
+int model_mlp_snapshot(const bcc_model* M, const bcc_scene* scene, MlpArgs* out, void** owned)
+{
+    // [fragments | b1 | b2 | b3 | lo | scale | codes], each part 256-byte aligned
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_frag = 0;
+    const size_t o_b1 = o_frag + al(sizeof(uint16_t) * kFrags * 64 * 8);
+    const size_t o_b2 = o_b1 + al(sizeof(float) * kHid);
+    const size_t o_b3 = o_b2 + al(sizeof(float) * kHid);
+    const size_t o_lo = o_b3 + al(sizeof(float) * kOut);
+    const size_t o_sc = o_lo + al(sizeof(float) * 16);
+    const size_t o_enc = o_sc + al(sizeof(float) * 16);
+    const size_t bytes = o_enc + al(sizeof(float) * kEnc * std::max(scene->num_envs, 1));
+    char* d = nullptr;
+    BCC_HIP(hipMalloc((void**)&d, bytes), "model snapshot");
+    hipLaunchKernelGGL(pack_fragments_kernel, dim3(blocks_for((long long)kFrags * 64 * 8)), dim3(256), 0, 0,
+                       M->wb + P_W1, M->wb + P_W2, M->wb + P_W3, (uint16_t*)(d + o_frag));
+    hipError_t e = hipGetLastError();
+    const std::pair<size_t, const void*> parts[] = {{o_b1, M->w + P_B1}, {o_b2, M->w + P_B2}};
+    for (const auto& pr : parts)
+        if (e == hipSuccess)
+            e = hipMemcpy(d + pr.first, pr.second, sizeof(float) * kHid, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d + o_b3, M->w + P_B3, sizeof(float) * kOut, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d + o_lo, M->lo, sizeof(float) * 16, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d + o_sc, M->scale, sizeof(float) * 16, hipMemcpyDeviceToDevice);
+    if (e == hipSuccess && scene->num_envs > 0) {
+        hipLaunchKernelGGL(encode_env_kernel, dim3(scene->num_envs), dim3(64), 0, 0, scene->d_ref_obs,
+                           scene->d_ref_meta, scene->num_envs, (float*)(d + o_enc));
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return hip_fail(e, "model snapshot");
+    }
+    *out = MlpArgs{(const bf16x8*)(d + o_frag), (const float*)(d + o_b1), (const float*)(d + o_b2),
+                   (const float*)(d + o_b3), (const float*)(d + o_enc), (const float*)(d + o_lo),
+                   (const float*)(d + o_sc), 0xffffffffu};
+    *owned = d;
+    return BCC_OK;
+}
+
+}  // namespace bcc_detail
+
+namespace {
+
+// Fused inference: the model's operands for the scene (weight fragments,
+// environment codes), then one kernel for all layers and the top-4
+// selection, all on stream s (no synchronization).  Pairs: every (edge, env)
+// pair of a dense batch, edge-major (pair_env null), or the list
+// (pair_env[p], pair_edge ? pair_edge[p] : p).  d_logits (optional) receives
+// [pairs][32].
+int order_fused(bcc_model* M, const bcc_scene* scene, const float* d_edges, long long pairs, const int* pair_env,
+                const int* pair_edge, uint32_t* d_order, float* d_logits, hipStream_t s)
+{
+    if (pairs == 0 || scene->num_envs == 0)
+        return BCC_OK;
+    MlpArgs m;
+    if (int rc = bcc_detail::model_mlp_args(M, scene, s, &m))
+        return rc;
     hipLaunchKernelGGL(fused_order_kernel, dim3((unsigned)((pairs + kFuseBlockPairs - 1) / kFuseBlockPairs)), dim3(256), 0, s,
-                       (const bf16x8*)M->frag, M->w + P_B1, M->w + P_B2, M->w + P_B3, M->enc, d_edges, M->dim, M->lo,
-                       M->scale, scene->num_envs, pair_env, pair_edge, pairs, d_order, d_logits);
+                       m.frag, m.b1, m.b2, m.b3, m.enc, d_edges, M->dim, m.lo, m.scale, scene->num_envs, pair_env,
+                       pair_edge, pairs, d_order, d_logits);
     if (hipError_t e = hipGetLastError(); e != hipSuccess)
         return hip_fail(e, "fused order");
     return BCC_OK;

@@ -15,6 +15,7 @@
 //   2, 3   plain dense kernel, high / low occupancy
 //   4, 5   ragged kernel, high / low occupancy
 //   6, 8   learned-order ragged kernel (predicted probe + plain finish), low / high occupancy
+//   9, 10  fused learned kernel (probe, inference, probe, plain finish), low / high occupancy
 #ifndef BCC_PART
 #define BCC_PART -1
 #endif
@@ -49,6 +50,10 @@ extern template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robo
     const LearnArgs&, hipStream_t);
 extern template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(
     const LearnArgs&, hipStream_t);
+extern template hipError_t launch_learned_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(
+    const LearnedArgs&, hipStream_t);
+extern template hipError_t launch_learned_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(
+    const LearnedArgs&, hipStream_t);
 #endif
 
 #if BCC_PART == -1 || BCC_PART == 0
@@ -70,6 +75,12 @@ hipError_t BCC_CAT2(launch_learn_, BCC_ROBOT)(const LearnArgs& a, hipStream_t st
 {
     using namespace robot_parts;
     return launch_learn<RobotT, HW, HU, LW, LU>(a, stream);
+}
+
+hipError_t BCC_CAT2(launch_learned_, BCC_ROBOT)(const LearnedArgs& a, hipStream_t stream)
+{
+    using namespace robot_parts;
+    return launch_learned<RobotT, HW, HU, LW, LU>(a, stream);
 }
 
 hipError_t BCC_CAT2(launch_fk_, BCC_ROBOT)(const float* qs, int n, float* out, hipStream_t stream)
@@ -130,6 +141,14 @@ template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robot_parts
 #if BCC_PART == 8
 template hipError_t launch_learn_finish_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(
     const LearnArgs&, hipStream_t);
+#endif
+#if BCC_PART == 9
+template hipError_t launch_learned_variant<robot_parts::RobotT, robot_parts::LW, robot_parts::LU>(const LearnedArgs&,
+                                                                                                  hipStream_t);
+#endif
+#if BCC_PART == 10
+template hipError_t launch_learned_variant<robot_parts::RobotT, robot_parts::HW, robot_parts::HU>(const LearnedArgs&,
+                                                                                                  hipStream_t);
 #endif
 
 }  // namespace bcc

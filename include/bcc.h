@@ -323,6 +323,55 @@ int bcc_time_learned_dense(bcc_model *model, const bcc_scene *scene, const float
                            int resolution, uint32_t *d_order, uint8_t *d_results, int warmup, int iters,
                            void *stream, double *step_ms, double *order_ms);
 
+/*
+ * Fused learned step (no reference counterpart; replaces the plain step loop
+ * of src/batch_cc.cu:118-129 as the order in which one pair's samples are
+ * visited).  One kernel does the whole step: each wave owns a tile of pairs
+ * of one environment and
+ *   1. probes every pair's step-0 samples at 2 evenly spaced bins (8, 24);
+ *   2. runs the model (BF16 MFMA) for the pairs the probe left undecided only:
+ *      its order word, the spread top-4 bins among the other 30;
+ *   3. probes the word's first 2 bins (with three probe passes, see
+ *      bcc_debug_learned_passes, then its next 2; model NULL: the fixed bins 0
+ *      and 16, then 4 and 20);
+ *   4. runs the plain step loop over all 32 n samples of the pairs still
+ *      undecided.
+ * Verdicts equal bcc_check_dense's for any model.  The model must have been
+ * created for `robot`.  Optional outputs ([num_edges][num_envs] like the
+ * results): d_stage p = decided by probe p + 1, else passes - 1 + the plain
+ * steps run (capped at 255; passes = 2 by default); d_words the order word (as
+ * bcc_check_dense_ex) inferred for each pair that reached step 2 (others
+ * untouched); d_stats int32[5] += pairs decided by probes 1, 2 and 3,
+ * inferred, sent to the step loop.  Asynchronous on `stream`; allocates the
+ * model's environment codes as bcc_model_order_dense does and a tile
+ * counter stream-ordered.
+ */
+int bcc_check_dense_learned(int robot, bcc_model *model, const bcc_scene *scene, const float *d_edges,
+                            int num_edges, int resolution, uint8_t *d_results, uint8_t *d_stage, uint32_t *d_words,
+                            int32_t *d_stats, void *stream);
+/* Average device time (ms) of `iters` learned steps (operands + fused
+ * kernel) after `warmup`, hipEvents on `stream`. */
+int bcc_time_dense_learned(int robot, bcc_model *model, const bcc_scene *scene, const float *d_edges,
+                           int num_edges, int resolution, uint8_t *d_results, int warmup, int iters, void *stream,
+                           double *avg_ms);
+/* A ragged plan running the fused learned kernel persistently over its
+ * tiles (one environment and at most 256 pairs each, a wave per tile),
+ * captured in a hipGraph like the plain plans.  It snapshots the model when it is made (later training does not
+ * change it; model NULL: fixed second-probe bins).  Verdicts equal the plain
+ * plan's.  No reference counterpart. */
+int bcc_ragged_plan_create_learned(int robot, const bcc_model *model, const bcc_scene *scene,
+                                   const int64_t *env_offsets, const float *d_edges, int resolution,
+                                   uint8_t *d_results, int tile_pairs, int groups_per_cu, bcc_ragged_plan **out);
+/* Tuning hooks (no reference counterpart; verdicts never depend on them):
+ * pairs per tile of dense learned launches (0 = chosen per launch), and the
+ * radius around the first probe's bins inside which the second probe's picks
+ * are taken only when no farther bin is left (0 = no preference), and the
+ * number of probe passes (1 .. 3; 0 = the default, 2).  All are read when a
+ * launch or plan is made. */
+int bcc_debug_learned_tile(int pairs);
+int bcc_debug_learned_far(int radius);
+int bcc_debug_learned_passes(int passes);
+
 /* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
 int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,
                    int m, int n, int k, float alpha, float beta);

@@ -77,17 +77,24 @@ def loss_only(params, x, labels):
     return np.mean(np.sum(np.maximum(Z3, 0) - Z3 * Y + np.log1p(np.exp(-np.abs(Z3))), axis=1))
 
 
-def spread_top4(z, spread=8):
+def spread_top4(z, spread=8, taken0=(), far0=None):
     """Order word of one pair's 32 logits: the highest logit first, then each
     time the highest among the bins at least `spread` bins from every bin
-    already chosen (any unchosen bin when none is), ties to the lower bin."""
-    taken = []
+    already chosen (any unchosen bin when none is), ties to the lower bin.
+    Bins in taken0 are never chosen; bins outside the mask far0 count as
+    near a chosen bin from the start."""
+    taken = list(taken0)
+    far = set(range(32)) if far0 is None else {c for c in range(32) if (far0 >> c) & 1}
+    picks = []
     for _ in range(4):
-        far = [c for c in range(32) if c not in taken and all(abs(c - t) >= spread for t in taken)]
-        cand = far if far else [c for c in range(32) if c not in taken]
+        cand = [c for c in range(32) if c not in taken and c in far]
+        if not cand:
+            cand = [c for c in range(32) if c not in taken]
         best = cand[0]
         for c in cand[1:]:
             if z[c] > z[best]:
                 best = c
         taken.append(best)
-    return taken[0] | taken[1] << 8 | taken[2] << 16 | taken[3] << 24
+        picks.append(best)
+        far -= {c for c in range(32) if abs(c - best) < spread}
+    return picks[0] | picks[1] << 8 | picks[2] << 16 | picks[3] << 24

@@ -165,6 +165,13 @@ def test_tuning_hooks_reject_out_of_range(bcc):
     assert L.bcc_debug_dense_chunk(5000) == 1
     assert L.bcc_debug_dense_chunk(0) == 0
     assert L.bcc_debug_cull_min(-1) == 0
+    assert L.bcc_debug_learned_tile(257) == 1
+    assert b"learned tile" in L.bcc_last_error()
+    assert L.bcc_debug_learned_tile(0) == 0
+    assert L.bcc_debug_learned_far(33) == 1
+    assert L.bcc_debug_learned_far(0) == 0
+    assert L.bcc_debug_learned_passes(4) == 1
+    assert L.bcc_debug_learned_passes(0) == 0
 
 
 def test_tuner_variant_range_selection():

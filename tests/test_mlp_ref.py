@@ -50,3 +50,30 @@ def test_spread_top4_picks_far_bins():
     for _ in range(200):
         w = mlp_ref.spread_top4(rng.standard_normal(32))
         assert len({(w >> (8 * k)) & 0xff for k in range(4)}) == 4
+
+
+def test_spread_top4_from_probed_bins():
+    """The second probe's rule: bins already probed are never picked, and
+    bins near them are picked only when no far bin is left."""
+    z = np.zeros(32)
+    z[8], z[9], z[23], z[12], z[0] = 9.0, 8.5, 8.0, 7.0, 1.0
+    # nothing near the probed bins 8 and 24 is excluded: 9 leads, then 23
+    # would be 14 from 9 (far), so it follows
+    w = mlp_ref.spread_top4(z, taken0=(8, 24))
+    bins = [(w >> (8 * k)) & 0xff for k in range(4)]
+    assert 8 not in bins and 24 not in bins
+    assert bins[:2] == [9, 23]
+    # radius 4 around 8 and 24: 9 and 23 are near, 12 (4 away) is the best far bin,
+    # then 0 (12 from 12, 8 from 8)
+    far0 = 0xffffffff
+    for b in (8, 24):
+        for d in range(-3, 4):
+            far0 &= ~(1 << (b + d))
+    w = mlp_ref.spread_top4(z, taken0=(8, 24), far0=far0)
+    assert [(w >> (8 * k)) & 0xff for k in (0, 1)] == [12, 0]
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        w = mlp_ref.spread_top4(rng.standard_normal(32), taken0=(8, 24), far0=far0)
+        b = [(w >> (8 * k)) & 0xff for k in range(4)]
+        assert len(set(b)) == 4 and 8 not in b and 24 not in b
+
