@@ -131,6 +131,7 @@ def lib():
     _sig(L, "bcc_debug_learned_tile", I, [I])
     _sig(L, "bcc_debug_learned_far", I, [I])
     _sig(L, "bcc_debug_learned_passes", I, [I])
+    _sig(L, "bcc_debug_learned_min_steps", I, [I])
     _lib = L
     return L
 
@@ -810,6 +811,10 @@ def debug_learned_far(radius: int):
 
 def debug_learned_passes(passes: int):
     _check(lib().bcc_debug_learned_passes(int(passes)))
+
+
+def debug_learned_min_steps(steps: int):
+    _check(lib().bcc_debug_learned_min_steps(int(steps)))
 
 
 def debug_gemm(ta: bool, tb: bool, a: np.ndarray, b: np.ndarray, m: int, n: int, k: int, alpha=1.0, beta=0.0,

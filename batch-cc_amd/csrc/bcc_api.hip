@@ -816,6 +816,14 @@ namespace {
 std::atomic<int> g_learned_tile{0};   // > 0: forced pairs per dense tile (tuning hook)
 std::atomic<int> g_learned_far{0};    // first-probe exclusion radius of the second probe's picks
 std::atomic<int> g_learned_passes{bcc::kLearnPassesDefault};   // probe passes (tuning hook)
+// Pairs of fewer steps skip inference and the later probes: a pair of one or
+// two steps costs less to finish than to infer and probe again (MI355X, c4's
+// parts: 3 beats 1 and 2 for every robot).
+constexpr int kLearnMinStepsDefault = 3;
+// Pairs per tile of learned ragged plans by default (MI355X, c4's parts: 64
+// beats 128 and 256 for every robot).
+constexpr int kLearnPlanTileDefault = 64;
+std::atomic<int> g_learned_min_steps{kLearnMinStepsDefault};
 
 // Pairs per tile of a learned launch (a wave's tile): about 16 tiles per
 // resident wave, a power of two in [32, 128] (MI355X: c1 runs best at 64 and
@@ -877,6 +885,7 @@ int dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const flo
     }
     m.far0 = learned_far_mask(g_learned_far.load());
     m.passes = g_learned_passes.load();
+    m.min_steps = g_learned_min_steps.load();
     const int variant = choose_variant(robot, mean_obstacles(scene));
     const bcc::VariantKnobs& k = knobs(robot);
     // one workgroup per resident slot: the variant's waves per SIMD
@@ -989,6 +998,14 @@ int bcc_debug_learned_passes(int passes)
     if (passes < 0 || passes > bcc::kLearnPassesMax)
         return fail(BCC_ERR_INVALID, "learned probe passes must be in [0, 3]");
     g_learned_passes.store(passes == 0 ? bcc::kLearnPassesDefault : passes);
+    return BCC_OK;
+}
+
+int bcc_debug_learned_min_steps(int steps)
+{
+    if (steps < 0)
+        return fail(BCC_ERR_INVALID, "learned minimum steps must be >= 0");
+    g_learned_min_steps.store(steps == 0 ? kLearnMinStepsDefault : steps);
     return BCC_OK;
 }
 
@@ -1427,6 +1444,7 @@ int create_plan_kind(int robot, const bcc_scene* scene, PlanBatch& b, const floa
         F.mlp = mlp ? *mlp : bcc::MlpArgs{};
         F.mlp.far0 = learned_far_mask(g_learned_far.load());
         F.mlp.passes = g_learned_passes.load();
+        F.mlp.min_steps = g_learned_min_steps.load();
         F.out = bcc::LearnedOut{b.slots, d_results, nullptr, nullptr, nullptr};
     } else {
         bcc::LearnArgs& L = p->learn;
@@ -1547,6 +1565,8 @@ int bcc_ragged_plan_create_learned(int robot, const bcc_model* model, const bcc_
     if (int rc = learned_args(robot, model, scene, d_edges, 0, resolution, d_results))
         return rc;
     PlanBatch b;
+    if (tile_pairs == 0)
+        tile_pairs = std::min(kLearnPlanTileDefault, bcc::kLearnTile);
     // the fused learned kernel queues a tile's undecided pairs in LDS
     if (int rc = ragged_batch(scene, env_offsets, robot, tile_pairs, groups_per_cu, bcc::kLearnTile, b, &tile_pairs))
         return rc;

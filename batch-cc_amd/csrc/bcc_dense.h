@@ -255,9 +255,10 @@ __device__ __forceinline__ int probe_bin(uint32_t top, int k)
 }
 
 // The step-0 sample of bin b of a pair, start + delta * (b n) (the plain
-// step-0 sample of lane b, same arithmetic), into Q.
+// step-0 sample of lane b, same arithmetic), into Q; returns the pair's step
+// count n.
 template <int D>
-__device__ __forceinline__ void probe_at(const float* __restrict__ se, int resolution, int bin, const LdsRow& Q)
+__device__ __forceinline__ int probe_at(const float* __restrict__ se, int resolution, int bin, const LdsRow& Q)
 {
     float s[D], d[D];
     const int n = setup_pair_sd<D>(se, resolution, s, d);
@@ -265,6 +266,7 @@ __device__ __forceinline__ void probe_at(const float* __restrict__ se, int resol
 #pragma unroll
     for (int j = 0; j < D; ++j)
         Q.at(j) = fmaf(d[j], bn, s[j]);
+    return n;
 }
 
 // Lane k of a probe group: step 0 of the pair's k-th probed bin into Q.
@@ -279,9 +281,9 @@ __device__ __forceinline__ void probe_sample(const float* __restrict__ se, int r
 // undecided): each half-wave takes the next queued pair from the LDS counter
 // *next.  pair_edge / pair_env / pair_out map a queue entry to its edge,
 // environment-independent verdict slot and rounds slot.
-template <class Robot, int kUnroll, class Pair>
+template <class Robot, int kUnroll, class Pair, class Queue>
 __device__ __forceinline__ void plain_queue(const EnvView& E, const float* __restrict__ edges, int resolution,
-                                            const int* queue, int nq, int* next, const LdsRow& Q, Pair pair)
+                                            const Queue& queue, int nq, int* next, const LdsRow& Q, Pair pair)
 {
     constexpr int D = Robot::dim;
     const int lane = threadIdx.x & 63;
@@ -718,9 +720,11 @@ hipError_t launch_learn(const LearnArgs& a, hipStream_t stream)
 //   1. probe: every pair's step-0 samples at K evenly spaced bins,
 //      (32 k + 16) / K, 64 / K pairs per batch; a hit decides the pair, the
 //      others are queued in LDS;
-//   2. inference, for the queued pairs only: the MLP on the matrix cores, 16
-//      pairs per MFMA column tile, giving each pair its order word (the
-//      spread top-4 among the bins not yet probed) in LDS;
+//   2. inference, for the queued pairs of at least min_steps steps only
+//      (shorter pairs go straight to the step loop: their whole sample set
+//      costs too little for a model to save more than it costs): the MLP on
+//      the matrix cores, 16 pairs per MFMA column tile, giving each pair its
+//      order word (the spread top-4 among the bins not yet probed) in LDS;
 //   3. probes of the queued pairs at the word's first K bins (and, with
 //      three passes, then at its next K); without a model: fixed bins between
 //      the ones probed before; each pass compacts the queue in place;
@@ -756,6 +760,15 @@ struct MlpArgs {
     const float* scale;   // 2 / joint range [dim]
     uint32_t far0;        // bins that count as far from the first probe's bins
     int passes;           // probe passes, 1 .. kLearnPassesMax
+    int min_steps;        // pairs of fewer steps skip inference and the later probes
+};
+
+// A wave's queue: `front` entries from the start of q, then the entries
+// written from the end of the array (q[cap - 1], q[cap - 2], ...).
+struct SplitQueue {
+    const int* q;
+    int front, cap;
+    __device__ __forceinline__ int operator[](int i) const { return i < front ? q[i] : q[cap - 1 - (i - front)]; }
 };
 
 // Where tiles come from: an explicit list (ragged batches), or dense tiles,
@@ -857,7 +870,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
         begin = __builtin_amdgcn_readfirstlane(begin);
         end = __builtin_amdgcn_readfirstlane(end);
         const EnvView E = load_env(obs, meta, env);
-        int nq = 0;   // queued pairs (wave-uniform)
+        int nq = 0;   // queued pairs at the front of the queue (wave-uniform)
+        int nshort = 0;   // pairs queued from the back, straight for the step loop
         int hits[kLearnPassesMax] = {0, 0, 0}, inferred = 0;
 #pragma unroll 1
         for (int pass = 0; pass < passes; ++pass) {
@@ -889,7 +903,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
             for (int j0 = 0; j0 < cnt; j0 += kPairs) {
                 const int j = j0 + grp;
                 const bool valid = j < cnt;
-                int i = 0;
+                int i = 0, n = 0;
                 uint32_t w = 0;   // the pair's order word (later passes, with a model)
                 if (valid) {
                     int bin;
@@ -905,19 +919,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                             bin = (32 * k + fixed_offset(pass)) / K;
                         }
                     }
-                    probe_at<D>(edges + (size_t)i * 2 * D, resolution, bin, Q);
+                    n = probe_at<D>(edges + (size_t)i * 2 * D, resolution, bin, Q);
                 }
                 const uint64_t live = __ballot(valid);
                 const uint64_t hit = Robot::template check<kUnroll, K>(Q, E, live);
                 const bool h = (hit >> lane) & 1ull;
                 const bool lead = valid && k == 0;
-                const uint64_t miss = __ballot(lead && !h);
+                const bool shortp = pass == 0 && n < mlp.min_steps;
+                const uint64_t miss = __ballot(lead && !h && !shortp);
+                const uint64_t miss_short = __ballot(lead && !h && shortp);
                 if (lead) {
                     if (h) {
                         const size_t sl = res.slots.slot(i, env);
                         res.out[sl] = 1;
                         if (res.stage)
                             res.stage[sl] = (uint8_t)pass;
+                    } else if (shortp) {
+                        queue[wave][kLearnTile - 1 - (nshort + __popcll(miss_short & below))] = i;
                     } else {
                         // the word moves with its pair
                         const int pos = nq + __popcll(miss & below);
@@ -925,6 +943,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                         word[wave][pos] = w;
                     }
                 }
+                nshort += __popcll(miss_short);
                 const int decided = __popcll(__ballot(lead && h));
                 if (pass == 0)
                     hits[0] += decided;
@@ -942,14 +961,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
             next_queued[wave] = 0;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        plain_queue<Robot, kUnroll>(E, edges, resolution, queue[wave], nq, &next_queued[wave], Q,
-                                    LearnedPairs{res, env, passes});
+        plain_queue<Robot, kUnroll>(E, edges, resolution, SplitQueue{queue[wave], nq, kLearnTile}, nq + nshort,
+                                    &next_queued[wave], Q, LearnedPairs{res, env, passes});
         if (res.stats && lane == 0) {
             atomicAdd(res.stats + 0, hits[0]);
             atomicAdd(res.stats + 1, hits[1]);
             atomicAdd(res.stats + 2, hits[2]);
             atomicAdd(res.stats + 3, inferred);
-            atomicAdd(res.stats + 4, nq);
+            atomicAdd(res.stats + 4, nq + nshort);
         }
     }
 }

@@ -7,20 +7,21 @@ Default workload: c3 (panda, 1000 environments x 100000 edges, 16 cuboids,
 8 spheres and 8 cylinders per environment).  A step is one pass of the path
 over the whole synthetic batch with inputs resident in HBM:
 
-  learned step (`value`): the step-ordering model ranks the sample bins of
-      every (env, edge) pair (fused MFMA inference), then the ordered kernel
-      checks each pair's step-0 samples of its two top bins, several pairs per
-      wave, and runs the plain step loop for the pairs still undecided;
+  learned step (`value`): one fused kernel per batch -- every pair's
+      step-0 samples at bins 8 and 24 first, then, for the pairs still
+      undecided only, the step-ordering model (BF16 MFMA inside the same
+      kernel) picks the two bins to probe next, and the plain step loop
+      finishes the rest;
   plain step (`plain_value`): the reference's order, 32 samples per step.
 
 Both are timed with hipEvents on the launch stream around exactly K steps
 after W untimed ones.  The model is trained first, on the GPU, on labels the
-exact checker writes for a separate seeded edge stream.  Verdicts of every
-variant are compared with the plain kernel's (0 mismatches, else exit 1).
-Beside the headline: the same ordered kernel with evenly spaced probe bins
-(the method without the network), with perfect bins taken from exact labels
-of the benchmark pairs themselves (the bound on any order), and how many
-colliding pairs each probe decides.
+exact checker writes for a separate seeded edge stream (the pairs the first
+probe misses).  Verdicts of every variant are compared with the plain
+kernel's (0 mismatches, else exit 1).  Beside the headline: the same fused
+kernel without a model (fixed second-probe bins: the method without the
+network), perfect probe bins taken from exact labels of the benchmark pairs
+themselves (the bound on any order), and how many pairs each stage decides.
 
 Roofline of the step's dominant kernel: before it touches the GPU, the run
 re-runs its own workload (one step of each kernel) three times in child
@@ -51,7 +52,6 @@ FP32_PEAK_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet)
 BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 MFMA peak (datasheet)
 HBM_PEAK_GBS = 8000.0
 DIM = {"panda": 7, "fetch": 8, "baxter": 14}
-PROBE = 2                    # predicted bins probed per pair (kProbe1 in csrc/bcc_dense.h)
 METRIC = "edge-env checks/sec at bit-exact verdicts, learned-ordered vs plain, 1 GPU"
 RID = {"panda": 0, "fetch": 1, "baxter": 2}
 
@@ -153,28 +153,36 @@ def cpu_baseline(robot, sc, edges, pair_env=None, seconds=10.0, one_thread_secon
 
 # ------------------------------------------------------------ learned step
 
-def train_model(bcc, W, a, robot, scene, E, n_train_edges, edge_law):
-    """The step-ordering model trained on the GPU: step-0 labels (the samples
-    the ordered kernel's probe checks) of a.train_pairs seeded (env, edge)
-    pairs of a separate edge stream, written by the exact checker."""
-    rng = np.random.default_rng([a.seed, 3])
-    train_edges = edge_law(robot, n_train_edges, rng)
+# first-probe bins of the fused learned kernel, (32 k + 16) / 2
+FIRST_PROBE_MASK = (1 << 8) | (1 << 24)
+
+
+def train_model(bcc, W, a, robot, scene, E, train_edges, rng):
+    """The step-ordering model trained on the GPU on the pairs it will be
+    asked about: step-0 labels (written by the exact checker) of seeded (env,
+    edge) pairs of a separate edge stream, kept when the fused kernel's first
+    probe would miss them (bins 8 and 24 free at step 0), a.train_pairs of
+    them from at most 16 times as many labelled pairs."""
     d_train = bcc.DeviceBuffer.from_array(train_edges)
     n = int(a.train_pairs)
-    pe = rng.integers(0, E, n).astype(np.int32)
-    pg = rng.integers(0, n_train_edges, n).astype(np.int32)
+    n_big = 16 * n
+    pe = rng.integers(0, E, n_big).astype(np.int32)
+    pg = rng.integers(0, train_edges.shape[0], n_big).astype(np.int32)
     t0 = time.perf_counter()
-    d_labels = bcc.label_pairs(robot, scene, d_train, pe, pg, steps=1)
+    lab = bcc.label_pairs(robot, scene, d_train, pe, pg, steps=1).to_array(np.uint32, (n_big,))
     t_label = time.perf_counter() - t0
+    keep = np.nonzero((lab & np.uint32(FIRST_PROBE_MASK)) == 0)[0][:n]
     model = bcc.Model(robot, seed=a.seed)
     t0 = time.perf_counter()
-    d_x = model.features(scene, d_train, pe, pg)
-    losses = model.train(d_x, d_labels, n, epochs=a.train_epochs, batch=8192, lr=2e-3)
+    d_x = model.features(scene, d_train, pe[keep], pg[keep])
+    d_labels = bcc.DeviceBuffer.from_array(lab[keep])
+    losses = model.train(d_x, d_labels, int(keep.size), epochs=a.train_epochs, batch=8192, lr=2e-3)
     bcc.synchronize()
     t_train = time.perf_counter() - t0
-    log(f"learned: {n} step-0 labels in {t_label:.1f} s, {a.train_epochs} epochs in {t_train:.1f} s, "
-        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
-    info = {"pairs": n, "epochs": a.train_epochs, "labels": "step-0 samples of each bin (steps=1)",
+    log(f"learned {robot}: {n_big} step-0 labels in {t_label:.1f} s, {keep.size} of them miss the first probe; "
+        f"{a.train_epochs} epochs in {t_train:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    info = {"labelled_pairs": n_big, "pairs": int(keep.size), "epochs": a.train_epochs,
+            "labels": "step-0 samples of each bin (steps=1) of pairs whose first-probe bins (8, 24) are free",
             "label_s": t_label, "train_s": t_train, "loss_first_epoch": float(losses[0]),
             "loss_last_epoch": float(losses[-1]), "edges": "separate seeded edge stream, same environments"}
     return model, info
@@ -212,44 +220,46 @@ def probe_decided(bcc, robot, scene, d_edges, M, E, d_order, plain_verdicts):
     return float((r[coll] == 0).mean()) if coll.any() else None
 
 
+def stage_counts(stats) -> dict:
+    return {"first_probe": int(stats[0]), "second_probe": int(stats[1]), "inferred": int(stats[3]),
+            "step_loop": int(stats[4])}
+
+
 def learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, plain_ms):
+    """The learned step on a dense batch: one fused kernel (first probe at
+    evenly spaced bins, in-kernel inference for the undecided pairs only,
+    probe of their predicted bins, plain steps), against the same kernel
+    without a model and the bound of perfect probe bins."""
+    rng = np.random.default_rng([a.seed, 3])
+    n_train_edges = max(1, min(M, 100_000))
     edge_law = W.lognormal_edges if robot == "fetch" else W.uniform_edges
-    model, train = train_model(bcc, W, a, robot, scene, E, max(1, min(M, 100_000)), edge_law)
-    d_order = bcc.DeviceBuffer(E * M * 4)
+    model, train = train_model(bcc, W, a, robot, scene, E, edge_law(robot, n_train_edges, rng), rng)
     d_out = bcc.DeviceBuffer(E * M)
-    step_ms, order_ms = model.time_learned_dense(scene, d_edges, M, d_order, d_out, a.warmup, a.steps)
-    got = d_out.to_array(np.uint8, (M, E))
-    mismatches = int((got != plain_verdicts).sum())
-    check_ms = bcc.time_dense_ex(robot, scene, d_edges, M, 32, d_out, d_order, a.warmup, a.steps)
-    mismatches += int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
-    log(f"learned step {step_ms:.3f} ms = inference {order_ms:.3f} ms + ordered check (alone {check_ms:.3f} ms); "
-        f"plain {plain_ms:.3f} ms; verdict mismatches {mismatches}")
+    step_ms = bcc.time_dense_learned(robot, model, scene, d_edges, M, d_out, a.warmup, a.steps)
+    mismatches = int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
+    st = bcc.check_dense_learned(robot, model, scene, d_edges, M, stats=True)
+    mismatches += int((st["verdicts"] != plain_verdicts).sum())
+    none_ms = bcc.time_dense_learned(robot, None, scene, d_edges, M, d_out, a.warmup, a.steps)
+    none_bad = int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
+    st0 = bcc.check_dense_learned(robot, None, scene, d_edges, M, stats=True)
+    none_bad += int((st0["verdicts"] != plain_verdicts).sum())
+    mismatches += none_bad
+    log(f"learned step {step_ms:.3f} ms (fused kernel, {stage_counts(st['stats'])}); no network {none_ms:.3f} ms "
+        f"({stage_counts(st0['stats'])}); plain {plain_ms:.3f} ms; verdict mismatches {mismatches}")
     res = {
         "ms_per_step": step_ms,
-        "inference_ms": order_ms,
-        "ordered_check_ms": check_ms,
         "speedup_vs_plain": plain_ms / step_ms,
         "verdict_mismatches_vs_plain": mismatches,
-        "inference_mfma": {"achieved": mlp_flops(robot) * E * M / (order_ms / 1e3) / 1e12,
-                           "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "flops_per_pair": mlp_flops(robot),
-                           "note": "hipEvent time of the inference (fragment packing, environment encoding, "
-                                   "fused 3-layer BF16 MFMA kernel with the top-4 selection)"},
-        "probe_decided_colliding": {"trained": probe_decided(bcc, robot, scene, d_edges, M, E, d_order,
-                                                             plain_verdicts)},
+        "kernel": "learned_kernel: first probe (bins 8, 24) -> in-kernel BF16 MFMA inference of the undecided "
+                  "pairs -> probe of their 2 top predicted bins -> plain steps, one persistent launch",
+        "pairs_by_stage": stage_counts(st["stats"]),
         "train": train,
+        "no_network": {"ms_per_step": none_ms, "speedup_vs_plain": plain_ms / none_ms,
+                       "verdict_mismatches_vs_plain": none_bad, "pairs_by_stage": stage_counts(st0["stats"]),
+                       "note": "the same fused kernel without a model: the second probe takes the fixed bins "
+                               "0 and 16"},
+        "network_gain_vs_no_network": none_ms / step_ms,
     }
-    res["inference_mfma"]["frac"] = res["inference_mfma"]["achieved"] / BF16_PEAK_TFLOPS
-    # the method without the network: evenly spaced probe bins
-    d_none = bcc.DeviceBuffer.from_array(np.full(E * M, 0xffffffff, np.uint32))
-    none_ms = bcc.time_dense_ex(robot, scene, d_edges, M, 32, d_out, d_none, a.warmup, a.steps)
-    none_bad = int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
-    res["no_network"] = {"ms_per_step": none_ms, "speedup_vs_plain": plain_ms / none_ms,
-                         "verdict_mismatches_vs_plain": none_bad,
-                         "note": "the same ordered kernel probing evenly spaced bins, no inference"}
-    res["probe_decided_colliding"]["evenly_spaced"] = probe_decided(bcc, robot, scene, d_edges, M, E, d_none,
-                                                                    plain_verdicts)
-    mismatches += none_bad
-    del d_none
     if a.bound:
         # perfect bins from exact step-0 labels of every benchmark pair
         t0 = time.perf_counter()
@@ -263,93 +273,73 @@ def learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, pl
         best_bad = int((d_out.to_array(np.uint8, (M, E)) != plain_verdicts).sum())
         res["oracle_bound"] = {"ordered_check_ms": best_ms, "speedup_vs_plain": plain_ms / best_ms,
                                "verdict_mismatches_vs_plain": best_bad,
-                               "note": "ordered check with perfect probe bins from exact step-0 labels of the "
-                                       "benchmark pairs (no inference): the bound on any learned order"}
+                               "probe_decided_colliding": probe_decided(bcc, robot, scene, d_edges, M, E, d_best,
+                                                                        plain_verdicts),
+                               "note": "every pair probed first at perfect bins from exact step-0 labels of the "
+                                       "benchmark pairs (ordered_dense_kernel, no inference): the bound on any "
+                                       "learned order"}
         res["oracle_bound_speedup"] = plain_ms / best_ms
-        res["probe_decided_colliding"]["perfect"] = probe_decided(bcc, robot, scene, d_edges, M, E, d_best,
-                                                                  plain_verdicts)
         mismatches += best_bad
-        log(f"no network {none_ms:.3f} ms, perfect bins {best_ms:.3f} ms ({time.perf_counter() - t0:.1f} s)")
+        log(f"perfect bins {best_ms:.3f} ms ({time.perf_counter() - t0:.1f} s)")
+        del d_best
+    res["verdict_mismatches_vs_plain"] = mismatches
     if mismatches:
-        log("ordered verdicts differ from plain order")
+        log("learned-ordered verdicts differ from plain order")
         sys.exit(1)
     model.close()
     return res
 
 
 def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
-    """Learned leg of the mixed-robot batch: one model per robot, trained on
-    step-0 labels of seeded (env, edge) pairs of that robot's environments
-    with a separate lognormal edge stream (the batch's edge law); one learned
-    step = inference for every pair of every robot (bcc_model_order_pairs) +
-    one replay of a group of ordered plans."""
-    n = int(a.train_pairs)
+    """Learned leg of the mixed-robot batch: one model per robot, trained as
+    for the dense configs on that robot's environments with a separate
+    lognormal edge stream (the batch's edge law); one learned step = one
+    replay of a group of learned plans (the fused kernel, persistent, per
+    robot), against the same group without models."""
     for k, r in enumerate(runs):
         rng = np.random.default_rng([a.seed, 3, k])
-        E = r["sc"].num_envs
         n_edges = max(1, min(r["edges"].shape[0], 100_000))
         train = W.lognormal_edges(r["robot"], n_edges, rng, median=W.MIXED_EDGE_MEDIAN)
-        d_train = bcc.DeviceBuffer.from_array(train)
-        pe = rng.integers(0, E, n).astype(np.int32)
-        pg = rng.integers(0, n_edges, n).astype(np.int32)
-        t0 = time.perf_counter()
-        d_labels = bcc.label_pairs(r["robot"], r["scene"], d_train, pe, pg, steps=1)
-        model = bcc.Model(r["robot"], seed=a.seed)
-        d_x = model.features(r["scene"], d_train, pe, pg)
-        losses = model.train(d_x, d_labels, n, epochs=a.train_epochs, batch=8192, lr=2e-3)
-        bcc.synchronize()
-        del d_x, d_labels, d_train
-        r["model"], r["losses"] = model, losses
-        r["pair_env"] = np.repeat(np.arange(E, dtype=np.int32), np.diff(r["off"]))
-        log(f"learned {r['robot']}: {n} step-0 labels, {a.train_epochs} epochs in "
-            f"{time.perf_counter() - t0:.1f} s, loss {losses[0]:.3f} -> {losses[-1]:.3f}")
-    order_ms = {}
+        r["model"], r["train"] = train_model(bcc, W, a, r["robot"], r["scene"], r["sc"].num_envs, train, rng)
+    res = {}
+    legs = (("learned", True),) if a.counter_child else (("learned", True), ("no_network", False))
+    for name, with_model in legs:
+        plans = []
+        for r in runs:
+            r["d_out_l"] = bcc.DeviceBuffer(max(1, r["edges"].shape[0]))
+            plans.append(bcc.RaggedPlan(r["robot"], r["scene"], r["off"], r["d_edges"], r["d_out_l"], learned=True,
+                                        model=r["model"] if with_model else None))
+        group = bcc.RaggedGroup(plans)
+        ms = group.time(a.warmup, a.steps)
+        alone = {r["robot"]: p.time(1, max(1, a.steps // 2)) for r, p in zip(runs, plans)}
+        bad = 0
+        for r in runs:
+            bad += int((r["d_out_l"].to_array(np.uint8, (r["edges"].shape[0],)) != r["v"]).sum())
+        group.close()
+        for p in plans:
+            p.close()
+        res[name] = {"ms_per_step": ms, "speedup_vs_plain": plain_ms / ms, "verdict_mismatches_vs_plain": bad,
+                     "ms_alone": alone}
+        log(f"{name} group {ms:.3f} ms (plain group {plain_ms:.3f} ms; alone "
+            + ", ".join(f"{k} {v:.2f}" for k, v in alone.items()) + f"); verdict mismatches {bad}")
     for r in runs:
-        d_env = bcc.DeviceBuffer.from_array(r["pair_env"])
-        r["d_order"] = r["model"].order_pairs(r["scene"], r["d_edges"], d_env)
-        ts = []
-        for _ in range(max(3, a.steps // 4)):
-            t0 = time.perf_counter()
-            r["model"].order_pairs(r["scene"], r["d_edges"], d_env, out=r["d_order"])
-            ts.append((time.perf_counter() - t0) * 1e3)
-        order_ms[r["robot"]] = float(np.median(ts))
-    for r in runs:
-        r["d_out_l"] = bcc.DeviceBuffer(max(1, r["edges"].shape[0]))
-        r["plan_l"] = bcc.RaggedPlan(r["robot"], r["scene"], r["off"], r["d_edges"], r["d_out_l"],
-                                     d_order=r["d_order"])
-    group = bcc.RaggedGroup([r["plan_l"] for r in runs])
-    check_ms = group.time(a.warmup, a.steps)
-    mismatches = 0
-    for r in runs:
-        mismatches += int((r["d_out_l"].to_array(np.uint8, (r["edges"].shape[0],)) != r["v"]).sum())
-    group.close()
-    for r in runs:
-        r["plan_l"].close()
         r["model"].close()
-    total_order = sum(order_ms.values())
-    log(f"learned step: order {total_order:.2f} ms (" + ", ".join(f"{k} {v:.2f}" for k, v in order_ms.items())
-        + f") + ordered group {check_ms:.3f} ms (plain group {plain_ms:.3f} ms); verdict mismatches {mismatches}")
+    if a.counter_child:
+        return dict(res["learned"])
+    mismatches = res["learned"]["verdict_mismatches_vs_plain"] + res["no_network"]["verdict_mismatches_vs_plain"]
     if mismatches:
         log("learned-ordered verdicts differ from plain order")
         sys.exit(1)
-    flops = sum(mlp_flops(r["robot"]) * r["edges"].shape[0] for r in runs)
-    mfma = flops / (total_order / 1e3) / 1e12
-    return {
-        "ms_per_step": total_order + check_ms,
-        "inference_ms": total_order,
-        "inference_ms_per_robot": order_ms,
-        "inference_mfma": {"achieved": mfma, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": mfma / BF16_PEAK_TFLOPS,
-                           "note": "wall time of the inference calls (fragment packing, encoder, fused kernel)"},
-        "ordered_check_ms": check_ms,
-        "speedup_vs_plain": plain_ms / (total_order + check_ms),
+    out = dict(res["learned"])
+    out.update({
+        "kernel": "learned_kernel per robot (first probe, in-kernel BF16 MFMA inference of the undecided pairs, "
+                  "predicted probe, plain steps), one persistent launch per robot, one hipGraph",
+        "no_network": res["no_network"],
+        "network_gain_vs_no_network": res["no_network"]["ms_per_step"] / res["learned"]["ms_per_step"],
         "verdict_mismatches_vs_plain": mismatches,
-        "train": {"pairs_per_robot": n, "epochs": a.train_epochs, "labels": "step-0 samples of each bin",
-                  "loss_first_epoch": {r["robot"]: float(r["losses"][0]) for r in runs},
-                  "loss_last_epoch": {r["robot"]: float(r["losses"][-1]) for r in runs},
-                  "edges": "separate seeded lognormal edge stream per robot, same environments"},
-    }
-
+        "train": {r["robot"]: r["train"] for r in runs},
+    })
+    return out
 
 
 def make_workload(W, a):
@@ -387,7 +377,7 @@ def child_args(a) -> list:
 def counter_child(bcc, W, C, a):
     """Runs under rocprofv3 --pmc: the known-work kernels, then one launch of
     the workload's plain kernel and, for learned configs, the model trained as
-    in the measured run and one learned step (inference + ordered check)."""
+    in the measured run and one learned step (the fused kernel)."""
     C.run_known_work(bcc)
     if a.config == "c4":
         a.steps, a.warmup, a.no_work, a.no_cpu_baseline = 1, 0, True, True
@@ -406,10 +396,10 @@ def counter_child(bcc, W, C, a):
         bcc.check_dense_device(robot, scene, d_edges, M, 32, d_out)
         bcc.synchronize()
         if a.learned:
+            rng = np.random.default_rng([a.seed, 3])
             edge_law = W.lognormal_edges if robot == "fetch" else W.uniform_edges
-            model, _ = train_model(bcc, W, a, robot, scene, E, max(1, min(M, 100_000)), edge_law)
-            d_order = bcc.DeviceBuffer(E * M * 4)
-            model.time_learned_dense(scene, d_edges, M, d_order, d_out, 0, 1)
+            model, _ = train_model(bcc, W, a, robot, scene, E, edge_law(robot, max(1, min(M, 100_000)), rng), rng)
+            bcc.time_dense_learned(robot, model, scene, d_edges, M, d_out, 0, 1)
             bcc.synchronize()
             model.close()
     scene.close()
@@ -458,6 +448,22 @@ def fp32_fma_rate(bcc) -> dict:
     flops = (blocks * 4) * 64 * (2 * C.VALU_CHAINS * iters + C.VALU_CHAINS - 1)
     return {"tflops": flops / (ms / 1e3) / 1e12, "ms": ms, "flops": float(flops),
             "kernel": f"valu_kernel: {blocks} x 256 threads, 8 chains x {iters} v_fma_f32 per lane"}
+
+
+def mfma_rate(hw, match, step_ms: float) -> dict:
+    """BF16 matrix-core work of the learned step (the in-kernel inference):
+    SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 flops per launch over the step time,
+    against the dense BF16 peak."""
+    out = {"peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "flops_per_pair": mlp_flops("panda"),
+           "note": "executed MFMA flops (padded K: 96 inputs per layer-1 row) of the inference inside the learned "
+                   "kernel, averaged over the whole step; inference runs for the undecided pairs only"}
+    if hw and "sq" in hw:
+        mops = bcc_counters().per_step(hw["sq"], match, "SQ_INSTS_VALU_MFMA_MOPS_BF16")
+        if mops:
+            out["executed_flops_per_launch"] = 512.0 * mops
+            out["achieved"] = 512.0 * mops / (step_ms / 1e3) / 1e12
+            out["frac"] = out["achieved"] / BF16_PEAK_TFLOPS
+    return out
 
 
 def is_kernel(prefix: str):
@@ -516,24 +522,17 @@ def bench_mixed(bcc, W, a, hw, cal) -> dict:
                 sys.exit(1)
             flops += float(work.sum(dtype=np.float64))
             trigs += float(trig.sum(dtype=np.float64))
-    # the learned step's check (learn_finish_kernel per robot) dominates it;
-    # the plain group's ragged_kernel per robot is reported beside it
+    # the learned step (learned_kernel per robot) is the measured step; the
+    # plain group's ragged_kernel per robot is reported beside it
     plain_rf = build_roofline(bcc_counters(), hw, cal, is_kernel("bcc::ragged_kernel<"),
                               "ragged_kernel x3 robots (one hipGraph, plain order)", ms, flops, trigs)
     if learned:
-        roofline = build_roofline(bcc_counters(), hw, cal, is_kernel("bcc::learn_finish_kernel<"),
-                                  "learn_finish_kernel x3 robots (one hipGraph, the learned step's check)",
-                                  learned["ordered_check_ms"], flops, trigs)
+        roofline = build_roofline(bcc_counters(), hw, cal, is_kernel("bcc::learned_kernel<"),
+                                  "learned_kernel x3 robots (one hipGraph: the whole learned step)",
+                                  learned["ms_per_step"], flops, trigs)
         plain_rf.pop("calibration", None)
         roofline["plain_kernel"] = plain_rf
-        if hw and "sq" in hw:
-            mops = bcc_counters().per_step(hw["sq"], is_kernel("bcc::fused_order_kernel"),
-                                           "SQ_INSTS_VALU_MFMA_MOPS_BF16")
-            if mops:
-                im = learned["inference_mfma"]
-                im["executed_flops_per_launch"] = 512.0 * mops * len(runs)
-                im["executed_tflops"] = im["executed_flops_per_launch"] / (learned["inference_ms"] / 1e3) / 1e12
-                im["executed_frac"] = im["executed_tflops"] / BF16_PEAK_TFLOPS
+        learned["mfma"] = mfma_rate(hw, is_kernel("bcc::learned_kernel<"), learned["ms_per_step"])
     else:
         roofline = plain_rf
     if roofline["achieved"] is None:
@@ -703,24 +702,14 @@ def main():
         trigs = float(trig.sum(dtype=np.float64))
         del work, trig, wv
         log(f"work count: {flops:.4g} flops/launch ({time.perf_counter() - t1:.1f} s)")
-    # the dominant kernel of the measured step: the ordered check of the
-    # learned step, else the plain kernel
+    # the measured step's kernel: the fused learned kernel, else the plain one
     if learned:
-        roofline = build_roofline(C, hw, cal, is_kernel("bcc::ordered_dense_kernel<"),
-                                  "ordered_dense_kernel (the learned step's check)", learned["ordered_check_ms"],
-                                  flops, trigs)
+        roofline = build_roofline(C, hw, cal, is_kernel("bcc::learned_kernel<"),
+                                  "learned_kernel (the whole learned step)", learned["ms_per_step"], flops, trigs)
         roofline["plain_kernel"] = build_roofline(C, hw, cal, is_kernel("bcc::dense_kernel<"),
                                                   "dense_kernel (plain order)", ms, flops, trigs)
         roofline["plain_kernel"].pop("calibration", None)
-        if hw and "sq" in hw:
-            mops = C.per_step(hw["sq"], is_kernel("bcc::fused_order_kernel"), "SQ_INSTS_VALU_MFMA_MOPS_BF16")
-            if mops:
-                im = learned["inference_mfma"]
-                im["executed_flops_per_launch"] = 512.0 * mops
-                im["executed_tflops"] = 512.0 * mops / (learned["inference_ms"] / 1e3) / 1e12
-                im["executed_frac"] = im["executed_tflops"] / BF16_PEAK_TFLOPS
-                im["executed_note"] = ("SQ_INSTS_VALU_MFMA_MOPS_BF16 x 512 of fused_order_kernel (padded K: 96 "
-                                       "inputs per layer-1 row) over the whole inference time")
+        learned["mfma"] = mfma_rate(hw, is_kernel("bcc::learned_kernel<"), learned["ms_per_step"])
     else:
         roofline = build_roofline(C, hw, cal, is_kernel("bcc::ragged_kernel<" if ragged else "bcc::dense_kernel<"),
                                   "ragged_kernel (hipGraph replay)" if ragged else "dense_kernel", ms, flops, trigs)
@@ -758,7 +747,8 @@ def main():
             "layout": "ragged CSR, one persistent launch replayed as a hipGraph" if ragged else "dense E x M",
             "pairs": pairs,
             "resolution": 32,
-            "order": "learned (probe of the 2 top predicted bins, then plain steps)" if learned else "plain",
+            "order": ("learned (one fused kernel: probe of bins 8 and 24, in-kernel inference of the undecided "
+                      "pairs, probe of their 2 top predicted bins, then plain steps)") if learned else "plain",
             "collision_fraction": coll,
         },
         "plain_value": plain_value,

@@ -64,9 +64,11 @@ def test_fused_learned_step_matches_the_oracle(bcc, oracle, gpu, robot):
         try:
             for v in (0, 1):
                 bcc.debug_force_variant(v)
-                for tile, passes in ((0, 2), (16, 3), (100, 1), (256, 3)):
+                for tile, passes, min_steps in ((0, 2, 2), (16, 3, 1), (100, 1, 2), (256, 3, 5)):
                     bcc.debug_learned_tile(tile)
                     bcc.debug_learned_passes(passes)
+                    bcc.debug_learned_min_steps(min_steps)
+                    long_ = np.broadcast_to(n[:, None] >= min_steps, exp.shape)
                     for name, model in models.items():
                         r = bcc.check_dense_learned(robot, model, scene, de, M, stage=True, stats=True)
                         assert np.array_equal(r["verdicts"], exp), (v, tile, passes, name)
@@ -75,15 +77,19 @@ def test_fused_learned_step_matches_the_oracle(bcc, oracle, gpu, robot):
                         plain_free = np.broadcast_to(n[:, None] + passes - 1, exp.shape)
                         assert np.all(st[~exp] == plain_free[~exp])
                         assert np.all(st <= n[:, None] + passes - 1)
+                        # pairs shorter than min_steps skip inference and the later probes
+                        assert not np.any((st >= 1) & (st < passes) & ~long_)
                         s = r["stats"]
                         for p in range(3):
                             assert s[p] == ((st == p).sum() if p < passes else 0), (p, passes)
                         assert s[4] == (st >= passes).sum()
-                        assert s[3] == (0 if model is None or passes == 1 else M * E - s[0])
+                        inferred = 0 if model is None or passes == 1 else ((st != 0) & long_).sum()
+                        assert s[3] == inferred, (s[3], inferred)
         finally:
             bcc.debug_force_variant(-1)
             bcc.debug_learned_tile(0)
             bcc.debug_learned_passes(0)
+            bcc.debug_learned_min_steps(0)
 
 
 @pytest.mark.parametrize("radius", [0, 4])
@@ -109,11 +115,13 @@ def test_stages_follow_the_step0_labels_and_the_inferred_words(bcc, gpu, radius)
         try:
             bcc.debug_learned_far(radius)
             bcc.debug_learned_passes(3)
+            bcc.debug_learned_min_steps(1)   # every pair the first probe misses is inferred
             r = bcc.check_dense_learned("panda", model, scene, de, M, stage=True, words=True)
             r0 = bcc.check_dense_learned("panda", None, scene, de, M, stage=True)
         finally:
             bcc.debug_learned_far(0)
             bcc.debug_learned_passes(0)
+            bcc.debug_learned_min_steps(0)
     bit = lambda b: (labels >> np.uint32(b)) & np.uint32(1)
     first = (bit(FIRST[0]) | bit(FIRST[1])).astype(bool)
     st, words = r["stage"], r["words"]
