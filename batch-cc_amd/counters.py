@@ -119,6 +119,14 @@ def per_step(kernels: dict, match, counter: str) -> float | None:
     return tot if seen else None
 
 
+# Counters are collected for these kernels only: the measured kernels and the
+# known-work ones.  The training kernels (tens of thousands of dispatches per
+# model) are left out; with them the mixed-robot child's passes ended without
+# a counter file.
+MEASURED_KERNELS = ("(learned_kernel|dense_kernel|ragged_kernel|learn_finish_kernel|fused_order_kernel|valu_kernel"
+                    "|copy_kernel)")
+
+
 def collect(bench_py: str, child_args: list, log, timeout: float = 300.0) -> dict | None:
     """Run the counter children; returns {pass name: parsed kernels} (a
     failed pass is left out) or None without rocprofv3."""
@@ -129,8 +137,8 @@ def collect(bench_py: str, child_args: list, log, timeout: float = 300.0) -> dic
     results = {}
     for name, counters in PASSES:
         out_dir = tempfile.mkdtemp(prefix=f"bcc_pmc_{name}_", dir="/tmp")
-        cmd = [prof, "--pmc", *counters, "-d", out_dir, "-o", "p", "--output-format", "csv", "--",
-               sys.executable, bench_py, *child_args]
+        cmd = [prof, "--pmc", *counters, "--kernel-include-regex", MEASURED_KERNELS, "-d", out_dir, "-o", "p",
+               "--output-format", "csv", "--", sys.executable, bench_py, *child_args]
         t0 = time.perf_counter()
         log_path = os.path.join(out_dir, "child.log")
         with open(log_path, "w") as lf:
