@@ -21,9 +21,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbcc.so")
 
-ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2}
-DIMS = {"panda": 7, "fetch": 8, "baxter": 14}
-NUM_SPHERES = {"panda": 70, "fetch": 125, "baxter": 100}
+ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2, "xarm7": 3}
+DIMS = {"panda": 7, "fetch": 8, "baxter": 14, "xarm7": 13}
+NUM_SPHERES = {"panda": 70, "fetch": 125, "baxter": 100, "xarm7": 74}
 # packed host layout: floats per primitive of kind
 # spheres, capsules, z-capsules, cylinders, cuboids, z-cuboids
 KIND_STRIDE = (4, 8, 8, 8, 15, 15)

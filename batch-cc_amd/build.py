@@ -18,7 +18,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-ROBOTS = ("panda", "fetch", "baxter")
+ROBOTS = ("panda", "fetch", "baxter", "xarm7")
 LIB = os.path.join(HERE, "libbcc.so")
 OBJDIR = os.path.join(HERE, "_build")
 

@@ -25,6 +25,7 @@
 #include "bcc_internal.h"
 #include "bcc_tuning.h"
 #include "robots/baxter_model.h"
+#include "robots/xarm7_model.h"
 #include "robots/fetch_model.h"
 #include "robots/panda_model.h"
 
@@ -366,6 +367,7 @@ const bcc::VariantKnobs& knobs(int robot)
     switch (robot) {
     case BCC_FETCH: return bcc::knobs_fetch;
     case BCC_BAXTER: return bcc::knobs_baxter;
+    case BCC_XARM7: return bcc::knobs_xarm7;
     default: return bcc::knobs_panda;
     }
 }
@@ -421,6 +423,7 @@ int dispatch_dense(int robot, const bcc_scene* scene, bcc::DenseArgs a, hipStrea
     case BCC_PANDA: e = bcc::launch_dense_panda(a, s); break;
     case BCC_FETCH: e = bcc::launch_dense_fetch(a, s); break;
     case BCC_BAXTER: e = bcc::launch_dense_baxter(a, s); break;
+    case BCC_XARM7: e = bcc::launch_dense_xarm7(a, s); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
     }
     if (e != hipSuccess)
@@ -561,6 +564,7 @@ int bcc_robot_dimension(int robot)
     case BCC_PANDA: return 7;
     case BCC_FETCH: return 8;
     case BCC_BAXTER: return 14;
+    case BCC_XARM7: return 13;
     default: return -1;
     }
 }
@@ -571,6 +575,7 @@ int bcc_robot_num_spheres(int robot)
     case BCC_PANDA: return 70;
     case BCC_FETCH: return 125;
     case BCC_BAXTER: return 100;
+    case BCC_XARM7: return 74;
     default: return -1;
     }
 }
@@ -860,6 +865,7 @@ int dispatch_learned(int robot, const bcc::LearnedArgs& a, hipStream_t s)
     case BCC_PANDA: e = bcc::launch_learned_panda(a, s); break;
     case BCC_FETCH: e = bcc::launch_learned_fetch(a, s); break;
     case BCC_BAXTER: e = bcc::launch_learned_baxter(a, s); break;
+    case BCC_XARM7: e = bcc::launch_learned_xarm7(a, s); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
     }
     if (e != hipSuccess)
@@ -1027,6 +1033,7 @@ const bcc_robot_model* host_model(int robot)
     case BCC_PANDA: return &bcc_panda_model;
     case BCC_FETCH: return &bcc_fetch_model;
     case BCC_BAXTER: return &bcc_baxter_model;
+    case BCC_XARM7: return &bcc_xarm7_model;
     default: return nullptr;
     }
 }
@@ -1148,6 +1155,7 @@ static int work_common(int robot, const bcc_scene* scene, const float* d_edges, 
     case BCC_PANDA: e = bcc::launch_work_panda(a, s); break;
     case BCC_FETCH: e = bcc::launch_work_fetch(a, s); break;
     case BCC_BAXTER: e = bcc::launch_work_baxter(a, s); break;
+    case BCC_XARM7: e = bcc::launch_work_xarm7(a, s); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id");
     }
     if (e != hipSuccess)
@@ -1174,6 +1182,7 @@ int bcc_audit_clearance(int robot, const bcc_scene* scene, const float* d_edges,
     case BCC_PANDA: e = bcc::launch_audit_panda(a, s); break;
     case BCC_FETCH: e = bcc::launch_audit_fetch(a, s); break;
     case BCC_BAXTER: e = bcc::launch_audit_baxter(a, s); break;
+    case BCC_XARM7: e = bcc::launch_audit_xarm7(a, s); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id");
     }
     if (e != hipSuccess)
@@ -1229,6 +1238,7 @@ int dispatch_ragged(int robot, const bcc::RaggedArgs& a, hipStream_t s)
     case BCC_PANDA: e = bcc::launch_ragged_panda(a, s); break;
     case BCC_FETCH: e = bcc::launch_ragged_fetch(a, s); break;
     case BCC_BAXTER: e = bcc::launch_ragged_baxter(a, s); break;
+    case BCC_XARM7: e = bcc::launch_ragged_xarm7(a, s); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
     }
     if (e != hipSuccess)
@@ -1243,6 +1253,7 @@ int dispatch_learn(int robot, const bcc::LearnArgs& a, hipStream_t s)
     case BCC_PANDA: e = bcc::launch_learn_panda(a, s); break;
     case BCC_FETCH: e = bcc::launch_learn_fetch(a, s); break;
     case BCC_BAXTER: e = bcc::launch_learn_baxter(a, s); break;
+    case BCC_XARM7: e = bcc::launch_learn_xarm7(a, s); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id " + std::to_string(robot));
     }
     if (e != hipSuccess)
@@ -2005,6 +2016,7 @@ int bcc_debug_fk(int robot, const float* d_q, int n, float* d_out)
     case BCC_PANDA: e = bcc::launch_fk_panda(d_q, n, d_out, nullptr); break;
     case BCC_FETCH: e = bcc::launch_fk_fetch(d_q, n, d_out, nullptr); break;
     case BCC_BAXTER: e = bcc::launch_fk_baxter(d_q, n, d_out, nullptr); break;
+    case BCC_XARM7: e = bcc::launch_fk_xarm7(d_q, n, d_out, nullptr); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id");
     }
     if (e != hipSuccess)
@@ -2035,6 +2047,7 @@ int bcc_debug_samples(int robot, const float* d_edges, int m, int resolution, in
     case BCC_PANDA: e = bcc::launch_samples_panda(d_edges, m, resolution, d_n, d_cfg, d_last, nullptr); break;
     case BCC_FETCH: e = bcc::launch_samples_fetch(d_edges, m, resolution, d_n, d_cfg, d_last, nullptr); break;
     case BCC_BAXTER: e = bcc::launch_samples_baxter(d_edges, m, resolution, d_n, d_cfg, d_last, nullptr); break;
+    case BCC_XARM7: e = bcc::launch_samples_xarm7(d_edges, m, resolution, d_n, d_cfg, d_last, nullptr); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id");
     }
     if (e != hipSuccess)

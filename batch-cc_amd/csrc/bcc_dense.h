@@ -1512,5 +1512,6 @@ hipError_t launch_samples(const float* edges, int m, int resolution, int* n_out,
 BCC_DECLARE_ROBOT(panda)
 BCC_DECLARE_ROBOT(fetch)
 BCC_DECLARE_ROBOT(baxter)
+BCC_DECLARE_ROBOT(xarm7)
 
 }  // namespace bcc

@@ -506,6 +506,7 @@ int dispatch_label(int robot, const LabelArgs& a, hipStream_t s)
     case BCC_PANDA: e = launch_label_panda(a, s); break;
     case BCC_FETCH: e = launch_label_fetch(a, s); break;
     case BCC_BAXTER: e = launch_label_baxter(a, s); break;
+    case BCC_XARM7: e = launch_label_xarm7(a, s); break;
     default: return fail(BCC_ERR_INVALID, "unknown robot id");
     }
     if (e != hipSuccess)
@@ -538,16 +539,18 @@ struct bcc_model {
 
 namespace {
 
-static const float kLo[3][16] = {
+static const float kLo[4][16] = {
     {-2.9671f, -1.8326f, -2.9671f, -3.1416f, -2.9671f, -0.0873f, -2.9671f},
     {0.0f, -1.6056f, -1.221f, -3.14159f, -2.251f, -3.14159f, -2.16f, -3.14159f},
     {-1.70167993878f, -2.147f, -3.05417993878f, -0.05f, -3.059f, -1.57079632679f, -3.059f, -1.70167993878f,
-     -2.147f, -3.05417993878f, -0.05f, -3.059f, -1.57079632679f, -3.059f}};
-static const float kRange[3][16] = {
+     -2.147f, -3.05417993878f, -0.05f, -3.059f, -1.57079632679f, -3.059f},
+    {-2.8f, -1.7f, -2.8f, 0.0f, -2.8f, 0.0f, -2.8f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}};
+static const float kRange[4][16] = {
     {5.9342f, 3.6652f, 5.9342f, 3.2289f, 5.9342f, 3.9095999999999997f, 5.9342f},
     {0.38615f, 3.2112f, 2.739f, 6.28318f, 4.502f, 6.28318f, 4.32f, 6.28318f},
     {3.40335987756f, 3.194f, 6.10835987756f, 2.6679999999999997f, 6.118f, 3.66479632679f, 6.118f,
-     3.40335987756f, 3.194f, 6.10835987756f, 2.6679999999999997f, 6.118f, 3.66479632679f, 6.118f}};
+     3.40335987756f, 3.194f, 6.10835987756f, 2.6679999999999997f, 6.118f, 3.66479632679f, 6.118f},
+    {5.6f, 3.4f, 5.6f, 3.0f, 5.6f, 3.1f, 5.6f, 0.85f, 0.85f, 0.85f, 0.85f, 0.85f, 0.85f}};
 
 void free_scratch(bcc_model* M)
 {

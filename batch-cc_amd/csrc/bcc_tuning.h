@@ -5,7 +5,7 @@
 #pragma once
 
 struct bcc_tune_entry {
-    int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER
+    int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER / BCC_XARM7
     int tiles_per_wave;   // target tiles per persistent wave
     int groups_per_cu;    // persistent workgroups per CU
 };
@@ -16,13 +16,14 @@ static const int kCullMinWork = 32;
 
 // Kernel variant per launch: the high-occupancy variant when the mean number
 // of tested obstacles per environment (weighted by pairs) is at least the
-// first value and below the second, per robot (panda, fetch, baxter).
-static const int kHiVariantMinObstacles[3] = {20, 0, 1073741824};
-static const int kHiVariantMaxObstacles[3] = {163, 1073741824, 1073741824};
+// first value and below the second, per robot (panda, fetch, baxter, xarm7).
+static const int kHiVariantMinObstacles[4] = {20, 0, 1073741824, 1073741824};
+static const int kHiVariantMaxObstacles[4] = {163, 1073741824, 1073741824, 1073741824};
 
 // source: tune.py on gfx950, 2026-12-04, seed 0
 static const bcc_tune_entry kTuneTable[] = {
     {0, 8, 6},   // panda: score 1.015
     {1, 4, 8},   // fetch: score 1.003
     {2, 16, 4},   // baxter: score 1.000
+    {3, 16, 4},   // xarm7: baxter's setting (not tuned)
 };

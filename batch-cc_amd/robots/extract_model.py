@@ -3,8 +3,12 @@ generated forward-kinematics + collision-check code.
 
 The reference ships its robot models only as generated straight-line CUDA
 (``src/utils.cuh``): ``fkcc<Panda>`` at src/utils.cuh:342-5442,
-``fkcc<Fetch>`` at src/utils.cuh:10382-24992 and ``fkcc<Baxter>`` at
-src/utils.cuh:24994-38241.  That code is data in disguise: a list of sphere
+``fkcc<Fetch>`` at src/utils.cuh:10382-24992, ``fkcc<Baxter>`` at
+src/utils.cuh:24994-38241 and ``fkcc<Xarm7>`` at src/utils.cuh:38243-44112.
+The Xarm7 code reads 13 joint values (q[0..12]: seven arm joints and the
+six joints of the gripper) although its traits declare dimension 7
+(src/Robots.hh:190-193); its scale tables hold 13 entries
+(src/Robots.hh:196-203), so the model is extracted with 13 joints.  That code is data in disguise: a list of sphere
 centres written as closed-form expressions of the joint angles, plus a fixed
 program of bounding-sphere-gated environment and self-collision tests.
 
@@ -27,7 +31,7 @@ evaluation of the reference expressions (max centre error is written into
 the model).  The output JSON is committed; this tool is only needed to
 re-derive it and needs the reference sources on disk.
 
-Usage:  python extract_model.py <path/to/utils.cuh> <panda|fetch|baxter> <out.json>
+Usage:  python extract_model.py <path/to/utils.cuh> <panda|fetch|baxter|xarm7> <out.json>
 """
 from __future__ import annotations
 
@@ -38,8 +42,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-ROBOT_DIM = {"panda": 7, "fetch": 8, "baxter": 14}
-ROBOT_TYPE = {"panda": "Panda", "fetch": "Fetch", "baxter": "Baxter"}
+ROBOT_DIM = {"panda": 7, "fetch": 8, "baxter": 14, "xarm7": 13}
+ROBOT_TYPE = {"panda": "Panda", "fetch": "Fetch", "baxter": "Baxter", "xarm7": "Xarm7"}
 
 # ---------------------------------------------------------------- tokenizer
 

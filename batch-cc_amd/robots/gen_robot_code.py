@@ -227,18 +227,20 @@ def padded_radius(r: float) -> float:
 # groups in one sweep.  Panda's (2, 4, 16) trades 5% on c3 for 16% on c4's
 # panda part; fetch's gate chunk 5 is best on c1 and c4's fetch part (4 and 6
 # are 7-8% slower: a register-allocation optimum rather than a trend).
-CHUNKS = {"panda": (2, 4, 16), "fetch": (1, 5, 16), "baxter": (1, 4, 16)}
+CHUNKS = {"panda": (2, 4, 16), "fetch": (1, 5, 16), "baxter": (1, 4, 16),
+          # xarm7's program has no gated groups: 74 lone environment spheres
+          "xarm7": (2, 4, 16)}
 # Sweeps below the culling threshold run plain counted loops over all
 # obstacles (a second copy of each sweep's code) or the culled loop with every
 # obstacle marked near.  The second copy pays for panda and fetch; for
 # baxter's much longer check program the extra code costs more in
 # instruction-cache misses than the plain loops save (MI355X, c2 and c4).
-DUAL_SWEEP = {"panda": True, "fetch": True, "baxter": False}
+DUAL_SWEEP = {"panda": True, "fetch": True, "baxter": False, "xarm7": False}
 # Dense kernels compute each round's samples from the edge (as the ragged and
 # learned-order kernels do) or step them by an increment row in LDS: the
 # first is faster for fetch (c1 -1.7%), no faster for panda and baxter
 # (within the 2% run-to-run spread).  MI355X measurements.
-ROUNDS_FROM_EDGE = {"panda": False, "fetch": True, "baxter": False}
+ROUNDS_FROM_EDGE = {"panda": False, "fetch": True, "baxter": False, "xarm7": False}
 GATE_CHUNK = 2      # defaults for robots not listed; set per robot by gen_check
 FINE_CHUNK = 4
 FIRST_CHUNK = 0     # size of the first environment chunk (0: GATE_CHUNK)

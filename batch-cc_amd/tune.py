@@ -39,8 +39,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 TILES_PER_WAVE = (4, 8, 16, 32, 64)
 GROUPS = (4, 5, 6, 8)
-ROBOTS = ("panda", "fetch", "baxter")
-PAIRS = {"panda": 400_000, "fetch": 400_000, "baxter": 200_000}
+ROBOTS = ("panda", "fetch", "baxter", "xarm7")
+PAIRS = {"panda": 400_000, "fetch": 400_000, "baxter": 200_000, "xarm7": 200_000}
 
 
 def load_pkg():
@@ -227,7 +227,7 @@ def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:
         "#pragma once",
         "",
         "struct bcc_tune_entry {",
-        "    int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER",
+        "    int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER / BCC_XARM7",
         "    int tiles_per_wave;   // target tiles per persistent wave",
         "    int groups_per_cu;    // persistent workgroups per CU",
         "};",
@@ -238,10 +238,10 @@ def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:
         "",
         "// Kernel variant per launch: the high-occupancy variant when the mean number",
         "// of tested obstacles per environment (weighted by pairs) is at least the",
-        "// first value and below the second, per robot (panda, fetch, baxter).",
-        "static const int kHiVariantMinObstacles[3] = {"
+        "// first value and below the second, per robot (" + ", ".join(ROBOTS) + ").",
+        f"static const int kHiVariantMinObstacles[{len(ROBOTS)}] = {{"
         + ", ".join(str(thresholds[r][0]) for r in ROBOTS) + "};",
-        "static const int kHiVariantMaxObstacles[3] = {"
+        f"static const int kHiVariantMaxObstacles[{len(ROBOTS)}] = {{"
         + ", ".join(str(thresholds[r][1]) for r in ROBOTS) + "};",
         "",
         f"// source: {source}",

@@ -40,6 +40,9 @@ LIMITS = {
                         np.float32),
                np.array([3.40335987756, 3.194, 6.10835987756, 2.6679999999999997, 6.118, 3.66479632679, 6.118] * 2,
                         np.float32)),
+    # seven arm joints, then the six gripper joints the generated model reads
+    "xarm7": (np.array([-2.8, -1.7, -2.8, 0.0, -2.8, 0.0, -2.8] + [0.0] * 6, np.float32),
+              np.array([5.6, 3.4, 5.6, 3.0, 5.6, 3.1, 5.6] + [0.85] * 6, np.float32)),
 }
 
 CENTRE_LO = np.array([-1.0, -1.0, 0.0], np.float32)

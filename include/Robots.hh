@@ -83,4 +83,25 @@ namespace ppln::robots
             return v[i];
         }
     };
+
+    // The reference's traits declare dimension 7 (src/Robots.hh:190-193), but
+    // its generated model fkcc<Xarm7> reads q[0..12] (src/utils.cuh:38243-44112:
+    // seven arm joints and the six gripper joints) and its scale tables hold
+    // 13 entries (src/Robots.hh:196-203).  The model here follows the
+    // generated code: 13 joints, limits from those tables.
+    struct Xarm7 : RobotBase<Xarm7, 13>
+    {
+        static constexpr auto name = "xarm7";
+        static constexpr int bcc_id = BCC_XARM7;
+        static constexpr float get_s_m(int i)
+        {
+            constexpr float v[] = {5.6f, 3.4f, 5.6f, 3.0f, 5.6f, 3.1f, 5.6f, 0.85f, 0.85f, 0.85f, 0.85f, 0.85f, 0.85f};
+            return v[i];
+        }
+        static constexpr float get_s_a(int i)
+        {
+            constexpr float v[] = {-2.8f, -1.7f, -2.8f, 0.0f, -2.8f, 0.0f, -2.8f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            return v[i];
+        }
+    };
 }  // namespace ppln::robots

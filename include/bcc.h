@@ -15,7 +15,8 @@
  *     bcc_last_error() returns a thread-local message for the last failure.
  *     Nothing here prints or exits (the reference prints timings and calls
  *     exit() on CUDA errors, src/utils.cuh:252-258).
- *   - Robots: BCC_PANDA (7 DoF), BCC_FETCH (8), BCC_BAXTER (14).
+ *   - Robots: BCC_PANDA (7 DoF), BCC_FETCH (8), BCC_BAXTER (14), BCC_XARM7 (13:
+ *     seven arm joints and the six gripper joints its generated model reads).
  *   - Edges: float [num_edges][2][dim] = (start, end) joint configurations
  *     (src/batch_cc.cu:292-295).
  *   - Results: uint8 [num_edges][num_envs], 1 = the edge collides in that
@@ -42,7 +43,7 @@ enum {
     BCC_ERR_UNSUPPORTED = 4
 };
 
-enum { BCC_PANDA = 0, BCC_FETCH = 1, BCC_BAXTER = 2 };
+enum { BCC_PANDA = 0, BCC_FETCH = 1, BCC_BAXTER = 2, BCC_XARM7 = 3 };
 
 typedef struct bcc_scene bcc_scene;
 

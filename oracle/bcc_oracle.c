@@ -17,7 +17,8 @@
  *     :160-210 (cuboids)
  *   - bounding-sphere gated env and self programs of fkcc<Robot>:
  *     src/utils.cuh:814-1173 and :1197-5434 (Panda; Fetch :10382-24992,
- *     Baxter :24994-38241), as tables in csrc/robots/<robot>_model.h
+ *     Baxter :24994-38241, Xarm7 :38243-44112), as tables in
+ *     csrc/robots/<robot>_model.h
  *
  * Three evaluation modes:
  *   FP32   the project's fixed FP32 arithmetic (explicit fmaf, written to
@@ -47,6 +48,7 @@
 #include "../batch-cc_amd/csrc/robots/panda_model.h"
 #include "../batch-cc_amd/csrc/robots/fetch_model.h"
 #include "../batch-cc_amd/csrc/robots/baxter_model.h"
+#include "../batch-cc_amd/csrc/robots/xarm7_model.h"
 
 #define ORC_MAX_DIM 16
 #define ORC_MAX_SPHERES 256
@@ -83,6 +85,7 @@ const bcc_robot_model *orc_model(int robot)
     case 0: return &bcc_panda_model;
     case 1: return &bcc_fetch_model;
     case 2: return &bcc_baxter_model;
+    case 3: return &bcc_xarm7_model;
     default: return NULL;
     }
 }

@@ -6,7 +6,8 @@ Run in a container that has the reference checkout (default
 fk_<robot>.json
     Sphere centres (float64) of every sphere the reference's generated
     fkcc<Robot> tests, evaluated from the reference's own FK expressions
-    (src/utils.cuh:342-741 Panda, :10382-24992 Fetch, :24994-38241 Baxter)
+    (src/utils.cuh:342-741 Panda, :10382-24992 Fetch, :24994-38241 Baxter,
+    :38243-44112 Xarm7)
     with exact sin/cos, at 24 seeded random joint configurations that were
     NOT used to fit the robot model.  Sphere order = the committed model's
     sphere order (spheres are matched by position, so a fit that maps a
@@ -25,7 +26,9 @@ structure_<robot>.json
     numbering the committed models use.
 
 limits.json
-    Joint limits (s_a, s_m) per robot as written in src/Robots.hh:17-166.
+    Joint limits (s_a, s_m) per robot as written in src/Robots.hh:17-166
+    and :196-203 (Xarm7: 13 entries each, the joints its generated model
+    reads, although its traits declare dimension 7).
 """
 from __future__ import annotations
 
@@ -51,8 +54,8 @@ def load_extractor():
 
 
 # fkcc<Robot> bodies in src/utils.cuh (Panda :342-5442, Fetch :10382-24992,
-# Baxter :24994-38241)
-FKCC_LINES = {"panda": (342, 5442), "fetch": (10382, 24992), "baxter": (24994, 38241)}
+# Baxter :24994-38241, Xarm7 :38244-44110)
+FKCC_LINES = {"panda": (342, 5442), "fetch": (10382, 24992), "baxter": (24994, 38241), "xarm7": (38244, 44110)}
 _TOKEN = re.compile(r"\b(sphere_environment_in_collision|sphere_sphere_self_collision)\s*\(|[{}]")
 
 
@@ -123,7 +126,7 @@ def scan_program(text: str) -> dict:
 def main(ref_root="/root/reference"):
     em = load_extractor()
     src = open(os.path.join(ref_root, "src", "utils.cuh")).read()
-    for robot in ("panda", "fetch", "baxter"):
+    for robot in ("panda", "fetch", "baxter", "xarm7"):
         model = json.load(open(os.path.join(REPO, "batch-cc_amd", "robots", f"{robot}.json")))
         dim = em.ROBOT_DIM[robot]
         qname, stmts = em.parse_robot(src, robot)
@@ -176,7 +179,7 @@ def main(ref_root="/root/reference"):
     # joint limits straight from the text of src/Robots.hh
     rtext = open(os.path.join(ref_root, "src", "Robots.hh")).read()
     limits = {}
-    for robot, typ in (("panda", "Panda"), ("fetch", "Fetch"), ("baxter", "Baxter")):
+    for robot, typ in (("panda", "Panda"), ("fetch", "Fetch"), ("baxter", "Baxter"), ("xarm7", "Xarm7")):
         body = rtext[rtext.index(f"struct {typ}"):]
         sm = body[body.index("get_s_m"):]
         sm = sm[sm.index("{", sm.index("values[]")) + 1:sm.index("}", sm.index("values[]"))]

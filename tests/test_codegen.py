@@ -22,7 +22,7 @@ def gen():
     return mod
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 @pytest.mark.parametrize("order", ["tree", "program"])
 def test_schedule_covers_every_group_once(gen, robot, order):
     model = json.load(open(os.path.join(ROBOTS, f"{robot}.json")))
@@ -35,7 +35,7 @@ def test_schedule_covers_every_group_once(gen, robot, order):
     assert all(1 <= len(it) <= gen.GATE_CHUNK for kind, it in items if kind == "env")
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_tree_order_is_monotone_in_frame_depth(gen, robot):
     model = json.load(open(os.path.join(ROBOTS, f"{robot}.json")))
     rank = gen.tree_rank(model)
@@ -65,7 +65,7 @@ def test_tree_order_is_monotone_in_frame_depth(gen, robot):
 def test_generated_sources_match_the_generator(gen, tmp_path):
     """The committed check code is what the generator writes for the
     committed models (default order and chunk sizes)."""
-    for robot in ("panda", "fetch", "baxter"):
+    for robot in ("panda", "fetch", "baxter", "xarm7"):
         model = json.load(open(os.path.join(ROBOTS, f"{robot}.json")))
         committed = open(os.path.join(REPO, "batch-cc_amd", "csrc", "robots", f"{robot}_check.inc")).read()
         assert gen.gen_check(model, "tree") == committed, robot

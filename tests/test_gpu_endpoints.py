@@ -11,7 +11,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2}
+ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2, "xarm7": 3}
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
@@ -20,7 +20,7 @@ def _degenerate(edges, k):
     return np.ascontiguousarray(np.repeat(edges[:, k:k + 1], 2, axis=1))
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_endpoint_mode_equals_the_oracle_composition(bcc, oracle, gpu, robot):
     from batch_cc_amd import workloads as W
     rng = np.random.default_rng(21)

@@ -48,7 +48,7 @@ def _far_mask(radius):
     return far
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_fused_learned_step_matches_the_oracle(bcc, oracle, gpu, robot):
     from batch_cc_amd import workloads as W
     rid = bcc.ROBOTS[robot]
@@ -154,7 +154,7 @@ def _csr(rng, E, sizes):
     return off
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_learned_plan_matches_the_plain_plan(bcc, oracle, gpu, robot):
     """Ragged batches with empty environments and environments larger than a
     tile: graph replays and eager launches of a learned plan (trained-like

@@ -10,7 +10,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2}
+ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2, "xarm7": 3}
 
 
 def test_sincos_bit_exact(bcc, oracle, gpu):
@@ -72,7 +72,7 @@ def test_primitive_values_bit_exact(bcc, oracle, gpu):
             assert got[i, 1] == got[i, 2]
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_fk_sphere_centres_bit_exact(bcc, oracle, gpu, robot):
     from batch_cc_amd import workloads as W
     rng = np.random.default_rng(2)
@@ -86,7 +86,7 @@ def test_fk_sphere_centres_bit_exact(bcc, oracle, gpu, robot):
         assert np.array_equal(got[i], exp), (robot, i, np.abs(got[i] - exp).max())
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_sample_set_bit_exact(bcc, oracle, gpu, robot):
     from batch_cc_amd import workloads as W
     rng = np.random.default_rng(3)
@@ -132,7 +132,7 @@ def _sparse_scene(rng, E, n_obs, size=0.08):
     return bcc.PackedScene.from_envs(envs)
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_dense_verdicts_match_oracle_on_sparse_scenes(bcc, oracle, gpu, robot):
     """Few small obstacles: many free pairs run every step and every gated
     fine sweep, which the dense benchmark scenes rarely reach."""
@@ -234,7 +234,7 @@ def test_full_size_fetch_batch_is_deterministic_and_matches_sampled_pairs(bcc, o
     assert np.array_equal(a[pg, pe], exp)
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_both_kernel_variants_match_oracle(bcc, oracle, gpu, robot):
     """The high- and low-occupancy kernel variants (dense and ragged), forced
     in turn, give the oracle's verdicts; the automatic choice gives the same."""
@@ -260,7 +260,7 @@ def test_both_kernel_variants_match_oracle(bcc, oracle, gpu, robot):
         bcc.debug_force_variant(-1)
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_high_resolution_matches_oracle(bcc, oracle, gpu, robot):
     """Resolution 1024: uniform edges take ~100-250 steps per pair, and in
     these sparse scenes many pairs are free and run every step.  Each step adds

@@ -6,7 +6,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2}
+ROBOTS = {"panda": 0, "fetch": 1, "baxter": 2, "xarm7": 3}
 
 
 def _ragged_case(bcc, robot, E, seed, max_edges=60):
@@ -22,7 +22,7 @@ def _ragged_case(bcc, robot, E, seed, max_edges=60):
     return bcc.PackedScene.from_envs(envs), off, edges
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_ragged_matches_oracle(bcc, oracle, gpu, robot):
     sc, off, edges = _ragged_case(bcc, robot, 24, seed=1)
     pair_env = np.repeat(np.arange(sc.num_envs, dtype=np.int32), np.diff(off).astype(np.int64))

@@ -31,9 +31,9 @@ def test_library_exports_every_declared_symbol(bcc):
 
 def test_robot_tables(bcc):
     L = bcc.lib()
-    assert [L.bcc_robot_dimension(i) for i in range(3)] == [7, 8, 14]
-    assert L.bcc_robot_dimension(3) == -1
-    assert [L.bcc_robot_num_spheres(i) for i in range(3)] == [70, 125, 100]
+    assert [L.bcc_robot_dimension(i) for i in range(4)] == [7, 8, 14, 13]
+    assert L.bcc_robot_dimension(4) == -1
+    assert [L.bcc_robot_num_spheres(i) for i in range(4)] == [70, 125, 100, 74]
 
 
 def test_scene_rejects_null_pointer_with_positive_count(bcc):
@@ -73,6 +73,39 @@ def test_cpp_shim_compiles_and_throws_like_the_reference(tmp_path, bcc):
                    check=True)
     r = subprocess.run([str(exe), "nullcheck"], capture_output=True, text=True)
     assert r.returncode == 0 and "runtime_error" in r.stdout
+
+
+def test_robot_traits_header_matches_the_reference_tables(tmp_path):
+    """include/Robots.hh (the drop-in traits) against the joint-limit tables
+    read from the reference's src/Robots.hh (tests/golden/limits.json)."""
+    import json
+    src = tmp_path / "traits.cpp"
+    src.write_text("""
+#include <cstdio>
+#include "Robots.hh"
+template <class R> void dump() {
+    std::printf("%s %d", R::name, (int)R::dimension);
+    for (int i = 0; i < R::dimension; ++i) std::printf(" %.9g", (double)R::get_s_m(i));
+    for (int i = 0; i < R::dimension; ++i) std::printf(" %.9g", (double)R::get_s_a(i));
+    std::printf("\\n");
+}
+int main() {
+    dump<ppln::robots::Panda>(); dump<ppln::robots::Fetch>(); dump<ppln::robots::Baxter>();
+    dump<ppln::robots::Xarm7>();
+    return 0;
+}
+""")
+    exe = tmp_path / "traits"
+    subprocess.run(["g++", "-std=c++17", "-I" + os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    lim = json.load(open(os.path.join(REPO, "tests", "golden", "limits.json")))
+    for line in filter(None, out):
+        tok = line.split()
+        name, dim = tok[0], int(tok[1])
+        vals = np.float32([float(v) for v in tok[2:]])
+        assert dim == len(lim[name]["s_m"]), name
+        assert np.array_equal(vals[:dim], np.float32(lim[name]["s_m"])), name
+        assert np.array_equal(vals[dim:], np.float32(lim[name]["s_a"])), name
 
 
 def test_workloads_follow_reference_conventions(bcc):
@@ -124,11 +157,11 @@ def test_tuning_table_covers_every_robot():
     text = open(os.path.join(REPO, "batch-cc_amd", "csrc", "bcc_tuning.h")).read()
     body = text[text.index("kTuneTable[]"):]
     rows = re.findall(r"\{\s*(\d+),\s*(\d+),\s*(\d+)\s*\}", body)
-    assert sorted(int(r) for r, _, _ in rows) == [0, 1, 2]
+    assert sorted(int(r) for r, _, _ in rows) == [0, 1, 2, 3]
     assert all(1 <= int(t) <= 1024 and 1 <= int(g) <= 32 for _, t, g in rows)
-    thr = re.search(r"kHiVariantMinObstacles\[3\] = \{(\d+), (\d+), (\d+)\}", text)
+    thr = re.search(r"kHiVariantMinObstacles\[4\] = \{(\d+), (\d+), (\d+), (\d+)\}", text)
     assert thr and all(int(x) >= 0 for x in thr.groups())
-    hi = re.search(r"kHiVariantMaxObstacles\[3\] = \{(\d+), (\d+), (\d+)\}", text)
+    hi = re.search(r"kHiVariantMaxObstacles\[4\] = \{(\d+), (\d+), (\d+), (\d+)\}", text)
     assert hi and all(int(b) > int(a) or int(a) == int(b) == 1 << 30 for a, b in zip(thr.groups(), hi.groups()))
     cull = re.search(r"kCullMinWork = (\d+);", text)
     assert cull and int(cull.group(1)) >= 0

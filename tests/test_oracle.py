@@ -20,7 +20,7 @@ import pytest
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
-ROBOT_IDS = {"panda": 0, "fetch": 1, "baxter": 2}
+ROBOT_IDS = {"panda": 0, "fetch": 1, "baxter": 2, "xarm7": 3}
 
 
 def load(name):
@@ -46,10 +46,10 @@ def test_sincos_within_two_ulp(oracle):
 
 # ---------------------------------------------------------------- FK
 
-TOL64 = {"panda": 1e-9, "fetch": 1e-9, "baxter": 5e-7}
+TOL64 = {"panda": 1e-9, "fetch": 1e-9, "baxter": 5e-7, "xarm7": 5e-7}
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_fk_matches_reference_expressions(oracle, robot):
     g = load(f"fk_{robot}.json")
     rid = ROBOT_IDS[robot]
@@ -130,7 +130,7 @@ def _structure_mismatch(m, s):
     return None
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_check_program_structure_matches_reference_text(bcc, robot):
     s = load(f"structure_{robot}.json")
     m = _model(robot)
@@ -144,7 +144,7 @@ def test_check_program_structure_matches_reference_text(bcc, robot):
     assert _structure_mismatch(m, s) is None, _structure_mismatch(m, s)
 
 
-@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter"])
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
 def test_structure_check_catches_a_moved_fine_sphere(robot):
     """A plausible extractor slip — a fine sphere or self pair attached to the
     neighbouring gate — must fail the membership comparison."""
@@ -162,11 +162,16 @@ def test_structure_check_catches_a_moved_fine_sphere(robot):
         bad["self_program"][b]["fine"].append(bad["self_program"][a]["fine"].pop())
         assert _structure_mismatch(bad, s) is not None
     # swapping a gate's two spheres, or two fine spheres within a group, too
+    if gated:
+        bad = copy.deepcopy(m)
+        g = bad["env_program"][gated[0]]
+        if len(g["fine"]) > 1:
+            g["fine"][0], g["fine"][1] = g["fine"][1], g["fine"][0]
+            assert _structure_mismatch(bad, s) is not None
+    # exchanging two neighbouring groups (a program without gating: xarm7)
     bad = copy.deepcopy(m)
-    g = bad["env_program"][gated[0]]
-    if len(g["fine"]) > 1:
-        g["fine"][0], g["fine"][1] = g["fine"][1], g["fine"][0]
-        assert _structure_mismatch(bad, s) is not None
+    bad["env_program"][0], bad["env_program"][1] = bad["env_program"][1], bad["env_program"][0]
+    assert _structure_mismatch(bad, s) is not None
     bad = copy.deepcopy(m)
     bad["self_program"][0]["gate"] = bad["self_program"][0]["gate"][::-1]
     assert _structure_mismatch(bad, s) is not None
@@ -175,7 +180,7 @@ def test_structure_check_catches_a_moved_fine_sphere(robot):
 def test_joint_limits_match_reference_text(bcc):
     from batch_cc_amd import workloads as W
     lim = load("limits.json")
-    for robot in ("panda", "fetch", "baxter"):
+    for robot in ("panda", "fetch", "baxter", "xarm7"):
         s_a, s_m = W.LIMITS[robot]
         assert np.array_equal(s_a, np.float32(lim[robot]["s_a"]))
         assert np.array_equal(s_m, np.float32(lim[robot]["s_m"]))
