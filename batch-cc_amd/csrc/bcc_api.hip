@@ -831,14 +831,14 @@ constexpr int kLearnPlanTileDefault = 64;
 std::atomic<int> g_learned_min_steps{kLearnMinStepsDefault};
 
 // Pairs per tile of a learned launch (a wave's tile): about 16 tiles per
-// resident wave, a power of two in [32, 128] (MI355X: c1 runs best at 64 and
-// c3 at 128 of 16 ... 256).
+// resident wave, a power of two in [8, 128] (MI355X: c3 runs best at 128,
+// c1 at 64, and the 1e5-pair batches (c0, the README run) at 8 of 2 ... 256).
 int learned_tile(long long pairs, long long resident_waves)
 {
     if (const int forced = g_learned_tile.load(); forced > 0)
         return std::min(forced, bcc::kLearnTile);
     const double per = (double)pairs / (16.0 * (double)std::max<long long>(resident_waves, 1));
-    int tile = 32;
+    int tile = 8;
     while (tile < 128 && 2.0 * tile <= per)
         tile *= 2;
     return tile;

@@ -830,7 +830,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     constexpr int D = Robot::dim;
     constexpr int kBlockWaves = kBlock / 64;
     constexpr int kPairs = 64 / K;
-    static_assert(K * (kLearnPassesMax - 1) <= 4, "the predicted passes use the order word's four bins");
+    // the predicted passes use the order word's four bins, K at a time
+    static_assert(K <= 4, "a probe pass visits at most the order word's four bins");
+    constexpr int kPassesK = 1 + 4 / K < kLearnPassesMax ? 1 + 4 / K : kLearnPassesMax;
     __shared__ int next_queued[kBlockWaves];
     __shared__ int queue[kBlockWaves][kLearnTile];
     __shared__ uint32_t word[kBlockWaves][kLearnTile];
@@ -841,7 +843,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
     const int k = lane - grp * K;
     const uint64_t below = (1ull << lane) - 1ull;
     const LdsRow Q{&qs[0][threadIdx.x]};
-    const int passes = min(max(mlp.passes, 1), kLearnPassesMax);
+    const int passes = min(max(mlp.passes, 1), kPassesK);
     uint32_t probed = 0;   // the first probe's bins
 #pragma unroll
     for (int kk = 0; kk < K; ++kk)

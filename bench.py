@@ -290,16 +290,20 @@ def learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, pl
     return res
 
 
-def learned_mixed(bcc, W, a, runs, plain_ms) -> dict:
-    """Learned leg of the mixed-robot batch: one model per robot, trained as
-    for the dense configs on that robot's environments with a separate
-    lognormal edge stream (the batch's edge law); one learned step = one
-    replay of a group of learned plans (the fused kernel, persistent, per
-    robot), against the same group without models."""
+def learned_mixed(bcc, W, a, runs, plain_ms, edge_law=None) -> dict:
+    """Learned leg of a ragged or mixed-robot batch: one model per robot,
+    trained as for the dense configs on that robot's environments with a
+    separate edge stream of the batch's edge law (default: c4's lognormal
+    law); one learned step = one replay of a group of learned plans (the
+    fused kernel, persistent, per robot), against the same group without
+    models."""
     for k, r in enumerate(runs):
         rng = np.random.default_rng([a.seed, 3, k])
         n_edges = max(1, min(r["edges"].shape[0], 100_000))
-        train = W.lognormal_edges(r["robot"], n_edges, rng, median=W.MIXED_EDGE_MEDIAN)
+        if edge_law is None:
+            train = W.lognormal_edges(r["robot"], n_edges, rng, median=W.MIXED_EDGE_MEDIAN)
+        else:
+            train = edge_law(r["robot"], n_edges, rng)
         r["model"], r["train"] = train_model(bcc, W, a, r["robot"], r["scene"], r["sc"].num_envs, train, rng)
     res = {}
     legs = (("learned", True),) if a.counter_child else (("learned", True), ("no_network", False))
@@ -392,6 +396,10 @@ def counter_child(bcc, W, C, a):
         plan.launch()
         bcc.synchronize()
         plan.close()
+        if a.learned:
+            run = dict(robot=robot, sc=sc, scene=scene, off=env_off, edges=edges, d_edges=d_edges,
+                       v=d_out.to_array(np.uint8, (pairs,)))
+            learned_mixed(bcc, W, a, [run], 1.0, edge_law=W.uniform_edges)
     else:
         bcc.check_dense_device(robot, scene, d_edges, M, 32, d_out)
         bcc.synchronize()
@@ -686,6 +694,11 @@ def main():
     if a.learned and not ragged:
         learned = learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, verdicts, ms)
         learned["value"] = pairs / (learned["ms_per_step"] / 1e3)
+    elif a.learned:
+        # one learned plan over the ragged batch (edges drawn uniformly, as c2's)
+        run = dict(robot=robot, sc=sc, scene=scene, off=env_off, edges=edges, d_edges=d_edges, v=verdicts)
+        learned = learned_mixed(bcc, W, a, [run], ms, edge_law=W.uniform_edges)
+        learned["value"] = pairs / (learned["ms_per_step"] / 1e3)
 
     flops = trigs = None
     if not a.no_work:
@@ -703,7 +716,15 @@ def main():
         del work, trig, wv
         log(f"work count: {flops:.4g} flops/launch ({time.perf_counter() - t1:.1f} s)")
     # the measured step's kernel: the fused learned kernel, else the plain one
-    if learned:
+    if learned and ragged:
+        roofline = build_roofline(C, hw, cal, is_kernel("bcc::learned_kernel<"),
+                                  "learned_kernel (learned plan, hipGraph replay)", learned["ms_per_step"], flops,
+                                  trigs)
+        roofline["plain_kernel"] = build_roofline(C, hw, cal, is_kernel("bcc::ragged_kernel<"),
+                                                  "ragged_kernel (plain plan, hipGraph replay)", ms, flops, trigs)
+        roofline["plain_kernel"].pop("calibration", None)
+        learned["mfma"] = mfma_rate(hw, is_kernel("bcc::learned_kernel<"), learned["ms_per_step"])
+    elif learned:
         roofline = build_roofline(C, hw, cal, is_kernel("bcc::learned_kernel<"),
                                   "learned_kernel (the whole learned step)", learned["ms_per_step"], flops, trigs)
         roofline["plain_kernel"] = build_roofline(C, hw, cal, is_kernel("bcc::dense_kernel<"),
