@@ -128,6 +128,8 @@ def lib():
     _sig(L, "bcc_check_dense_learned", I, [I, P, P, P, I, I, P, P, P, P, P])
     _sig(L, "bcc_time_dense_learned", I, [I, P, P, P, I, I, P, I, I, P, ctypes.POINTER(ctypes.c_double)])
     _sig(L, "bcc_ragged_plan_create_learned", I, [I, P, P, P, P, I, P, I, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_ragged_plan_create_shaped", I, [I, P, P, P, I, P, I, I, I, ctypes.POINTER(P)])
+    _sig(L, "bcc_ragged_plan_tile_steps", I, [P])
     _sig(L, "bcc_debug_learned_tile", I, [I])
     _sig(L, "bcc_debug_learned_far", I, [I])
     _sig(L, "bcc_debug_learned_passes", I, [I])
@@ -418,7 +420,8 @@ class RaggedPlan:
 
     def __init__(self, robot, scene: Scene, env_off, d_edges: DeviceBuffer, d_out: DeviceBuffer,
                  resolution: int = 32, tile_pairs: int = 0, groups_per_cu: int = 0,
-                 d_order: DeviceBuffer | None = None, learned: bool = False, model: "Model | None" = None):
+                 d_order: DeviceBuffer | None = None, learned: bool = False, model: "Model | None" = None,
+                 tile_steps: int | None = None):
         self.off = np.ascontiguousarray(env_off, np.int64)
         self.robot = robot
         self.scene, self.d_edges, self.d_out, self.d_order = scene, d_edges, d_out, d_order
@@ -444,6 +447,12 @@ class RaggedPlan:
                                                         scene.handle, _ptr(self.off), d_edges.ptr, int(resolution),
                                                         d_out.ptr, int(tile_pairs), int(groups_per_cu),
                                                         ctypes.byref(self.handle)))
+        elif tile_steps is not None:
+            if d_order is not None:
+                raise BCCError("a plan with a step dimension takes no order")
+            _check(lib().bcc_ragged_plan_create_shaped(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
+                                                       int(resolution), d_out.ptr, int(tile_pairs), int(tile_steps),
+                                                       int(groups_per_cu), ctypes.byref(self.handle)))
         elif d_order is None:
             _check(lib().bcc_ragged_plan_create_tuned(robot_id(robot), scene.handle, _ptr(self.off), d_edges.ptr,
                                                       int(resolution), d_out.ptr, int(tile_pairs),
@@ -456,6 +465,11 @@ class RaggedPlan:
     @property
     def num_tiles(self) -> int:
         return lib().bcc_ragged_plan_num_tiles(self.handle)
+
+    @property
+    def tile_steps(self) -> int:
+        """Steps per tile chunk of a plain plan (0: no step dimension)."""
+        return lib().bcc_ragged_plan_tile_steps(self.handle)
 
     @property
     def config(self) -> tuple:

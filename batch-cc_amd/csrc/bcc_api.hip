@@ -1222,6 +1222,7 @@ struct bcc_ragged_plan {
     // learned plans
     bcc::LearnedArgs fused{};
     void* d_mlp = nullptr;            // the model snapshot (null: no model)
+    uint8_t* d_results = nullptr;     // plain plans with a step dimension zero-fill these first
 };
 
 struct bcc_ragged_group {
@@ -1293,11 +1294,26 @@ int device_cus()
     return cus;
 }
 
+// Fills n bytes with v (a kernel node, not a memset node, when captured).
+__global__ void fill_bytes_kernel(uint8_t* __restrict__ p, long long n, int v)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = (uint8_t)v;
+}
+
 // Every launch of a plan (captured once into its graph, or run eagerly).
 int run_plan(const bcc_ragged_plan* p, hipStream_t s)
 {
-    if (p->kind == kPlanPlain)
+    if (p->kind == kPlanPlain) {
+        if (p->args.tile_steps > 0 && p->num_pairs > 0) {
+            // chunks of one pair only ever write a collision: start from 0
+            hipLaunchKernelGGL(fill_bytes_kernel, dim3((unsigned)std::min<int64_t>((p->num_pairs + 255) / 256, 4096)),
+                               dim3(256), 0, s, p->d_results, (long long)p->num_pairs, 0);
+            if (hipError_t e = hipGetLastError(); e != hipSuccess)
+                return hip_fail(e, "verdict zero-fill");
+        }
         return dispatch_ragged(p->robot, p->args, s);
+    }
     if (p->kind == kPlanLearned) {
         hipLaunchKernelGGL(bcc::reset_counter_kernel, dim3(1), dim3(64), 0, s, p->d_counter);
         if (hipError_t e = hipGetLastError(); e != hipSuccess)
@@ -1352,6 +1368,7 @@ struct PlanBatch {
     bcc::PairSlots slots{1, 0};
     int64_t total = 0;
     double mean_obstacles = 0.0;
+    int tile_steps = 0;   // > 0: tiles carry step chunks of this many steps
 };
 
 // Captures the plan's launch sequence into its graph (the plan's buffers
@@ -1443,6 +1460,8 @@ int create_plan_kind(int robot, const bcc_scene* scene, PlanBatch& b, const floa
         p->args = bcc::RaggedArgs{scene->d_obs, scene->d_meta, d_edges, p->d_tiles, p->num_tiles, p->d_counter,
                                   resolution, grid, d_results};
         p->args.variant = variant;
+        p->args.tile_steps = b.tile_steps;
+        p->d_results = d_results;
     } else if (kind == kPlanLearned) {
         bcc::LearnedArgs& F = p->fused;
         F.obs = scene->d_obs;
@@ -1524,9 +1543,72 @@ int ragged_batch(const bcc_scene* scene, const int64_t* env_offsets, int robot, 
 }
 
 // Plain or ordered plan over one CSR layout.
+// Step counts n of every pair of a ragged batch (the arithmetic of
+// setup_pair_sd, so the same n the kernels compute), into n_out.
+__global__ void pair_steps_kernel(const float* __restrict__ edges, int dim, long long pairs, int resolution,
+                                  int* __restrict__ n_out)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < pairs;
+         p += (long long)gridDim.x * blockDim.x) {
+        const float* se = edges + p * 2 * dim;
+        float acc = 0.0f;
+        for (int j = 0; j < dim; ++j) {
+            const float diff = se[j] - se[dim + j];
+            acc = fmaf(diff, diff, acc);
+        }
+        const float dist = sqrtf(acc);
+        n_out[p] = (int)fminf(fmaxf(ceilf((dist / 31.0f) * (float)resolution), 1.0f), 4194304.0f);
+    }
+}
+
+// The step dimension of a plain plan's tiles: tile_steps > 0 splits every
+// tile whose pairs run more steps than that into chunks of tile_steps steps;
+// 0 asks the tuning table's steps per tile, applied only when the batch has
+// fewer pairs than twice the grid's half-waves (with more, the pairs alone
+// fill the GPU); < 0 never splits.  Splitting reads every pair's step count
+// from the device once, here.
+int split_steps(int robot, const float* d_edges, int resolution, int tile_steps, int groups_per_cu, PlanBatch& b)
+{
+    if (tile_steps == 0) {
+        const bcc_tune_entry& tune = tuned(robot);
+        const bcc::VariantKnobs& k = knobs(robot);
+        const int groups = groups_per_cu > 0 ? groups_per_cu : tune.groups_per_cu;
+        const int variant = choose_variant(robot, b.mean_obstacles);
+        const long long halves = (long long)device_cus() * std::min(groups, variant ? k.hi_waves : k.lo_waves) *
+                                 (bcc::kBlock / 32);
+        tile_steps = (tune.steps_per_tile > 0 && b.total < 2 * halves) ? tune.steps_per_tile : -1;
+    }
+    if (tile_steps < 0 || b.total == 0 || b.tiles.empty())
+        return BCC_OK;
+    const int dim = bcc_robot_dimension(robot);
+    int* d_n = nullptr;
+    BCC_HIP(hipMalloc((void**)&d_n, sizeof(int) * (size_t)b.total), "step counts");
+    hipLaunchKernelGGL(pair_steps_kernel, dim3((unsigned)std::min<int64_t>((b.total + 255) / 256, 4096)), dim3(256),
+                       0, 0, d_edges, dim, (long long)b.total, resolution, d_n);
+    std::vector<int> n((size_t)b.total);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpy(n.data(), d_n, sizeof(int) * (size_t)b.total, hipMemcpyDeviceToHost);
+    (void)hipFree(d_n);
+    if (e != hipSuccess)
+        return hip_fail(e, "step counts");
+    std::vector<bcc::RaggedTile> tiles;
+    for (const bcc::RaggedTile& t : b.tiles) {
+        int nmax = 1;
+        for (int i = t.begin; i < t.end; ++i)
+            nmax = std::max(nmax, n[(size_t)i]);
+        const int chunks = (nmax + tile_steps - 1) / tile_steps;
+        for (int c = 0; c < chunks; ++c)
+            tiles.push_back(bcc::RaggedTile{t.env, t.begin, t.end, c});
+    }
+    b.tiles.swap(tiles);
+    b.tile_steps = tile_steps;
+    return BCC_OK;
+}
+
 int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, const float* d_edges, int resolution,
                 bool ordered, const uint32_t* d_order, uint8_t* d_results, int tile_pairs, int groups_per_cu,
-                bcc_ragged_plan** out)
+                bcc_ragged_plan** out, int tile_steps = -1)
 {
     if (!out)
         return fail(BCC_ERR_INVALID, "out is NULL");
@@ -1542,6 +1624,9 @@ int create_plan(int robot, const bcc_scene* scene, const int64_t* env_offsets, c
         return fail(BCC_ERR_INVALID, "edges/results buffer is NULL");
     if (ordered && b.total > 0 && !d_order)
         return fail(BCC_ERR_INVALID, "d_order is NULL");
+    if (!ordered)
+        if (int rc = split_steps(robot, d_edges, resolution, tile_steps, groups_per_cu, b))
+            return rc;
     return create_plan_kind(robot, scene, b, d_edges, resolution, ordered ? kPlanOrdered : kPlanPlain, d_order,
                             d_results, tile_pairs, groups_per_cu, out);
 }
@@ -1555,7 +1640,22 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene* scene, const int64_
                                  int groups_per_cu, bcc_ragged_plan** out)
 {
     return create_plan(robot, scene, env_offsets, d_edges, resolution, false, nullptr, d_results, tile_pairs,
-                       groups_per_cu, out);
+                       groups_per_cu, out, 0);
+}
+
+int bcc_ragged_plan_create_shaped(int robot, const bcc_scene* scene, const int64_t* env_offsets,
+                                  const float* d_edges, int resolution, uint8_t* d_results, int tile_pairs,
+                                  int tile_steps, int groups_per_cu, bcc_ragged_plan** out)
+{
+    if (tile_steps < -1 || tile_steps > (1 << 22))
+        return fail(BCC_ERR_INVALID, "tile_steps must be -1, 0 or in 1..2^22");
+    return create_plan(robot, scene, env_offsets, d_edges, resolution, false, nullptr, d_results, tile_pairs,
+                       groups_per_cu, out, tile_steps);
+}
+
+int bcc_ragged_plan_tile_steps(const bcc_ragged_plan* plan)
+{
+    return plan && plan->kind == kPlanPlain ? plan->args.tile_steps : 0;
 }
 
 int bcc_ragged_plan_create_ordered(int robot, const bcc_scene* scene, const int64_t* env_offsets,

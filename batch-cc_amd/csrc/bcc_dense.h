@@ -426,16 +426,28 @@ struct RaggedTile {
     int env;
     int begin;   // first pair
     int end;     // one past the last pair
-    int pad;
+    int chunk;   // step chunk: with tile_steps > 0 the tile runs steps [chunk, chunk + 1) * tile_steps
 };
 
+// The step dimension: with tile_steps > 0 a tile covers steps [chunk *
+// tile_steps, (chunk + 1) * tile_steps) of its pairs, so a pair of n steps
+// is spread over ceil(n / tile_steps) tiles that waves take independently
+// (batches with fewer pairs than half-waves, long edges).  Such a pair's
+// samples are the same values: a chunk starts from the step-0 sample and
+// adds delta once per step skipped, the reference's own sequence
+// (src/batch_cc.cu:114-128).  Its verdict is an OR over the chunks: the
+// plan zero-fills the verdicts, a chunk writes 1 on a collision, and a chunk
+// reads the verdict before each step and stops once another chunk decided
+// the pair.  A pair that fits one chunk is decided by it alone, as without
+// the step dimension.
+//
 // Steps are taken from the edge (plain_next): on MI355X this is 4-6% faster
 // for fetch and baxter batches than an LDS increment row and equal for panda.
 template <class Robot, int kWaves, int kUnroll>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 8))) void ragged_kernel(
     const float* __restrict__ obs, const int* __restrict__ meta, const float* __restrict__ edges,
     const RaggedTile* __restrict__ tiles, int num_tiles, int* __restrict__ tile_counter, int resolution,
-    uint8_t* __restrict__ out)
+    int tile_steps, uint8_t* __restrict__ out)
 {
     constexpr int D = Robot::dim;
     constexpr int kBlockWaves = kBlock / 64;
@@ -458,27 +470,47 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
         const RaggedTile T = tiles[tile];
         const int env = __builtin_amdgcn_readfirstlane(T.env);
         const int p_end = __builtin_amdgcn_readfirstlane(T.end);
+        const int s_begin = tile_steps > 0 ? __builtin_amdgcn_readfirstlane(T.chunk) * tile_steps : 0;
         if (lane == 0)
             next_pair[wave] = T.begin;
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         const EnvView E = load_env(obs, meta, env);
-        int cur = -1, step = 0, n = 0;
+        int cur = -1, step = 0, s_end = 0;
+        bool split = false;   // the current pair spans several chunks
         bool exhausted = false;
         for (;;) {
-            const bool want = (cur < 0) && !exhausted;
-            int g = 0;
-            if (want && t == 0)
-                g = atomicAdd(&next_pair[wave], 1);
-            g = __shfl(g, half0);
-            if (want) {
-                if (g < p_end) {
-                    cur = g;
-                    step = 0;
-                    n = plain_round0<D>(edges + (size_t)g * 2 * D, resolution, t, Q);
-                } else {
+            bool want = (cur < 0) && !exhausted;
+            while (want) {   // half-uniform: take pairs until one has steps in this chunk
+                int g = 0;
+                if (t == 0)
+                    g = atomicAdd(&next_pair[wave], 1);
+                g = __shfl(g, half0);
+                if (g >= p_end) {
                     exhausted = true;
+                    break;
                 }
+                const float* se = edges + (size_t)g * 2 * D;
+                const int n = plain_round0<D>(se, resolution, t, Q);
+                if (s_begin >= n)
+                    continue;   // the pair ends before this chunk
+                cur = g;
+                step = s_begin;
+                split = tile_steps > 0 && n > tile_steps;
+                s_end = split ? min(n, s_begin + tile_steps) : n;
+                if (s_begin > 0) {
+                    // the chunk's first sample: step 0 plus s_begin increments
+                    float s0[D], d[D];
+                    setup_pair_sd<D>(se, resolution, s0, d);
+#pragma unroll
+                    for (int j = 0; j < D; ++j) {
+                        float q = Q[j];
+                        for (int i = 0; i < s_begin; ++i)
+                            q = q + d[j];
+                        Q.at(j) = q;
+                    }
+                }
+                want = false;
             }
             const uint64_t live = __ballot(cur >= 0);
             if (live == 0)
@@ -489,12 +521,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kWaves, 
                 bool fin = h;
                 if (!h) {
                     ++step;
-                    fin = (step == n);
+                    fin = (step == s_end);
+                    if (!fin && split) {
+                        // another chunk of this pair may have decided it
+                        int decided = 0;
+                        if (t == 0)
+                            decided = __atomic_load_n(out + cur, __ATOMIC_RELAXED);
+                        fin = __shfl(decided, half0) != 0;
+                    }
                     if (!fin)
                         plain_next<D>(edges + (size_t)cur * 2 * D, resolution, Q);
                 }
                 if (fin) {
-                    if (t == 0)
+                    if (t == 0 && (h || !split))
                         out[cur] = h ? 1 : 0;
                     cur = -1;
                 }
@@ -521,7 +560,8 @@ struct RaggedArgs {
     int resolution;
     int grid;        // persistent workgroups
     uint8_t* out;
-    int variant = 0;   // 1: high-occupancy kernel variant, 0: low
+    int variant = 0;      // 1: high-occupancy kernel variant, 0: low
+    int tile_steps = 0;   // > 0: tiles carry a step chunk of this many steps (verdicts zero-filled first)
 };
 
 // Kernel variant knobs of a robot: (waves per SIMD, obstacle-loop unroll) of
@@ -536,7 +576,7 @@ template <class Robot, int W, int U>
 hipError_t launch_ragged_variant(const RaggedArgs& a, hipStream_t stream)
 {
     hipLaunchKernelGGL((ragged_kernel<Robot, W, U>), dim3(a.grid), dim3(kBlock), 0, stream, a.obs, a.meta, a.edges,
-                       a.tiles, a.num_tiles, a.tile_counter, a.resolution, a.out);
+                       a.tiles, a.num_tiles, a.tile_counter, a.resolution, a.tile_steps, a.out);
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@ struct bcc_tune_entry {
     int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER / BCC_XARM7
     int tiles_per_wave;   // target tiles per persistent wave
     int groups_per_cu;    // persistent workgroups per CU
+    int steps_per_tile;   // step chunk of small batches' tiles (0: no step dimension)
 };
 
 // Exact culling of distant obstacles: a sweep of N robot spheres culls when
@@ -20,10 +21,10 @@ static const int kCullMinWork = 32;
 static const int kHiVariantMinObstacles[4] = {20, 0, 1073741824, 1073741824};
 static const int kHiVariantMaxObstacles[4] = {163, 1073741824, 1073741824, 1073741824};
 
-// source: tune.py on gfx950, 2026-12-04, seed 0
+// source: tune.py on gfx950, 2026-12-04, seed 0 (step dimension; launch shapes kept)
 static const bcc_tune_entry kTuneTable[] = {
-    {0, 8, 6},   // panda: score 1.015
-    {1, 4, 8},   // fetch: score 1.003
-    {2, 16, 4},   // baxter: score 1.000
-    {3, 16, 4},   // xarm7: baxter's setting (not tuned)
+    {0, 8, 6, 2},   // panda: score 1.015
+    {1, 4, 8, 2},   // fetch: score 1.003
+    {2, 16, 4, 0},   // baxter: score 1.000
+    {3, 16, 4, 1},   // xarm7: baxter's setting (not tuned)
 };

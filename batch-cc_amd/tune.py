@@ -15,6 +15,11 @@ Three settings, in this order:
    pairs, few environments, short pairs: see batch()) are timed as replayed
    hipGraphs for every candidate (target tiles per wave, workgroups per CU);
    the candidate with the lowest mean of time / best time goes into the table.
+4. Per robot, the step dimension of small batches' tiles: a batch with fewer
+   pairs than the grid has half-waves and long edges (see steps_batch) is
+   timed with every candidate step chunk, and without one; the fastest goes
+   into the table (0: no step dimension).  `--only-steps` runs this stage
+   alone and keeps the rest of the existing table.
 
 Every timing run must reproduce the first run's verdicts.  The library reads
 the table when a plan is created with tile_pairs = 0 / groups_per_cu = 0 and
@@ -230,6 +235,7 @@ def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:
         "    int robot;            // BCC_PANDA / BCC_FETCH / BCC_BAXTER / BCC_XARM7",
         "    int tiles_per_wave;   // target tiles per persistent wave",
         "    int groups_per_cu;    // persistent workgroups per CU",
+        "    int steps_per_tile;   // step chunk of small batches' tiles (0: no step dimension)",
         "};",
         "",
         "// Exact culling of distant obstacles: a sweep of N robot spheres culls when",
@@ -248,10 +254,67 @@ def header(table: dict, source: str, thresholds: dict, cull_min: int) -> str:
         "static const bcc_tune_entry kTuneTable[] = {",
     ]
     for robot in ROBOTS:
-        per_wave, groups, score = table[robot]
-        lines.append(f"    {{{rid[robot]}, {per_wave}, {groups}}},   // {robot}: score {score:.3f}")
+        per_wave, groups, note, steps = table[robot]
+        note = f"score {note:.3f}" if isinstance(note, float) else note
+        lines.append(f"    {{{rid[robot]}, {per_wave}, {groups}, {steps}}},   // {robot}: {note}")
     lines.append("};")
     return "\n".join(lines) + "\n"
+
+
+STEP_CANDIDATES = (-1, 1, 2, 4, 8, 16)
+
+
+def steps_batch(W, robot: str, seed: int):
+    """Few pairs, long runs: 64 environments of 16-48 mixed obstacles with
+    2..16 pairs each (about 600 pairs, fewer than the GPU's half-waves), edges
+    uniform in the joint limits, checked at resolution 128 (n ~ 20-40)."""
+    from batch_cc_amd import PackedScene
+    rs = np.random.default_rng([seed, 17, ROBOTS.index(robot)])
+    E = 64
+    counts = rs.integers(2, 17, E).astype(np.int64)
+    envs = [W.mixed_env(rs, int(rs.integers(16, 49)), capsule_frac=0.2) for _ in range(E)]
+    off = np.zeros(E + 1, np.int64)
+    off[1:] = np.cumsum(counts)
+    return PackedScene.from_envs(envs), off, W.uniform_edges(robot, int(off[-1]), rs), 128
+
+
+def tune_steps(bcc, W, robot: str, seed: int):
+    """Steps per tile for small batches: the fastest candidate (0 = none)."""
+    sc, off, edges, res = steps_batch(W, robot, seed)
+    n = edges.shape[0]
+    ms_of, ref, rows = {}, None, []
+    with bcc.Scene(sc) as scene:
+        de = bcc.DeviceBuffer.from_array(edges)
+        do = bcc.DeviceBuffer(n)
+        for c in STEP_CANDIDATES:
+            plan = bcc.RaggedPlan(robot, scene, off, de, do, resolution=res, tile_steps=c)
+            ms = plan.time(2, 10)
+            v = do.to_array(np.uint8, (n,))
+            plan.close()
+            if ref is None:
+                ref = v
+            elif not np.array_equal(v, ref):
+                raise SystemExit(f"tune: verdicts differ for {robot} steps {c}")
+            ms_of[c] = ms
+            rows.append({"robot": robot, "batch": "steps", "pairs": n, "tile_steps": c, "ms": ms})
+    best = min(ms_of, key=ms_of.get)
+    print(f"[tune] {robot} steps: {n} pairs at resolution {res}: " +
+          ", ".join(f"{c}: {ms_of[c]:.3f} ms" for c in STEP_CANDIDATES) + f" -> {max(best, 0)}", flush=True)
+    return max(best, 0), rows
+
+
+def read_table(path: str):
+    """The current table: culling threshold, variant ranges, launch shapes."""
+    import re
+    text = open(path).read()
+    cull = int(re.search(r"kCullMinWork = (\d+);", text).group(1))
+    lo = [int(x) for x in re.search(r"kHiVariantMinObstacles\[\d+\] = \{([^}]*)\}", text).group(1).split(",")]
+    hi = [int(x) for x in re.search(r"kHiVariantMaxObstacles\[\d+\] = \{([^}]*)\}", text).group(1).split(",")]
+    table = {}
+    for m in re.finditer(r"\{(\d+), (\d+), (\d+)(?:, (\d+))?\},\s*// (\w+): (.*)", text):
+        table[m.group(5)] = [int(m.group(2)), int(m.group(3)), m.group(6), int(m.group(4) or 0)]
+    thresholds = {r: (lo[i], hi[i]) for i, r in enumerate(ROBOTS)}
+    return cull, thresholds, table
 
 
 def main():
@@ -260,9 +323,26 @@ def main():
     ap.add_argument("--json", default=None, help="write every timing here")
     ap.add_argument("--quick", action="store_true", help="fewer candidates and smaller batches")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--only-steps", action="store_true", help="tune the step dimension only; keep the rest")
     a = ap.parse_args()
     bcc = load_pkg()
     from batch_cc_amd import workloads as W
+    if a.only_steps:
+        cull_min, thresholds, old = read_table(a.out)
+        table, timings = {}, []
+        for robot in ROBOTS:
+            steps, rows = tune_steps(bcc, W, robot, a.seed)
+            timings += rows
+            per_wave, groups, note, _ = old[robot]
+            table[robot] = (per_wave, groups, note, steps)
+        src = f"tune.py on gfx950, {datetime.date.today().isoformat()}, seed {a.seed} (step dimension; launch shapes kept)"
+        with open(a.out, "w") as f:
+            f.write(header(table, src, thresholds, cull_min))
+        if a.json:
+            with open(a.json, "w") as f:
+                json.dump(timings, f)
+        print(f"[tune] wrote {a.out}")
+        return
     per_waves = (8, 16, 32) if a.quick else TILES_PER_WAVE
     groups = (4, 6) if a.quick else GROUPS
     scale = 0.25 if a.quick else 1.0
@@ -315,7 +395,9 @@ def main():
             print(f"[tune] {robot} {kind}: {n} pairs, {sc.num_envs} envs, best tiles/wave {kb[0]} groups {kb[1]} "
                   f"{best:.3f} ms ({time.perf_counter() - t0:.1f} s)", flush=True)
         k = min(scores, key=scores.get)
-        table[robot] = (k[0], k[1], scores[k])
+        steps, rows = tune_steps(bcc, W, robot, a.seed)
+        timings += rows
+        table[robot] = (k[0], k[1], scores[k], steps)
         print(f"[tune] {robot}: tiles per wave {k[0]}, groups per CU {k[1]} (mean time / best {scores[k]:.3f})",
               flush=True)
     props = bcc.device_name() if hasattr(bcc, "device_name") else "gfx950"

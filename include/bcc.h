@@ -220,6 +220,19 @@ int bcc_ragged_plan_create_tuned(int robot, const bcc_scene *scene, const int64_
 int bcc_ragged_plan_create_ordered(int robot, const bcc_scene *scene, const int64_t *env_offsets,
                                    const float *d_edges, int resolution, const uint32_t *d_order,
                                    uint8_t *d_results, int tile_pairs, int groups_per_cu, bcc_ragged_plan **out);
+/* The same with the tile shape's step dimension (no reference counterpart;
+ * the reference runs a pair's steps serially in one block,
+ * src/batch_cc.cu:118-129): tile_steps > 0 splits each tile whose pairs run
+ * more steps into chunks of tile_steps steps that waves take independently
+ * (the samples are the same values; a pair's verdict is the OR of its
+ * chunks, and a chunk stops once another decided the pair); 0 applies the
+ * tuning table's chunk to batches with fewer pairs than twice the grid's
+ * half-waves; -1 never splits.  bcc_ragged_plan_tile_steps gives the chunk a
+ * plan uses (0: none).  Verdicts equal the plain plan's. */
+int bcc_ragged_plan_create_shaped(int robot, const bcc_scene *scene, const int64_t *env_offsets,
+                                  const float *d_edges, int resolution, uint8_t *d_results, int tile_pairs,
+                                  int tile_steps, int groups_per_cu, bcc_ragged_plan **out);
+int bcc_ragged_plan_tile_steps(const bcc_ragged_plan *plan);
 int bcc_ragged_plan_launch(bcc_ragged_plan *plan, void *stream);        /* graph replay */
 int bcc_ragged_plan_launch_eager(bcc_ragged_plan *plan, void *stream);  /* same kernels, no graph */
 int bcc_ragged_plan_num_tiles(const bcc_ragged_plan *plan);

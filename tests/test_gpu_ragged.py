@@ -111,3 +111,54 @@ def test_c2_full_size_graph_is_deterministic_and_matches_sampled_pairs(bcc, orac
     exp, _, _, _ = oracle.check_pairs(2, sc, edges, pair_env[idx], idx)
     assert np.array_equal(a[idx], exp)
     print(f"c2: {total} pairs, collision fraction {a.mean():.3f}")
+
+
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
+def test_step_chunks_match_the_unsplit_plan(bcc, oracle, gpu, robot):
+    """The tile shape's step dimension: pairs split over chunks of 1, 2, 3 or
+    7 steps (graph replays and eager launches, resolution 32 and 96) give the
+    unsplit plan's verdicts, which match the oracle."""
+    sc, off, edges = _ragged_case(bcc, robot, 12, seed=21, max_edges=30)
+    total = int(off[-1])
+    pair_env = np.repeat(np.arange(sc.num_envs, dtype=np.int32), np.diff(off).astype(np.int64))
+    with bcc.Scene(sc) as scene:
+        de = bcc.DeviceBuffer.from_array(edges)
+        do = bcc.DeviceBuffer(total)
+        for res in (32, 96):
+            exp, _, _, _ = oracle.check_pairs(ROBOTS[robot], sc, edges, pair_env, np.arange(total, dtype=np.int32),
+                                              res)
+            plain = bcc.RaggedPlan(robot, scene, off, de, do, resolution=res, tile_steps=-1)
+            assert plain.tile_steps == 0
+            plain.launch()
+            bcc.synchronize()
+            assert np.array_equal(do.to_array(np.uint8, (total,)), exp), (robot, res)
+            plain.close()
+            for steps in (1, 2, 3, 7):
+                plan = bcc.RaggedPlan(robot, scene, off, de, do, resolution=res, tile_steps=steps, tile_pairs=4)
+                assert plan.tile_steps == steps
+                ones = np.ones(total, np.uint8)
+                for launch in (plan.launch, plan.launch_eager, plan.launch):
+                    # stale verdicts of 1: the plan's zero-fill must clear the split pairs
+                    bcc._check(bcc.lib().bcc_memcpy_htod(do.ptr, ones.ctypes.data, total))
+                    launch()
+                    bcc.synchronize()
+                    assert np.array_equal(do.to_array(np.uint8, (total,)), exp), (robot, res, steps)
+                plan.close()
+
+
+def test_small_batches_take_the_tuned_step_chunk(bcc, oracle, gpu):
+    """tile_steps = 0: a batch with fewer pairs than the grid has half-waves
+    takes the tuning table's step chunk; verdicts equal the oracle's."""
+    sc, off, edges = _ragged_case(bcc, "panda", 6, seed=22, max_edges=10)
+    total = int(off[-1])
+    pair_env = np.repeat(np.arange(sc.num_envs, dtype=np.int32), np.diff(off).astype(np.int64))
+    exp, _, _, _ = oracle.check_pairs(0, sc, edges, pair_env, np.arange(total, dtype=np.int32), 128)
+    with bcc.Scene(sc) as scene:
+        de = bcc.DeviceBuffer.from_array(edges)
+        do = bcc.DeviceBuffer(total)
+        plan = bcc.RaggedPlan("panda", scene, off, de, do, resolution=128, tile_steps=0)
+        plan.launch()
+        bcc.synchronize()
+        print(f"{total} pairs: step chunk {plan.tile_steps}")
+        assert np.array_equal(do.to_array(np.uint8, (total,)), exp)
+        plan.close()

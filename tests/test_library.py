@@ -156,9 +156,9 @@ def test_ragged_plan_and_group_reject_bad_arguments(bcc):
 def test_tuning_table_covers_every_robot():
     text = open(os.path.join(REPO, "batch-cc_amd", "csrc", "bcc_tuning.h")).read()
     body = text[text.index("kTuneTable[]"):]
-    rows = re.findall(r"\{\s*(\d+),\s*(\d+),\s*(\d+)\s*\}", body)
-    assert sorted(int(r) for r, _, _ in rows) == [0, 1, 2, 3]
-    assert all(1 <= int(t) <= 1024 and 1 <= int(g) <= 32 for _, t, g in rows)
+    rows = re.findall(r"\{\s*(\d+),\s*(\d+),\s*(\d+),\s*(\d+)\s*\}", body)
+    assert sorted(int(r) for r, _, _, _ in rows) == [0, 1, 2, 3]
+    assert all(1 <= int(t) <= 1024 and 1 <= int(g) <= 32 and 0 <= int(st) <= 1 << 22 for _, t, g, st in rows)
     thr = re.search(r"kHiVariantMinObstacles\[4\] = \{(\d+), (\d+), (\d+), (\d+)\}", text)
     assert thr and all(int(x) >= 0 for x in thr.groups())
     hi = re.search(r"kHiVariantMaxObstacles\[4\] = \{(\d+), (\d+), (\d+), (\d+)\}", text)
