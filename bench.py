@@ -176,7 +176,9 @@ def train_model(bcc, W, a, robot, scene, E, train_edges, rng):
     t0 = time.perf_counter()
     d_x = model.features(scene, d_train, pe[keep], pg[keep])
     d_labels = bcc.DeviceBuffer.from_array(lab[keep])
-    losses = model.train(d_x, d_labels, int(keep.size), epochs=a.train_epochs, batch=8192, lr=2e-3)
+    # 64 epochs at lr 3e-3 (MI355X, c3: the second probe decides 5.04M of the
+    # 6.95M inferred pairs, against 4.90M after 8 epochs at 2e-3)
+    losses = model.train(d_x, d_labels, int(keep.size), epochs=a.train_epochs, batch=8192, lr=3e-3)
     bcc.synchronize()
     t_train = time.perf_counter() - t0
     log(f"learned {robot}: {n_big} step-0 labels in {t_label:.1f} s, {keep.size} of them miss the first probe; "
@@ -599,7 +601,7 @@ def main():
     ap.add_argument("--no-bound", dest="bound", action="store_false",
                     help="skip the perfect-bin bound (exact step-0 labels of every benchmark pair)")
     ap.add_argument("--train-pairs", type=int, default=2_000_000)
-    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--train-epochs", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel on BYTES (counter calibration) and exit")
     ap.add_argument("--no-counters", dest="counters", action="store_false",
