@@ -821,14 +821,30 @@ namespace {
 std::atomic<int> g_learned_tile{0};   // > 0: forced pairs per dense tile (tuning hook)
 std::atomic<int> g_learned_far{0};    // first-probe exclusion radius of the second probe's picks
 std::atomic<int> g_learned_passes{bcc::kLearnPassesDefault};   // probe passes (tuning hook)
-// Pairs of fewer steps skip inference and the later probes: a pair of one or
-// two steps costs less to finish than to infer and probe again (MI355X, c4's
-// parts: 3 beats 1 and 2 for every robot).
-constexpr int kLearnMinStepsDefault = 3;
-// Pairs per tile of learned ragged plans by default (MI355X, c4's parts: 64
-// beats 128 and 256 for every robot).
-constexpr int kLearnPlanTileDefault = 64;
-std::atomic<int> g_learned_min_steps{kLearnMinStepsDefault};
+// Per robot: the fewest steps a pair needs to be inferred and probed again
+// (dense launches, learned plans) and the pairs per tile of learned plans.
+// Inference pays where the pairs a first probe leaves undecided still
+// collide often and have many steps left; on short edges it does not.
+// Measured on MI355X: panda dense 3 (c3: 71.1 ms, 6: 73.3), fetch dense 6
+// (c1: 4.53 ms, 3: 4.73); learned plans of c4's parts at 64-pair tiles and 6
+// steps: panda 22.9 ms, fetch 23.9, baxter 43.5 (3 steps: 23.6, 24.5, 44.1;
+// 32-pair tiles gain another 1% there but lose 6% on c2's Zipf batch, 16-pair
+// tiles lose 8-20%).  Xarm7 takes panda's.
+struct LearnDefaults {
+    int dense_min_steps;
+    int plan_tile;
+    int plan_min_steps;
+};
+constexpr LearnDefaults kLearnDefaults[4] = {{3, 64, 6}, {6, 64, 6}, {3, 64, 6}, {3, 64, 6}};
+std::atomic<int> g_learned_min_steps{0};   // > 0: forced for every launch (tuning hook)
+
+int learn_min_steps(int robot, bool plan)
+{
+    if (const int forced = g_learned_min_steps.load(); forced > 0)
+        return forced;
+    const LearnDefaults& d = kLearnDefaults[robot >= 0 && robot < 4 ? robot : 0];
+    return plan ? d.plan_min_steps : d.dense_min_steps;
+}
 
 // Pairs per tile of a learned launch (a wave's tile): about 16 tiles per
 // resident wave, a power of two in [8, 128] (MI355X: c3 runs best at 128,
@@ -891,7 +907,7 @@ int dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const flo
     }
     m.far0 = learned_far_mask(g_learned_far.load());
     m.passes = g_learned_passes.load();
-    m.min_steps = g_learned_min_steps.load();
+    m.min_steps = learn_min_steps(robot, false);
     const int variant = choose_variant(robot, mean_obstacles(scene));
     const bcc::VariantKnobs& k = knobs(robot);
     // one workgroup per resident slot: the variant's waves per SIMD
@@ -1011,7 +1027,7 @@ int bcc_debug_learned_min_steps(int steps)
 {
     if (steps < 0)
         return fail(BCC_ERR_INVALID, "learned minimum steps must be >= 0");
-    g_learned_min_steps.store(steps == 0 ? kLearnMinStepsDefault : steps);
+    g_learned_min_steps.store(steps);
     return BCC_OK;
 }
 
@@ -1474,7 +1490,7 @@ int create_plan_kind(int robot, const bcc_scene* scene, PlanBatch& b, const floa
         F.mlp = mlp ? *mlp : bcc::MlpArgs{};
         F.mlp.far0 = learned_far_mask(g_learned_far.load());
         F.mlp.passes = g_learned_passes.load();
-        F.mlp.min_steps = g_learned_min_steps.load();
+        F.mlp.min_steps = learn_min_steps(robot, true);
         F.out = bcc::LearnedOut{b.slots, d_results, nullptr, nullptr, nullptr};
     } else {
         bcc::LearnArgs& L = p->learn;
@@ -1677,7 +1693,7 @@ int bcc_ragged_plan_create_learned(int robot, const bcc_model* model, const bcc_
         return rc;
     PlanBatch b;
     if (tile_pairs == 0)
-        tile_pairs = std::min(kLearnPlanTileDefault, bcc::kLearnTile);
+        tile_pairs = std::min(kLearnDefaults[robot].plan_tile, bcc::kLearnTile);
     // the fused learned kernel queues a tile's undecided pairs in LDS
     if (int rc = ragged_batch(scene, env_offsets, robot, tile_pairs, groups_per_cu, bcc::kLearnTile, b, &tile_pairs))
         return rc;

@@ -344,9 +344,9 @@ int bcc_time_learned_dense(bcc_model *model, const bcc_scene *scene, const float
  * of one environment and
  *   1. probes every pair's step-0 samples at 2 evenly spaced bins (8, 24);
  *   2. runs the model (BF16 MFMA) for the pairs the probe left undecided
- *      only, and of those only for pairs of at least 3 steps (shorter ones go
- *      to step 4 directly): its order word, the spread top-4 bins among the
- *      other 30;
+ *      only, and of those only for pairs of at least 3 steps (fetch: 6;
+ *      shorter ones go to step 4 directly): its order word, the spread top-4
+ *      bins among the other 30;
  *   3. probes the word's first 2 bins (with three probe passes, see
  *      bcc_debug_learned_passes, then its next 2; model NULL: the fixed bins 0
  *      and 16, then 4 and 20);
@@ -372,7 +372,9 @@ int bcc_time_dense_learned(int robot, bcc_model *model, const bcc_scene *scene, 
                            double *avg_ms);
 /* A ragged plan running the fused learned kernel persistently over its
  * tiles (one environment and at most 256 pairs each, a wave per tile;
- * tile_pairs = 0 takes 64), captured in a hipGraph like the plain plans.  It snapshots the model when it is made (later training does not
+ * tile_pairs = 0 takes 64; pairs of fewer than 6 steps skip inference),
+ * captured in a hipGraph like the plain plans.  It snapshots the model when
+ * it is made (later training does not
  * change it; model NULL: fixed second-probe bins).  Verdicts equal the plain
  * plan's.  No reference counterpart. */
 int bcc_ragged_plan_create_learned(int robot, const bcc_model *model, const bcc_scene *scene,
@@ -383,7 +385,8 @@ int bcc_ragged_plan_create_learned(int robot, const bcc_model *model, const bcc_
  * radius around the first probe's bins inside which the second probe's picks
  * are taken only when no farther bin is left (0 = no preference), and the
  * number of probe passes (1 .. 3; 0 = the default, 2), and the fewest steps
- * n a pair needs to be inferred and probed again (0 = the default, 3).  All
+ * n a pair needs to be inferred and probed again (0 = the per-robot
+ * defaults above).  All
  * are read when a launch or plan is made. */
 int bcc_debug_learned_tile(int pairs);
 int bcc_debug_learned_far(int radius);
