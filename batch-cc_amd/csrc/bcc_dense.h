@@ -131,6 +131,62 @@ __device__ __forceinline__ void plain_next(const float* __restrict__ se, int res
         Q.at(j) = Q[j] + d[j];
 }
 
+// Block order for the step loops of the learned-order kernels: step 0 is the
+// plain step 0 (lane t's first sample, the 32 evenly spaced samples that most
+// often hold a collision); step k >= 1 covers the samples 32(k-1) .. 32k - 1
+// of the remaining ones flattened as t (n-1) + (i-1), i >= 1 (lane t's i-th
+// sample), which lie next to each other along the edge, so a half-wave's
+// culling boxes (bcc_device.h) are tight and more obstacles are skipped.
+// Sample (t, i) is fmaf(delta, t n, start) plus delta added i times: the
+// plain order's own arithmetic, so every sample's bits, the sample set and
+// the verdict are the plain order's; only the visiting order differs.  Pairs
+// of more than kBlockOrderMaxSteps steps keep the plain order (the i
+// additions per lane cost O(n) a step).
+constexpr int kBlockOrderMaxSteps = 64;
+template <int D>
+__device__ __forceinline__ int block_step(const float* __restrict__ se, int resolution, int step, int t,
+                                          const LdsRow& Q)
+{
+    float s[D], d[D];
+    const int n = setup_pair_sd<D>(se, resolution, s, d);
+    if (n > kBlockOrderMaxSteps) {
+        if (step == 0) {
+            const float tn = (float)(t * n);
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                Q.at(j) = fmaf(d[j], tn, s[j]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+                Q.at(j) = Q[j] + d[j];
+        }
+        return n;
+    }
+    int tt, i;
+    if (step == 0) {
+        tt = t;
+        i = 0;
+    } else {
+        const int g = 32 * (step - 1) + t;
+        tt = g / (n - 1);
+        i = 1 + (g - tt * (n - 1));
+    }
+    const float tn = (float)(tt * n);
+    float q[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        q[j] = fmaf(d[j], tn, s[j]);
+    for (int k = 0; k < i; ++k) {
+#pragma unroll
+        for (int j = 0; j < D; ++j)
+            q[j] = q[j] + d[j];
+    }
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+        Q.at(j) = q[j];
+    return n;
+}
+
 // Plain-order pair setup into LDS rows, configuration and increment (same
 // arithmetic as setup_pair).
 template <int D>
@@ -277,10 +333,12 @@ __device__ __forceinline__ void probe_sample(const float* __restrict__ se, int r
     probe_at<D>(se, resolution, probe_bin<K>(top, k), Q);
 }
 
-// Plain step loop of one wave over a queue of pairs (the pairs a probe left
+// Step loop of one wave over a queue of pairs (the pairs a probe left
 // undecided): each half-wave takes the next queued pair from the LDS counter
-// *next.  pair_edge / pair_env / pair_out map a queue entry to its edge,
-// environment-independent verdict slot and rounds slot.
+// *next and runs its steps in plain order, or in block order after step 0
+// where the robot's kBlockSteps says so (block_step).  pair_edge / pair_env /
+// pair_out map a queue entry to its edge, environment-independent verdict
+// slot and rounds slot.
 template <class Robot, int kUnroll, class Pair, class Queue>
 __device__ __forceinline__ void plain_queue(const EnvView& E, const float* __restrict__ edges, int resolution,
                                             const Queue& queue, int nq, int* next, const LdsRow& Q, Pair pair)
@@ -303,7 +361,10 @@ __device__ __forceinline__ void plain_queue(const EnvView& E, const float* __res
                 cur = queue[qi];
                 step = 0;
                 se = edges + pair.edge(cur) * 2 * D;
-                n = plain_round0<D>(se, resolution, t, Q);
+                if constexpr (Robot::kBlockSteps)
+                    n = block_step<D>(se, resolution, 0, t, Q);
+                else
+                    n = plain_round0<D>(se, resolution, t, Q);
             } else {
                 exhausted = true;
             }
@@ -318,8 +379,12 @@ __device__ __forceinline__ void plain_queue(const EnvView& E, const float* __res
             if (!h) {
                 ++step;
                 fin = (step == n);
-                if (!fin)
-                    plain_next<D>(se, resolution, Q);
+                if (!fin) {
+                    if constexpr (Robot::kBlockSteps)
+                        block_step<D>(se, resolution, step, t, Q);
+                    else
+                        plain_next<D>(se, resolution, Q);
+                }
             }
             if (fin) {
                 if (t == 0)

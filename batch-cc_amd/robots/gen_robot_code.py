@@ -241,6 +241,12 @@ DUAL_SWEEP = {"panda": True, "fetch": True, "baxter": False, "xarm7": False}
 # first is faster for fetch (c1 -1.7%), no faster for panda and baxter
 # (within the 2% run-to-run spread).  MI355X measurements.
 ROUNDS_FROM_EDGE = {"panda": False, "fetch": True, "baxter": False, "xarm7": False}
+# The step loops of the learned-order kernels visit a pair's samples in block
+# order after step 0 (the samples of one step lie next to each other along the
+# edge, so the culling boxes are tight) or in plain order: block order is 4.5%
+# faster on c3 (panda), 5% slower on c1 and c4's part (fetch) and even on
+# baxter's batches (c2, c4).  MI355X measurements.
+BLOCK_STEPS = {"panda": True, "fetch": False, "baxter": False, "xarm7": False}
 GATE_CHUNK = 2      # defaults for robots not listed; set per robot by gen_check
 FINE_CHUNK = 4
 FIRST_CHUNK = 0     # size of the first environment chunk (0: GATE_CHUNK)
@@ -467,6 +473,8 @@ def gen_check(model: dict, order: str | None = None) -> str:
         f"    static constexpr bool kDualSweep = {'true' if DUAL_SWEEP.get(robot, True) else 'false'};",
         "    // dense kernels compute each round's samples from the edge",
         f"    static constexpr bool kRoundsFromEdge = {'true' if ROUNDS_FROM_EDGE.get(robot, True) else 'false'};",
+        "    // learned-order step loops visit steps after the first in block order",
+        f"    static constexpr bool kBlockSteps = {'true' if BLOCK_STEPS.get(robot, False) else 'false'};",
         "    // Collision predicate for one sample per lane.  Lanes form aligned groups of",
         "    // G (32: a half-wave, the reference's 32 samples of one pair per step; fewer:",
         "    // several pairs per half-wave, G samples each).  `live` holds the lanes whose",

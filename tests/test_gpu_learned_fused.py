@@ -92,6 +92,46 @@ def test_fused_learned_step_matches_the_oracle(bcc, oracle, gpu, robot):
             bcc.debug_learned_min_steps(0)
 
 
+@pytest.mark.parametrize("robot", ["panda", "fetch"])
+def test_step_loop_orders_match_the_oracle_at_high_resolution(bcc, oracle, gpu, robot):
+    """The learned-order kernels' step loops (panda: block order after step 0,
+    plain order beyond 64 steps; fetch: plain order) at resolutions 32, 256
+    and 1024, where pairs run up to ~200 steps: verdicts equal the oracle's
+    at that resolution for the fused learned step and the ordered kernel, and
+    a free pair still runs every one of its n steps."""
+    from batch_cc_amd import workloads as W
+    rid = bcc.ROBOTS[robot]
+    rng = np.random.default_rng(71 + rid)
+    envs, edges = _scene(W, rng, robot, 6, 30, n_cub=2, n_sph=1, n_cyl=1)
+    sc = bcc.PackedScene.from_envs(envs)
+    M, E = edges.shape[0], sc.num_envs
+    model = _biased_model(bcc, robot, 5)
+    try:
+        with bcc.Scene(sc) as scene:
+            de = bcc.DeviceBuffer.from_array(edges)
+            do = bcc.DeviceBuffer(E * M)
+            dr = bcc.DeviceBuffer(E * M)
+            order = rng.permuted(np.tile(np.arange(32), (M * E, 1)), axis=1)[:, :4].astype(np.uint32)
+            d_order = bcc.DeviceBuffer.from_array(order[:, 0] | order[:, 1] << 8 | order[:, 2] << 16 | order[:, 3] << 24)
+            bcc.debug_learned_min_steps(1)
+            for res in (32, 256, 1024):
+                exp = oracle.check_dense(rid, sc, edges, res).astype(bool)
+                n = np.array([oracle.steps(edges[m, 0], edges[m, 1], res)[0] for m in range(M)])
+                for m in (model, None):
+                    r = bcc.check_dense_learned(robot, m, scene, de, M, resolution=res, stage=True)
+                    assert np.array_equal(r["verdicts"], exp), (res, m is None)
+                    st = r["stage"].astype(np.int64)
+                    # passes = 2: a free pair's stage is 1 + n (capped at 255)
+                    free_stage = np.minimum(np.broadcast_to(n[:, None] + 1, exp.shape), 255)
+                    assert np.all(st[~exp] == free_stage[~exp]), res
+                bcc.check_dense_ex(robot, scene, de, M, res, do, d_order, dr)
+                assert np.array_equal(do.to_array(np.uint8, (M, E)).astype(bool), exp), res
+            print(f"{robot}: steps up to {n.max()} at resolution 1024, collision fraction {exp.mean():.3f}")
+    finally:
+        bcc.debug_learned_min_steps(0)
+        model.close()
+
+
 @pytest.mark.parametrize("radius", [0, 4])
 def test_stages_follow_the_step0_labels_and_the_inferred_words(bcc, gpu, radius):
     """Stage 0 exactly when a first-probe bin's step-0 sample collides; the
