@@ -134,6 +134,7 @@ def lib():
     _sig(L, "bcc_debug_learned_far", I, [I])
     _sig(L, "bcc_debug_learned_passes", I, [I])
     _sig(L, "bcc_debug_learned_min_steps", I, [I])
+    _sig(L, "bcc_debug_learned_locality", I, [I])
     _lib = L
     return L
 
@@ -829,6 +830,11 @@ def debug_learned_passes(passes: int):
 
 def debug_learned_min_steps(steps: int):
     _check(lib().bcc_debug_learned_min_steps(int(steps)))
+
+
+def debug_learned_locality(mode: int):
+    """0 never, 1 launches of 2^22 pairs or more (the default), 2 always."""
+    _check(lib().bcc_debug_learned_locality(int(mode)))
 
 
 def debug_gemm(ta: bool, tb: bool, a: np.ndarray, b: np.ndarray, m: int, n: int, k: int, alpha=1.0, beta=0.0,

@@ -7,6 +7,7 @@
 // errors are returned as status codes with a thread-local message instead of
 // exceptions/exit(), and nothing is printed.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 #include <cstdio>
@@ -837,6 +838,7 @@ struct LearnDefaults {
 };
 constexpr LearnDefaults kLearnDefaults[4] = {{3, 64, 6}, {6, 64, 6}, {3, 64, 6}, {3, 64, 6}};
 std::atomic<int> g_learned_min_steps{0};   // > 0: forced for every launch (tuning hook)
+std::atomic<int> g_learned_locality{1};    // locality order: 0 never, 1 large launches, 2 always
 
 int learn_min_steps(int robot, bool plan)
 {
@@ -889,6 +891,108 @@ int dispatch_learned(int robot, const bcc::LearnedArgs& a, hipStream_t s)
     return BCC_OK;
 }
 
+// Locality order of a dense learned launch: the first probe takes 32 pairs
+// of one environment per batch, and its culling boxes (bcc_device.h) bound
+// the robot spheres of all of them, so pairs whose arms point the same way
+// should share a batch.  Key of an edge: the Morton interleave of the two
+// joints that place the arm (panda, baxter, xarm7: 0 and 1; fetch: 1 and 2,
+// after the torso) at the first probe's first bin (8/31 of the way), 10 bits
+// each over the joint's range.  The launch checks the edges gathered in key
+// order and scatters each result row back to its edge, so every output is
+// where it would be without it.  Only for launches of at least
+// kLocalityMinPairs pairs (sorting and the row copies cost ~50 us).
+constexpr int kOrderBits = 10;
+constexpr long long kLocalityMinPairs = 1LL << 22;
+__global__ void locality_key_kernel(const float* __restrict__ edges, int dim, int m, int j0, int j1, float lo0,
+                                    float inv0, float lo1, float inv1, uint32_t* __restrict__ keys,
+                                    int* __restrict__ idx)
+{
+    const float top = (float)((1 << kOrderBits) - 1);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
+        const float* se = edges + (size_t)i * 2 * dim;
+        const float f = 8.0f / 31.0f;
+        const float q0 = se[j0] + (se[dim + j0] - se[j0]) * f;
+        const float q1 = se[j1] + (se[dim + j1] - se[j1]) * f;
+        // NaN compares false and lands on 0: any key is a valid order
+        const uint32_t u0 = (uint32_t)(fminf(fmaxf((q0 - lo0) * inv0, 0.0f), 1.0f) * top);
+        const uint32_t u1 = (uint32_t)(fminf(fmaxf((q1 - lo1) * inv1, 0.0f), 1.0f) * top);
+        uint32_t key = 0;
+        for (int b = 0; b < kOrderBits; ++b)
+            key |= (((u0 >> b) & 1u) << (2 * b)) | (((u1 >> b) & 1u) << (2 * b + 1));
+        keys[i] = key;
+        idx[i] = i;
+    }
+}
+
+// Row p of dst = row perm[p] of src (gather), or row perm[p] of dst = row p
+// of src (scatter); rows of `width` elements.
+template <class T>
+__global__ void permute_rows_kernel(const T* __restrict__ src, T* __restrict__ dst, const int* __restrict__ perm,
+                                   long long rows, int width, int scatter)
+{
+    const long long total = rows * width;
+    for (long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x; x < total;
+         x += (long long)gridDim.x * blockDim.x) {
+        const long long p = x / width;
+        const long long c = x - p * width;
+        const long long q = perm[p];
+        if (scatter)
+            dst[q * width + c] = src[x];
+        else
+            dst[x] = src[q * width + c];
+    }
+}
+
+template <class T>
+void permute_rows(const T* src, T* dst, const int* perm, long long rows, int width, bool scatter, hipStream_t s)
+{
+    const long long total = rows * width;
+    const int blocks = (int)std::min<long long>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL((permute_rows_kernel<T>), dim3(blocks), dim3(256), 0, s, src, dst, perm, rows, width,
+                       scatter ? 1 : 0);
+}
+
+// The edges' locality order on stream s: *order (sorted position -> edge
+// row) points into *alloc, a stream-ordered allocation the caller frees with
+// hipFreeAsync after the launches that read it.
+int locality_order(int robot, const float* d_edges, int num_edges, hipStream_t s, void** alloc, const int** order)
+{
+    *alloc = nullptr;
+    *order = nullptr;
+    const int dim = bcc_robot_dimension(robot);
+    const int j0 = robot == BCC_FETCH ? 1 : 0;
+    const int j1 = j0 + 1;
+    float lo0, r0, lo1, r1;
+    bcc_detail::joint_range(robot, j0, &lo0, &r0);
+    bcc_detail::joint_range(robot, j1, &lo1, &r1);
+    size_t temp = 0;
+    BCC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (uint32_t*)nullptr, (uint32_t*)nullptr, (int*)nullptr,
+                                               (int*)nullptr, num_edges, 0, 2 * kOrderBits, s),
+            "locality sort size");
+    // keys in, keys out, indices in, indices out, then the sort's workspace
+    const size_t part = ((size_t)num_edges * 4 + 255) / 256 * 256;
+    char* buf = nullptr;
+    BCC_HIP(hipMallocAsync((void**)&buf, 4 * part + temp, s), "locality order buffers");
+    uint32_t* keys = (uint32_t*)buf;
+    uint32_t* keys_out = (uint32_t*)(buf + part);
+    int* idx = (int*)(buf + 2 * part);
+    int* idx_out = (int*)(buf + 3 * part);
+    const int blocks = (int)std::min<long long>(((long long)num_edges + 255) / 256, 4096);
+    hipLaunchKernelGGL(locality_key_kernel, dim3(blocks), dim3(256), 0, s, d_edges, dim, num_edges, j0, j1, lo0,
+                       1.0f / r0, lo1, 1.0f / r1, keys, idx);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(buf + 4 * part, temp, keys, keys_out, idx, idx_out, num_edges, 0,
+                                               2 * kOrderBits, s);
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(buf, s);
+        return hip_fail(e, "locality order");
+    }
+    *alloc = buf;
+    *order = idx_out;
+    return BCC_OK;
+}
+
 // One dense learned step on stream s: the model's operands (fragments and
 // environment codes, two small kernels), then the fused learned kernel,
 // persistent over the dense tiles (a wave that ends a tile takes the next
@@ -917,25 +1021,75 @@ int dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const flo
     const long long tiles = (long long)scene->num_envs * chunks;
     if (tiles > 0x7fffffffLL - 65536)
         return fail(BCC_ERR_INVALID, "too many tiles for one dense learned launch");
+    // locality order: the launch reads gathered edges and writes its rows
+    // to staging buffers that are scattered back afterwards
+    const int E = scene->num_envs;
+    const int width = 2 * bcc_robot_dimension(robot);
+    void* order_alloc = nullptr;
+    const int* order = nullptr;
+    char* stage_buf = nullptr;
+    const float* edges_in = d_edges;
+    uint8_t* res_out = d_results;
+    uint8_t* stage_out = d_stage;
+    uint32_t* words_out = d_words;
+    const int locality = g_learned_locality.load();
+    if (num_edges >= 2 && (locality == 2 || (locality == 1 && (long long)E * num_edges >= kLocalityMinPairs))) {
+        if (int rc = locality_order(robot, d_edges, num_edges, s, &order_alloc, &order))
+            return rc;
+        const size_t pairs = (size_t)E * num_edges;
+        const size_t eb = ((size_t)num_edges * width * sizeof(float) + 255) / 256 * 256;
+        const size_t rb = (pairs + 255) / 256 * 256;
+        const size_t bytes = eb + rb + (d_stage ? rb : 0) + (d_words ? pairs * sizeof(uint32_t) : 0);
+        if (hipError_t e = hipMallocAsync((void**)&stage_buf, bytes, s); e != hipSuccess) {
+            (void)hipFreeAsync(order_alloc, s);
+            return hip_fail(e, "locality staging buffers");
+        }
+        float* sorted = (float*)stage_buf;
+        res_out = (uint8_t*)(stage_buf + eb);
+        stage_out = d_stage ? (uint8_t*)(stage_buf + eb + rb) : nullptr;
+        words_out = d_words ? (uint32_t*)(stage_buf + eb + rb + (d_stage ? rb : 0)) : nullptr;
+        permute_rows<float>(d_edges, sorted, order, num_edges, width, false, s);
+        // words the launch does not write keep the caller's values
+        if (d_words)
+            permute_rows<uint32_t>(d_words, words_out, order, num_edges, E, false, s);
+        edges_in = sorted;
+    }
     int* d_counter = nullptr;
-    BCC_HIP(hipMallocAsync((void**)&d_counter, sizeof(int), s), "learned tile counter");
+    if (hipError_t e = hipMallocAsync((void**)&d_counter, sizeof(int), s); e != hipSuccess) {
+        (void)hipFreeAsync(stage_buf, s);
+        (void)hipFreeAsync(order_alloc, s);
+        return hip_fail(e, "learned tile counter");
+    }
     hipLaunchKernelGGL(bcc::reset_counter_kernel, dim3(1), dim3(64), 0, s, d_counter);
     bcc::LearnedArgs a{};
     a.obs = scene->d_obs;
     a.meta = scene->d_meta;
-    a.edges = d_edges;
+    a.edges = edges_in;
     a.tiles = bcc::LearnTiles{nullptr, (int)tiles, d_counter, chunks, num_edges, tile};
     a.resolution = resolution;
     a.variant = variant;
     a.grid = (int)std::min<long long>(slots, (tiles + bcc::kBlock / 64 - 1) / (bcc::kBlock / 64));
     a.mlp = m;
-    a.out = bcc::LearnedOut{bcc::PairSlots{scene->num_envs, 1}, d_results, d_stage, d_words, d_stats};
+    a.out = bcc::LearnedOut{bcc::PairSlots{scene->num_envs, 1}, res_out, stage_out, words_out, d_stats};
     int rc = BCC_OK;
     if (hipError_t e = hipGetLastError(); e != hipSuccess)
         rc = hip_fail(e, "learned tile counter reset");
     if (rc == BCC_OK)
         rc = dispatch_learned(robot, a, s);
+    if (rc == BCC_OK && order) {
+        permute_rows<uint8_t>(res_out, d_results, order, num_edges, E, true, s);
+        if (d_stage)
+            permute_rows<uint8_t>(stage_out, d_stage, order, num_edges, E, true, s);
+        if (d_words)
+            permute_rows<uint32_t>(words_out, d_words, order, num_edges, E, true, s);
+        if (hipError_t e = hipGetLastError(); e != hipSuccess)
+            rc = hip_fail(e, "locality scatter");
+    }
     (void)hipFreeAsync(d_counter, s);
+    if (stage_buf)
+        (void)hipFreeAsync(stage_buf, s);
+    if (order_alloc)
+        (void)hipFreeAsync(order_alloc, s);
     return rc;
 }
 
@@ -1028,6 +1182,14 @@ int bcc_debug_learned_min_steps(int steps)
     if (steps < 0)
         return fail(BCC_ERR_INVALID, "learned minimum steps must be >= 0");
     g_learned_min_steps.store(steps);
+    return BCC_OK;
+}
+
+int bcc_debug_learned_locality(int mode)
+{
+    if (mode < 0 || mode > 2)
+        return fail(BCC_ERR_INVALID, "locality must be 0, 1 or 2");
+    g_learned_locality.store(mode);
     return BCC_OK;
 }
 

@@ -42,6 +42,9 @@ int model_mlp_args(bcc_model* model, const bcc_scene* scene, hipStream_t s, bcc:
 // synchronised: a snapshot of the model as it is now, for plans.
 int model_mlp_snapshot(const bcc_model* model, const bcc_scene* scene, bcc::MlpArgs* out, void** owned);
 
+// Lower limit and range of joint j of a robot (the model's feature scaling).
+void joint_range(int robot, int j, float* lo, float* range);
+
 }  // namespace bcc_detail
 
 #define BCC_HIP(call, what)                                  \

@@ -943,6 +943,16 @@ int bcc_model_logits(bcc_model* M, const uint16_t* d_x, int64_t rows, float* d_l
 
 namespace bcc_detail {
 
+void joint_range(int robot, int j, float* lo, float* range)
+{
+    *lo = kLo[robot][j];
+    *range = kRange[robot][j];
+}
+
+}  // namespace bcc_detail
+
+namespace bcc_detail {
+
 int model_robot(const bcc_model* M)
 {
     return M->robot;

@@ -132,6 +132,38 @@ def test_step_loop_orders_match_the_oracle_at_high_resolution(bcc, oracle, gpu, 
         model.close()
 
 
+@pytest.mark.parametrize("robot", ["panda", "fetch"])
+def test_locality_order_changes_no_per_pair_output(bcc, oracle, gpu, robot):
+    """Dense learned launches check the edges in a locality order sorted on
+    the device and scatter the results back (forced here with mode 2; large
+    launches by default); every per-pair output (verdict, stage, inferred
+    word) is the same as in the given order, and the verdicts are the
+    oracle's."""
+    from batch_cc_amd import workloads as W
+    rid = bcc.ROBOTS[robot]
+    rng = np.random.default_rng(91 + rid)
+    envs, edges = _scene(W, rng, robot, 7, 300)
+    sc = bcc.PackedScene.from_envs(envs)
+    M = edges.shape[0]
+    exp = oracle.check_dense(rid, sc, edges).astype(bool)
+    model = _biased_model(bcc, robot, 9)
+    try:
+        with bcc.Scene(sc) as scene:
+            de = bcc.DeviceBuffer.from_array(edges)
+            bcc.debug_learned_min_steps(1)
+            out = {}
+            for on in (0, 2):
+                bcc.debug_learned_locality(on)
+                out[on] = bcc.check_dense_learned(robot, model, scene, de, M, stage=True, words=True, stats=True)
+                assert np.array_equal(out[on]["verdicts"], exp), on
+            for key in ("stage", "words", "stats"):
+                assert np.array_equal(out[0][key], out[2][key]), key
+    finally:
+        bcc.debug_learned_locality(1)
+        bcc.debug_learned_min_steps(0)
+        model.close()
+
+
 @pytest.mark.parametrize("radius", [0, 4])
 def test_stages_follow_the_step0_labels_and_the_inferred_words(bcc, gpu, radius):
     """Stage 0 exactly when a first-probe bin's step-0 sample collides; the

@@ -207,6 +207,9 @@ def test_tuning_hooks_reject_out_of_range(bcc):
     assert L.bcc_debug_learned_passes(0) == 0
     assert L.bcc_debug_learned_min_steps(-1) == 1
     assert L.bcc_debug_learned_min_steps(0) == 0
+    assert L.bcc_debug_learned_locality(3) == 1
+    assert b"locality" in L.bcc_last_error()
+    assert L.bcc_debug_learned_locality(1) == 0
 
 
 def test_tuner_variant_range_selection():
