@@ -833,7 +833,7 @@ def debug_learned_min_steps(steps: int):
 
 
 def debug_learned_locality(mode: int):
-    """0 never, 1 launches of 2^22 pairs or more (the default), 2 always."""
+    """0 never, 1 panda launches of 2^22 pairs or more (the default), 2 always."""
     _check(lib().bcc_debug_learned_locality(int(mode)))
 
 

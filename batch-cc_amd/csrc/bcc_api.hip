@@ -831,14 +831,19 @@ std::atomic<int> g_learned_passes{bcc::kLearnPassesDefault};   // probe passes (
 // steps: panda 22.9 ms, fetch 23.9, baxter 43.5 (3 steps: 23.6, 24.5, 44.1;
 // 32-pair tiles gain another 1% there but lose 6% on c2's Zipf batch, 16-pair
 // tiles lose 8-20%).  Xarm7 takes panda's.
+// The locality order of large dense launches (below) is on for panda only:
+// c3 61.0 vs 66.7 ms; fetch's c1 ran 4.66 vs 4.50 ms with it (its sort and
+// row copies cost more than its first probe gains); baxter and xarm7 have no
+// dense benchmark batch to decide on.
 struct LearnDefaults {
     int dense_min_steps;
     int plan_tile;
     int plan_min_steps;
+    int locality;
 };
-constexpr LearnDefaults kLearnDefaults[4] = {{3, 64, 6}, {6, 64, 6}, {3, 64, 6}, {3, 64, 6}};
+constexpr LearnDefaults kLearnDefaults[4] = {{3, 64, 6, 1}, {6, 64, 6, 0}, {3, 64, 6, 0}, {3, 64, 6, 0}};
 std::atomic<int> g_learned_min_steps{0};   // > 0: forced for every launch (tuning hook)
-std::atomic<int> g_learned_locality{1};    // locality order: 0 never, 1 large launches, 2 always
+std::atomic<int> g_learned_locality{1};    // locality order: 0 never, 1 per-robot default, 2 always
 
 int learn_min_steps(int robot, bool plan)
 {
@@ -896,12 +901,16 @@ int dispatch_learned(int robot, const bcc::LearnedArgs& a, hipStream_t s)
 // the robot spheres of all of them, so pairs whose arms point the same way
 // should share a batch.  Key of an edge: the Morton interleave of the two
 // joints that place the arm (panda, baxter, xarm7: 0 and 1; fetch: 1 and 2,
-// after the torso) at the first probe's first bin (8/31 of the way), 10 bits
-// each over the joint's range.  The launch checks the edges gathered in key
+// after the torso) at both first-probe samples (8/31 and 24/31 of the way),
+// 8 bits each over the joint's range (MI355X, c3 pre-sorted by candidate
+// keys, learned step without a model: these four 58.6 ms, the two joints at
+// bin 8 alone 60.0, three joints at bin 8 59.4, unsorted 64.4).  The launch
+// checks the edges gathered in key
 // order and scatters each result row back to its edge, so every output is
 // where it would be without it.  Only for launches of at least
-// kLocalityMinPairs pairs (sorting and the row copies cost ~50 us).
-constexpr int kOrderBits = 10;
+// kLocalityMinPairs pairs (sorting and the row copies cost ~0.1 ms) of the
+// robots whose LearnDefaults ask for it.
+constexpr int kOrderBits = 8;   // per coordinate; four coordinates
 constexpr long long kLocalityMinPairs = 1LL << 22;
 __global__ void locality_key_kernel(const float* __restrict__ edges, int dim, int m, int j0, int j1, float lo0,
                                     float inv0, float lo1, float inv1, uint32_t* __restrict__ keys,
@@ -910,15 +919,19 @@ __global__ void locality_key_kernel(const float* __restrict__ edges, int dim, in
     const float top = (float)((1 << kOrderBits) - 1);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x) {
         const float* se = edges + (size_t)i * 2 * dim;
-        const float f = 8.0f / 31.0f;
-        const float q0 = se[j0] + (se[dim + j0] - se[j0]) * f;
-        const float q1 = se[j1] + (se[dim + j1] - se[j1]) * f;
-        // NaN compares false and lands on 0: any key is a valid order
-        const uint32_t u0 = (uint32_t)(fminf(fmaxf((q0 - lo0) * inv0, 0.0f), 1.0f) * top);
-        const uint32_t u1 = (uint32_t)(fminf(fmaxf((q1 - lo1) * inv1, 0.0f), 1.0f) * top);
+        uint32_t u[4];
+        for (int c = 0; c < 4; ++c) {
+            const float f = (c < 2 ? 8.0f : 24.0f) / 31.0f;
+            const int j = (c & 1) ? j1 : j0;
+            const float lo = (c & 1) ? lo1 : lo0, inv = (c & 1) ? inv1 : inv0;
+            const float q = se[j] + (se[dim + j] - se[j]) * f;
+            // NaN compares false and lands on 0: any key is a valid order
+            u[c] = (uint32_t)(fminf(fmaxf((q - lo) * inv, 0.0f), 1.0f) * top);
+        }
         uint32_t key = 0;
         for (int b = 0; b < kOrderBits; ++b)
-            key |= (((u0 >> b) & 1u) << (2 * b)) | (((u1 >> b) & 1u) << (2 * b + 1));
+            for (int c = 0; c < 4; ++c)
+                key |= ((u[c] >> b) & 1u) << (4 * b + c);
         keys[i] = key;
         idx[i] = i;
     }
@@ -967,7 +980,7 @@ int locality_order(int robot, const float* d_edges, int num_edges, hipStream_t s
     bcc_detail::joint_range(robot, j1, &lo1, &r1);
     size_t temp = 0;
     BCC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, (uint32_t*)nullptr, (uint32_t*)nullptr, (int*)nullptr,
-                                               (int*)nullptr, num_edges, 0, 2 * kOrderBits, s),
+                                               (int*)nullptr, num_edges, 0, 4 * kOrderBits, s),
             "locality sort size");
     // keys in, keys out, indices in, indices out, then the sort's workspace
     const size_t part = ((size_t)num_edges * 4 + 255) / 256 * 256;
@@ -983,7 +996,7 @@ int locality_order(int robot, const float* d_edges, int num_edges, hipStream_t s
     hipError_t e = hipGetLastError();
     if (e == hipSuccess)
         e = hipcub::DeviceRadixSort::SortPairs(buf + 4 * part, temp, keys, keys_out, idx, idx_out, num_edges, 0,
-                                               2 * kOrderBits, s);
+                                               4 * kOrderBits, s);
     if (e != hipSuccess) {
         (void)hipFreeAsync(buf, s);
         return hip_fail(e, "locality order");
@@ -1033,7 +1046,8 @@ int dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const flo
     uint8_t* stage_out = d_stage;
     uint32_t* words_out = d_words;
     const int locality = g_learned_locality.load();
-    if (num_edges >= 2 && (locality == 2 || (locality == 1 && (long long)E * num_edges >= kLocalityMinPairs))) {
+    if (num_edges >= 2 && (locality == 2 || (locality == 1 && kLearnDefaults[robot].locality &&
+                                             (long long)E * num_edges >= kLocalityMinPairs))) {
         if (int rc = locality_order(robot, d_edges, num_edges, s, &order_alloc, &order))
             return rc;
         const size_t pairs = (size_t)E * num_edges;

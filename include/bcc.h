@@ -343,8 +343,8 @@ int bcc_time_learned_dense(bcc_model *model, const bcc_scene *scene, const float
  * visited).  One kernel does the whole step: each wave owns a tile of pairs
  * of one environment and
  *   1. probes every pair's step-0 samples at 2 evenly spaced bins (8, 24)
- *      (launches of 2^22 pairs or more: the edges in a locality order sorted
- *      on the device, the results scattered back to their rows);
+ *      (panda launches of 2^22 pairs or more: the edges in a locality order
+ *      sorted on the device, the results scattered back to their rows);
  *   2. runs the model (BF16 MFMA) for the pairs the probe left undecided
  *      only, and of those only for pairs of at least 3 steps (fetch: 6;
  *      shorter ones go to step 4 directly): its order word, the spread top-4
@@ -394,12 +394,12 @@ int bcc_debug_learned_tile(int pairs);
 int bcc_debug_learned_far(int radius);
 int bcc_debug_learned_passes(int passes);
 int bcc_debug_learned_min_steps(int steps);
-/* Tuning hook (no reference counterpart): dense learned launches of at
- * least 2^22 pairs check the edges in a locality order (sorted on the device
- * by where the first probe places the arm, so each probe batch's culling
- * boxes are small) and scatter the results back: mode 1 (the default); 0 =
- * never, 2 = every launch.  Verdicts and per-pair outputs never depend on
- * it. */
+/* Tuning hook (no reference counterpart): panda's dense learned launches
+ * of at least 2^22 pairs check the edges in a locality order (sorted on the
+ * device by where the first probe places the arm, so each probe batch's
+ * culling boxes are small) and scatter the results back: mode 1 (the
+ * default); 0 = never, 2 = every launch of every robot.  Verdicts and
+ * per-pair outputs never depend on it. */
 int bcc_debug_learned_locality(int mode);
 
 /* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
