@@ -655,7 +655,14 @@ def encode_envs(scene: Scene) -> np.ndarray:
 
 class Model:
     """The learned step-ordering MLP (bcc_model)."""
-    IN, HID, OUT = 96, 256, 32
+    IN, OUT = 96, 32
+
+    @staticmethod
+    def hidden() -> int:
+        """Hidden width H of the compiled model (96 -> H -> H -> 32), from its
+        parameter count H^2 + (IN + OUT + 2) H + OUT."""
+        n, b = Model.num_params(), Model.IN + Model.OUT + 2
+        return int(round((-b + (b * b + 4 * (n - Model.OUT)) ** 0.5) / 2))
 
     def __init__(self, robot=None, seed: int = 0, handle=None):
         self.handle = ctypes.c_void_p()

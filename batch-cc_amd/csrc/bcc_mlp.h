@@ -79,7 +79,11 @@ __device__ __forceinline__ bf16x8 load_slots(const uint16_t* row, int g)
 // ------------------------------------------------------------ model shape
 constexpr int kEnc = 64;      // occupancy cells (4 x 4 x 4)
 constexpr int kIn = 96;       // padded input width: 64 + 2 * dim (dim <= 16)
-constexpr int kHid = 256;
+#ifndef BCC_MLP_HIDDEN
+#define BCC_MLP_HIDDEN 64
+#endif
+constexpr int kHid = BCC_MLP_HIDDEN;   // hidden width, a multiple of 32
+static_assert(kHid % 32 == 0 && kHid >= 32, "hidden width: whole 32-wide k blocks");
 constexpr int kOut = 32;
 
 // Weight fragments of the fused inference kernels: fragment (rt, kt) of a
@@ -216,9 +220,9 @@ __device__ __forceinline__ u32x4 load16(__amdgpu_buffer_rsrc_t r, uint32_t voff,
 // same bias, ReLU and BF16 roundings, so the logits are bit-identical to it
 // and to the layer-by-layer path.  Lane l ends with z[4 r3 + v] = logit of
 // bin 16 r3 + 4 (l/16) + v of the pair in column l % 16.  Weight fragments
-// are read straight from memory (L2-resident, 192 KiB) through a buffer
-// resource; activations stay in registers: layer 1's 256 x 16 BF16 outputs
-// are 32 VGPRs, layer 2 is produced two row tiles at a time and consumed at
+// are read straight from memory (L2-resident, 2 kFrags KiB) through a buffer
+// resource; activations stay in registers: layer 1's kHid x 16 BF16 outputs
+// are kHid / 8 VGPRs, layer 2 is produced two row tiles at a time and consumed at
 // once by layer 3.  Every lane of the wave must be active (MFMA).
 __device__ __forceinline__ void mlp_tile_logits(const bf16x8* __restrict__ frag, const float* __restrict__ b1,
                                                 const float* __restrict__ b2, const float* __restrict__ b3,

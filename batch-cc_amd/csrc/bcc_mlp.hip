@@ -1,5 +1,5 @@
 // Learned step ordering: obstacle encoder, pair features and a 3-layer BF16
-// MLP (96 -> 256 -> 256 -> 32 logits) trained and evaluated with hand-written
+// MLP (96 -> kHid -> kHid -> 32 logits, kHid = BCC_MLP_HIDDEN, 64 by default) trained and evaluated with hand-written
 // MFMA GEMMs and elementwise kernels.  The model only proposes an order in
 // which the exact checker visits the sample bins of a pair; verdicts never
 // depend on it.
@@ -444,7 +444,7 @@ unsigned blocks_for(long long n, int bs = 256)
     return (unsigned)((n + bs - 1) / bs);
 }
 
-// Split-K workspace: enough for 32 partial 256 x 256 tiles.
+// Split-K workspace: enough for 32 partial kHid x kHid tiles.
 constexpr long long kSplitWs = 32LL * kHid * kHid;
 
 // C = alpha op(A) op(B) + beta C.  Long, thin products (K >= 1024 and fewer

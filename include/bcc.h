@@ -276,12 +276,13 @@ int bcc_time_dense_ex(int robot, const bcc_scene *scene, const float *d_edges, i
 
 /*
  * Learned step ordering (no reference counterpart).  A bcc_model is a
- * 3-layer MLP (96 -> 256 -> 256 -> 32) over a 64-cell max-pooled obstacle
+ * 3-layer MLP (96 -> H -> H -> 32; H = 64, the build's BCC_MLP_HIDDEN, read
+ * back as the root of bcc_model_num_params() = H^2 + 130 H + 32) over a 64-cell max-pooled obstacle
  * occupancy code of the environment and the normalised edge endpoints,
  * predicting which of the 32 sample bins of a pair hold a collision.  BF16
  * weights and activations, FP32 accumulation on MFMA, FP32 master weights
  * and Adam state; trained on the GPU from labels the exact checker writes.
- * Parameter order: W1 [96][256], b1, W2 [256][256], b2, W3 [256][32], b3.
+ * Parameter order: W1 [96][H], b1, W2 [H][H], b2, W3 [H][32], b3.
  */
 typedef struct bcc_model bcc_model;
 int bcc_model_num_params(void);

@@ -1,12 +1,38 @@
 """CPU restatement of the learned-ordering MLP (test side): forward and
-backward of a 96-256-256-32 ReLU network with a mean binary cross-entropy
+backward of a 96-H-H-32 ReLU network (H, the hidden width, follows from the
+length of the parameter vector: the library's BCC_MLP_HIDDEN) with a mean binary cross-entropy
 over 32 bins, evaluated in float64 with BF16 rounding at exactly the points
 where the GPU stores BF16 (inputs, weights, hidden activations and the
 backpropagated errors fed to the next GEMM)."""
 import numpy as np
 
-IN, HID, OUT = 96, 256, 32
-SIZES = [("W1", (IN, HID)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)), ("W3", (HID, OUT)), ("b3", (OUT,))]
+IN, HID, OUT = 96, 64, 32   # HID: the library's default hidden width
+
+
+def sizes(hid=HID):
+    """Parameter blocks of the flat vector, in the library's order."""
+    return [("W1", (IN, hid)), ("b1", (hid,)), ("W2", (hid, hid)), ("b2", (hid,)), ("W3", (hid, OUT)), ("b3", (OUT,))]
+
+
+def num_params(hid=HID):
+    return sum(int(np.prod(s)) for _, s in sizes(hid))
+
+
+def hidden_of(n):
+    """Hidden width of a parameter vector of n entries (H^2 + (IN + OUT + 2) H + OUT = n)."""
+    b = IN + OUT + 2
+    h = int(round((-b + (b * b + 4 * (n - OUT)) ** 0.5) / 2))
+    if num_params(h) != n:
+        raise ValueError(f"{n} parameters fit no hidden width")
+    return h
+
+
+def sizes_of(n):
+    """Parameter blocks of a vector of n entries."""
+    return sizes(hidden_of(n))
+
+
+SIZES = sizes()
 
 
 def bf16(x):
@@ -23,7 +49,7 @@ def bf16_bits(x):
 
 def unpack(p):
     out, o = {}, 0
-    for name, shape in SIZES:
+    for name, shape in sizes_of(p.size):
         n = int(np.prod(shape))
         out[name] = p[o:o + n].reshape(shape)
         o += n
@@ -31,7 +57,7 @@ def unpack(p):
 
 
 def pack(d):
-    return np.concatenate([d[name].ravel() for name, _ in SIZES])
+    return np.concatenate([d[name].ravel() for name in ("W1", "b1", "W2", "b2", "W3", "b3")])
 
 
 def forward_backward(params, x, labels, rounding=True):

@@ -96,7 +96,7 @@ def test_gradients_match_cpu_reference(bcc, gpu):
     ref_loss, ref = mlp_ref.forward_backward(p, _f(xb), lab, rounding=True)
     assert abs(loss - ref_loss) < 1e-3 * max(1.0, abs(ref_loss))
     o = 0
-    for name, shape in mlp_ref.SIZES:
+    for name, shape in mlp_ref.sizes_of(p.size):
         n = int(np.prod(shape))
         a, b = g[o:o + n].astype(np.float64), ref[o:o + n]
         rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
@@ -260,7 +260,7 @@ def test_fused_inference_matches_layer_by_layer(bcc, gpu, robot):
     model = bcc.Model(robot, seed=4)
     p = model.params()
     o = 0
-    for name, shape in mlp_ref.SIZES:
+    for name, shape in mlp_ref.sizes_of(p.size):
         n = int(np.prod(shape))
         if name.startswith("b"):
             p[o:o + n] = rng.standard_normal(n).astype(np.float32) * 0.1

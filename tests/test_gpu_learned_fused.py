@@ -30,7 +30,7 @@ def _biased_model(bcc, robot, seed):
     p = model.params()
     rng = np.random.default_rng(seed)
     o = 0
-    for name, shape in mlp_ref.SIZES:
+    for name, shape in mlp_ref.sizes_of(p.size):
         n = int(np.prod(shape))
         if name.startswith("b"):
             p[o:o + n] = rng.standard_normal(n).astype(np.float32) * 0.3

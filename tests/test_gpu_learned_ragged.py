@@ -43,7 +43,7 @@ def test_pair_inference_matches_dense_inference(bcc, gpu, robot):
     model = bcc.Model(robot, seed=7)
     p = model.params()
     o = 0
-    for name, shape in mlp_ref.SIZES:
+    for name, shape in mlp_ref.sizes_of(p.size):
         n = int(np.prod(shape))
         if name.startswith("b"):
             p[o:o + n] = rng.standard_normal(n).astype(np.float32) * 0.1

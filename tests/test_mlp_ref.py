@@ -1,13 +1,24 @@
 """Pins the CPU MLP reference: its analytic gradients (without BF16
 rounding) against central finite differences of its loss in float64."""
 import numpy as np
+import pytest
 
 import mlp_ref
 
 
-def test_reference_gradients_match_finite_differences():
+def test_hidden_width_follows_the_parameter_count():
+    for h in (32, 64, 96, 128, 256):
+        n = mlp_ref.num_params(h)
+        assert n == 96 * h + h + h * h + h + h * 32 + 32
+        assert mlp_ref.hidden_of(n) == h
+    with pytest.raises(ValueError):
+        mlp_ref.hidden_of(mlp_ref.num_params(64) + 1)
+
+
+@pytest.mark.parametrize("hid", [mlp_ref.HID, 32])
+def test_reference_gradients_match_finite_differences(hid):
     rng = np.random.default_rng(0)
-    n = sum(int(np.prod(s)) for _, s in mlp_ref.SIZES)
+    n = mlp_ref.num_params(hid)
     p = (rng.standard_normal(n) * 0.08).astype(np.float64)
     x = rng.uniform(-1, 1, (8, mlp_ref.IN))
     labels = rng.integers(0, 2**32, 8, dtype=np.uint64).astype(np.uint32)
