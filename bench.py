@@ -58,8 +58,10 @@ RID = {"panda": 0, "fetch": 1, "baxter": 2}
 
 def mlp_flops(robot: str) -> int:
     """Algorithmic flops of one inference: 3 layers over the unpadded input
-    (64-cell encoding + start and end)."""
-    return 2 * ((64 + 2 * DIM[robot]) * 256 + 256 * 256 + 256 * 32)
+    (64-cell encoding + start and end), hidden width H of the built model."""
+    from batch_cc_amd import Model
+    h = Model.hidden()
+    return 2 * ((64 + 2 * DIM[robot]) * h + h * h + h * 32)
 
 
 def load_pkg():
