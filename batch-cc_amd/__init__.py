@@ -135,6 +135,8 @@ def lib():
     _sig(L, "bcc_debug_learned_passes", I, [I])
     _sig(L, "bcc_debug_learned_min_steps", I, [I])
     _sig(L, "bcc_debug_learned_locality", I, [I])
+    _sig(L, "bcc_debug_learned_small", I, [ctypes.c_int64])
+    _sig(L, "bcc_learned_small_pairs", ctypes.c_int64, [])
     _lib = L
     return L
 
@@ -837,6 +839,18 @@ def debug_learned_passes(passes: int):
 
 def debug_learned_min_steps(steps: int):
     _check(lib().bcc_debug_learned_min_steps(int(steps)))
+
+
+def debug_learned_small(max_pairs: int):
+    """Pair limit of the small-batch learned path (negative: the default)."""
+    _check(lib().bcc_debug_learned_small(int(max_pairs)))
+
+
+def learned_small_pairs() -> int:
+    """Dense learned launches of at most this many pairs (no stage, words or
+    stats asked for) infer every pair (no model: the invalid word) and run the
+    ordered dense kernel."""
+    return int(lib().bcc_learned_small_pairs())
 
 
 def debug_learned_locality(mode: int):

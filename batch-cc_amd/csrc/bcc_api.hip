@@ -844,6 +844,20 @@ struct LearnDefaults {
 constexpr LearnDefaults kLearnDefaults[4] = {{3, 64, 6, 1}, {6, 64, 6, 0}, {3, 64, 6, 0}, {3, 64, 6, 0}};
 std::atomic<int> g_learned_min_steps{0};   // > 0: forced for every launch (tuning hook)
 std::atomic<int> g_learned_locality{1};    // locality order: 0 never, 1 per-robot default, 2 always
+// Dense learned launches of at most this many pairs (no per-pair
+// diagnostics asked for) run the ordered dense kernel instead of the fused
+// persistent kernel, with every pair's inferred word (with a model) or the
+// invalid word (without one: the probe takes bins 8 and 24): at 1e5 pairs
+// the persistent kernel has about two 8-pair tiles per resident wave and
+// each wave's chain (probe, inference, probe, step loop on two half-waves)
+// outlasts the whole plain launch, while inference of every pair costs
+// ~0.04 ms (MI355X, 100 environments x 1000 panda edges: 0.36 ms with the
+// model and 0.31 without, against 0.54 fused and 0.39 plain).  Measured on
+// such batches of growing size, this route is the faster one up to 5.3e5
+// pairs (1.01 ms against 1.18 fused) and the fused kernel from 2e6 (2.73
+// against 3.44); at 1e6 the two tie (1.98, 2.02).
+constexpr long long kLearnSmallPairs = 3LL << 18;
+std::atomic<long long> g_learned_small{-1};   // >= 0: forced limit (tuning hook)
 
 int learn_min_steps(int robot, bool plan)
 {
@@ -1006,6 +1020,13 @@ int locality_order(int robot, const float* d_edges, int num_edges, hipStream_t s
     return BCC_OK;
 }
 
+__global__ void fill_words_kernel(uint32_t* __restrict__ p, long long n, uint32_t v)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = v;
+}
+
 // One dense learned step on stream s: the model's operands (fragments and
 // environment codes, two small kernels), then the fused learned kernel,
 // persistent over the dense tiles (a wave that ends a tile takes the next
@@ -1017,6 +1038,29 @@ int dense_learned(int robot, bcc_model* model, const bcc_scene* scene, const flo
 {
     if (num_edges == 0 || scene->num_envs == 0)
         return BCC_OK;
+    const long long small = g_learned_small.load();
+    const long long pairs = (long long)scene->num_envs * num_edges;
+    if (!d_stage && !d_words && !d_stats && pairs <= (small >= 0 ? small : kLearnSmallPairs)) {
+        // small batch: every pair's order word, then the ordered dense
+        // kernel (probe at the word's first bins, the workgroup's plain step
+        // loop for the misses)
+        uint32_t* d_order = nullptr;
+        if (hipError_t e = hipMallocAsync((void**)&d_order, (size_t)pairs * sizeof(uint32_t), s); e != hipSuccess)
+            return hip_fail(e, "small-batch order words");
+        int rc = BCC_OK;
+        if (model) {
+            rc = bcc_model_order_dense(model, scene, d_edges, num_edges, d_order, s);
+        } else {
+            hipLaunchKernelGGL(fill_words_kernel, dim3((unsigned)((pairs + 255) / 256)), dim3(256), 0, s, d_order,
+                               pairs, 0xffffffffu);
+            if (hipError_t e = hipGetLastError(); e != hipSuccess)
+                rc = hip_fail(e, "small-batch order words");
+        }
+        if (rc == BCC_OK)
+            rc = bcc_check_dense_ex(robot, scene, d_edges, num_edges, resolution, d_order, nullptr, d_results, s);
+        (void)hipFreeAsync(d_order, s);
+        return rc;
+    }
     bcc::MlpArgs m{};
     if (model) {
         if (int rc = bcc_detail::model_mlp_args(model, scene, s, &m))
@@ -1197,6 +1241,18 @@ int bcc_debug_learned_min_steps(int steps)
         return fail(BCC_ERR_INVALID, "learned minimum steps must be >= 0");
     g_learned_min_steps.store(steps);
     return BCC_OK;
+}
+
+int bcc_debug_learned_small(int64_t max_pairs)
+{
+    g_learned_small.store(max_pairs < 0 ? -1 : max_pairs);
+    return BCC_OK;
+}
+
+int64_t bcc_learned_small_pairs(void)
+{
+    const long long small = g_learned_small.load();
+    return small >= 0 ? small : kLearnSmallPairs;
 }
 
 int bcc_debug_learned_locality(int mode)

@@ -248,20 +248,27 @@ def learned_dense(bcc, W, a, robot, scene, sc, d_edges, E, M, plain_verdicts, pl
     st0 = bcc.check_dense_learned(robot, None, scene, d_edges, M, stats=True)
     none_bad += int((st0["verdicts"] != plain_verdicts).sum())
     mismatches += none_bad
-    log(f"learned step {step_ms:.3f} ms (fused kernel, {stage_counts(st['stats'])}); no network {none_ms:.3f} ms "
-        f"({stage_counts(st0['stats'])}); plain {plain_ms:.3f} ms; verdict mismatches {mismatches}")
+    small = E * M <= bcc.learned_small_pairs()
+    route = "small batch: inference of every pair, then the ordered dense kernel" if small else "fused kernel"
+    log(f"learned step {step_ms:.3f} ms ({route}; fused-kernel stages {stage_counts(st['stats'])}); no network "
+        f"{none_ms:.3f} ms ({stage_counts(st0['stats'])}); plain {plain_ms:.3f} ms; verdict mismatches {mismatches}")
     res = {
         "ms_per_step": step_ms,
         "speedup_vs_plain": plain_ms / step_ms,
         "verdict_mismatches_vs_plain": mismatches,
-        "kernel": "learned_kernel: first probe (bins 8, 24) -> in-kernel BF16 MFMA inference of the undecided "
-                  "pairs -> probe of their 2 top predicted bins -> plain steps, one persistent launch",
+        "kernel": ("fused_order_kernel (BF16 MFMA inference of every pair) -> ordered_dense_kernel (probe of each "
+                   f"pair's 2 top predicted bins, then plain steps): batches of at most {bcc.learned_small_pairs()} "
+                   "pairs" if small else
+                   "learned_kernel: first probe (bins 8, 24) -> in-kernel BF16 MFMA inference of the undecided "
+                   "pairs -> probe of their 2 top predicted bins -> plain steps, one persistent launch"),
         "pairs_by_stage": stage_counts(st["stats"]),
         "train": train,
         "no_network": {"ms_per_step": none_ms, "speedup_vs_plain": plain_ms / none_ms,
                        "verdict_mismatches_vs_plain": none_bad, "pairs_by_stage": stage_counts(st0["stats"]),
-                       "note": "the same fused kernel without a model: the second probe takes the fixed bins "
-                               "0 and 16"},
+                       "note": ("the ordered dense kernel with the invalid word for every pair: the probe takes "
+                                "the fixed bins 8 and 24" if small else
+                                "the same fused kernel without a model: the second probe takes the fixed bins 0 "
+                                "and 16")},
         "network_gain_vs_no_network": none_ms / step_ms,
     }
     if a.bound:

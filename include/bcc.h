@@ -363,7 +363,10 @@ int bcc_time_learned_dense(bcc_model *model, const bcc_scene *scene, const float
  * untouched); d_stats int32[5] += pairs decided by probes 1, 2 and 3,
  * inferred, sent to the step loop.  Asynchronous on `stream`; allocates the
  * model's environment codes as bcc_model_order_dense does and a tile
- * counter stream-ordered.
+ * counter stream-ordered.  Small batches (bcc_learned_small_pairs() pairs or
+ * fewer) that ask for none of the three optional outputs run instead as
+ * inference of every pair (model NULL: the invalid word) followed by
+ * bcc_check_dense_ex with those words.
  */
 int bcc_check_dense_learned(int robot, bcc_model *model, const bcc_scene *scene, const float *d_edges,
                             int num_edges, int resolution, uint8_t *d_results, uint8_t *d_stage, uint32_t *d_words,
@@ -402,6 +405,16 @@ int bcc_debug_learned_min_steps(int steps);
  * default); 0 = never, 2 = every launch of every robot.  Verdicts and
  * per-pair outputs never depend on it. */
 int bcc_debug_learned_locality(int mode);
+/* Tuning hook (no reference counterpart): dense learned launches of at most
+ * bcc_learned_small_pairs() pairs (3 x 2^18 by default; max_pairs >= 0 sets it,
+ * negative restores the default) that ask for no stage, words or statistics
+ * run as bcc_model_order_dense for every pair (without a model: the invalid
+ * word, so the probe takes bins 8 and 24) followed by bcc_check_dense_ex
+ * with those words (one stream-ordered words buffer); larger ones, and
+ * every launch that asks for those outputs, run the fused kernel.  Verdicts
+ * never depend on it. */
+int bcc_debug_learned_small(int64_t max_pairs);
+int64_t bcc_learned_small_pairs(void);
 
 /* test hook: C = alpha * op(A) op(B) + beta * C on the MFMA GEMM (BF16 in, FP32 out) */
 int bcc_debug_gemm(int ta, int tb, const uint16_t *d_a, int lda, const uint16_t *d_b, int ldb, float *d_c, int ldc,

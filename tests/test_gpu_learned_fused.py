@@ -345,3 +345,37 @@ def test_trained_model_gain_in_the_fused_step(bcc, gpu):
         assert np.array_equal(res["fixed bins"][0], res[name][0]), name
     assert res["trained"][1] > res["untrained"][1]
     assert res["trained"][1] > res["fixed bins"][1]
+
+
+@pytest.mark.parametrize("robot", ["panda", "fetch", "baxter", "xarm7"])
+def test_small_batches_infer_every_pair_then_run_the_ordered_kernel(bcc, oracle, gpu, robot):
+    """A dense learned launch of at most learned_small_pairs() pairs and no
+    per-pair diagnostics infers every pair (no model: the invalid word) and
+    runs the ordered dense kernel with those words; forcing the limit below
+    the batch sends the same launch to the fused kernel.  Both routes, with
+    and without a model, through the check and the timing entry points, give
+    the oracle's verdicts."""
+    from batch_cc_amd import workloads as W
+    rid = bcc.ROBOTS[robot]
+    rng = np.random.default_rng(71 + rid)
+    envs, edges = _scene(W, rng, robot, 9, 70)
+    sc = bcc.PackedScene.from_envs(envs)
+    M, E = edges.shape[0], sc.num_envs
+    assert M * E <= bcc.learned_small_pairs()
+    exp = oracle.check_dense(rid, sc, edges).astype(bool)
+    with bcc.Scene(sc) as scene:
+        de = bcc.DeviceBuffer.from_array(edges)
+        model = _biased_model(bcc, robot, 5)
+        d_out = bcc.DeviceBuffer(M * E)
+        try:
+            for limit in (-1, M * E, M * E - 1, 0):
+                bcc.debug_learned_small(limit)
+                for m in (model, None):
+                    r = bcc.check_dense_learned(robot, m, scene, de, M)
+                    assert np.array_equal(r["verdicts"], exp), (limit, m is None)
+                    d_out.zero()
+                    bcc.time_dense_learned(robot, m, scene, de, M, d_out, 1, 2)
+                    assert np.array_equal(d_out.to_array(np.uint8, (M, E)).astype(bool), exp), (limit, m is None)
+        finally:
+            bcc.debug_learned_small(-1)
+            model.close()
